@@ -1,0 +1,321 @@
+"""graphone_amd: GraphOne's graph-computation hot path on MI355X (gfx950).
+
+Host-side Python view of the C-ABI in include/graphone_hip.h. Device memory, kernels and engine loops
+live in libgraphone_hip.so; this module only moves arrays and forwards calls. Every compute entry point
+raises if the HIP library is missing: there is no CPU fallback.
+
+Mapping to the reference API (paths relative to the reference root):
+  DeviceArray  ~ DenseVec / Buffer           include/GraphOne/vec/dense.h:7-42, base/buffer.h:12-202
+  HostGraph    ~ GraphLoader + EdgeCache      include/GraphOne/loader/graph_loader.h, edge_cache.h
+  Graph        ~ graph::build<arch, views>    include/GraphOne/graph/builder.h:9-46
+  Graph.bfs/sssp/pagerank/ppr ~ Run<Functor>(comp, frontier) with the apps' functors
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (G1_ENGINE_ADVANCE, G1_ENGINE_BLAS_CSRVEC, G1_ENGINE_BLAS_MERGE, G1_ENGINE_DIROPT, G1_INF,
+                   G1_PR_L1_F64, G1_PR_MAX_ABS, G1Error, HostCSR, RunStats, call, lib)
+
+INF = G1_INF
+
+ENGINES = {
+    "advance": G1_ENGINE_ADVANCE,
+    "merge": G1_ENGINE_BLAS_MERGE,
+    "csrvec": G1_ENGINE_BLAS_CSRVEC,
+    "diropt": G1_ENGINE_DIROPT,
+}
+
+
+def _engine(e) -> int:
+    if isinstance(e, str):
+        return ENGINES[e]
+    return int(e)
+
+
+# ----------------------------------------------------------------------------------------- device memory
+
+class DeviceArray:
+    """A typed device (HBM) array owned by Python, allocated through g1_malloc."""
+
+    def __init__(self, count: int, dtype):
+        self.count = int(count)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.count * self.dtype.itemsize
+        p = C.c_void_p()
+        call("g1_malloc", C.byref(p), C.c_size_t(self.nbytes))
+        self._ptr = p.value or 0
+
+    @property
+    def ptr(self) -> int:
+        return self._ptr
+
+    @classmethod
+    def from_numpy(cls, arr: np.ndarray) -> "DeviceArray":
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.size, arr.dtype)
+        if d.nbytes:
+            call("g1_memcpy_h2d", C.c_void_p(d.ptr), arr.ctypes.data_as(C.c_void_p), C.c_size_t(d.nbytes), None)
+        return d
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.count, dtype=self.dtype)
+        if self.nbytes:
+            call("g1_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), C.c_size_t(self.nbytes),
+                 None)
+        return out
+
+    def copy_from(self, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.size == self.count
+        if self.nbytes:
+            call("g1_memcpy_h2d", C.c_void_p(self.ptr), arr.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes),
+                 None)
+
+    def fill(self, value) -> None:
+        if self.dtype == np.uint32:
+            call("g1_fill_u32", C.c_void_p(self.ptr), C.c_uint32(int(value)), C.c_size_t(self.count), None)
+        elif self.dtype == np.float32:
+            call("g1_fill_f32", C.c_void_p(self.ptr), C.c_float(float(value)), C.c_size_t(self.count), None)
+        else:
+            raise TypeError(f"fill: unsupported dtype {self.dtype}")
+
+    def free(self) -> None:
+        if self._ptr:
+            lib().g1_free(C.c_void_p(self._ptr))
+            self._ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize(stream=None) -> None:
+    call("g1_stream_synchronize", stream)
+
+
+def device_info() -> dict:
+    count, cus, hbm = C.c_int32(), C.c_int32(), C.c_uint64()
+    name = C.create_string_buffer(64)
+    call("g1_device_info", C.byref(count), C.byref(cus), C.byref(hbm), name, C.c_size_t(64))
+    return {"devices": count.value, "cus": cus.value, "hbm_bytes": hbm.value, "arch": name.value.decode()}
+
+
+# ----------------------------------------------------------------------------------------- host graphs
+
+class HostGraph:
+    """A host CSR (row_offsets u32[n+1], col_indices u32[m], optional u32 weights)."""
+
+    def __init__(self, csr: HostCSR, owned_by_lib: bool, keep=()):
+        self._csr = csr
+        self._owned = owned_by_lib
+        self._keep = keep  # numpy arrays backing a borrowed CSR
+
+    # constructors ------------------------------------------------------------------------
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1, undirected: bool = False) -> "HostGraph":
+        csr = HostCSR()
+        call("g1_gen_rmat", scale, edge_factor, a, b, c, C.c_uint64(seed), int(undirected), C.byref(csr))
+        return cls(csr, True)
+
+    @classmethod
+    def grid(cls, rows: int, cols: int, p_keep: float = 0.6, seed: int = 1) -> "HostGraph":
+        csr = HostCSR()
+        call("g1_gen_grid", rows, cols, C.c_double(p_keep), C.c_uint64(seed), C.byref(csr))
+        return cls(csr, True)
+
+    @classmethod
+    def from_edges(cls, n: int, src, dst, weights=None, directed: bool = True, keep_self: bool = False,
+                   keep_dup: bool = False) -> "HostGraph":
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.uint32)
+        csr = HostCSR()
+        call("g1_csr_from_edges", n, C.c_uint64(src.size), src.ctypes.data_as(C.c_void_p),
+             dst.ctypes.data_as(C.c_void_p), None if w is None else w.ctypes.data_as(C.c_void_p),
+             int(directed), int(keep_self), int(keep_dup), C.byref(csr))
+        return cls(csr, True)
+
+    @classmethod
+    def from_csr(cls, row_offsets, col_indices, weights=None) -> "HostGraph":
+        ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)
+        ci = np.ascontiguousarray(col_indices, dtype=np.uint32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.uint32)
+        csr = HostCSR()
+        csr.n = ro.size - 1
+        csr.m = ci.size
+        csr.row_offsets = ro.ctypes.data_as(C.POINTER(C.c_uint32))
+        csr.col_indices = ci.ctypes.data_as(C.POINTER(C.c_uint32))
+        csr.weights = w.ctypes.data_as(C.POINTER(C.c_uint32)) if w is not None else None
+        return cls(csr, False, keep=(ro, ci, w))
+
+    # views -------------------------------------------------------------------------------
+    @property
+    def n(self) -> int:
+        return self._csr.n
+
+    @property
+    def m(self) -> int:
+        return self._csr.m
+
+    @property
+    def row_offsets(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._csr.row_offsets, shape=(self.n + 1,))
+
+    @property
+    def col_indices(self) -> np.ndarray:
+        if self.m == 0:
+            return np.zeros(0, dtype=np.uint32)
+        return np.ctypeslib.as_array(self._csr.col_indices, shape=(self.m,))
+
+    @property
+    def weights(self):
+        if not self._csr.weights:
+            return None
+        if self.m == 0:
+            return np.zeros(0, dtype=np.uint32)
+        return np.ctypeslib.as_array(self._csr.weights, shape=(self.m,))
+
+    def out_degrees(self) -> np.ndarray:
+        return np.diff(self.row_offsets.astype(np.int64)).astype(np.uint32)
+
+    def assign_weights(self, lo: int, hi: int, seed: int = 2, symmetric: bool = False) -> "HostGraph":
+        if not self._owned:
+            raise ValueError("assign_weights needs a library-owned graph (use a generator or from_edges)")
+        call("g1_assign_weights_u32", C.byref(self._csr), lo, hi, C.c_uint64(seed), int(symmetric))
+        return self
+
+    def transpose(self) -> "HostGraph":
+        out = HostCSR()
+        call("g1_host_csr_transpose", C.byref(self._csr), C.byref(out))
+        return HostGraph(out, True)
+
+    def __del__(self):
+        try:
+            if self._owned:
+                lib().g1_host_csr_free(C.byref(self._csr))
+                self._owned = False
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------------------------------- device graph
+
+@dataclass
+class RunResult:
+    values: np.ndarray
+    stats: dict
+
+
+class Graph:
+    """A device-resident graph (g1_graph) with its transpose, and the whole-run engines."""
+
+    def __init__(self, host: HostGraph, transpose: bool = True, symmetric: bool = False):
+        flags = (_lib.G1_GRAPH_TRANSPOSE if transpose else 0) | (_lib.G1_GRAPH_SYMMETRIC if symmetric else 0)
+        h = C.c_void_p()
+        call("g1_graph_create", C.byref(h), C.byref(host._csr), flags)
+        self._h = h
+        self.n = host.n
+        self.m = host.m
+        self.weighted = host.weights is not None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        call("g1_graph_info", self._h, None, None, C.byref(b))
+        return b.value
+
+    def csr_dev(self, transpose: bool = False):
+        ro, ci, w = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        call("g1_graph_csr_dev", self._h, int(transpose), C.byref(ro), C.byref(ci), C.byref(w))
+        return ro.value, ci.value, w.value
+
+    def _run_u32(self, fn: str, src: int, engine, out: DeviceArray | None, stream):
+        d = out if out is not None else DeviceArray(self.n, np.uint32)
+        st = RunStats()
+        call(fn, self._h, C.c_uint32(src), C.c_void_p(d.ptr), _engine(engine), C.byref(st), stream)
+        return d, st.as_dict()
+
+    def bfs(self, src: int, engine="advance", out: DeviceArray | None = None, stream=None,
+            to_host: bool = True):
+        d, st = self._run_u32("g1_bfs", src, engine, out, stream)
+        return RunResult(d.numpy() if to_host else d, st)
+
+    def sssp(self, src: int, engine="advance", out: DeviceArray | None = None, stream=None,
+             to_host: bool = True):
+        d, st = self._run_u32("g1_sssp", src, engine, out, stream)
+        return RunResult(d.numpy() if to_host else d, st)
+
+    def pagerank(self, alpha: float = 0.85, max_iter: int = 50, tol: float = 1e-7, mode: str = "l1",
+                 engine="merge", out: DeviceArray | None = None, stream=None, to_host: bool = True):
+        r = out if out is not None else DeviceArray(self.n, np.float32)
+        st = RunStats()
+        cm = G1_PR_L1_F64 if mode == "l1" else G1_PR_MAX_ABS
+        call("g1_pagerank", self._h, C.c_float(alpha), C.c_uint32(max_iter), C.c_double(tol), cm, _engine(engine),
+             C.c_void_p(r.ptr), C.byref(st), stream)
+        return RunResult(r.numpy() if to_host else r, st.as_dict())
+
+    def ppr(self, seed: int, restart: float = 0.15, max_iter: int = 50, tol: float = 1e-7,
+            out: DeviceArray | None = None, stream=None, to_host: bool = True):
+        r = out if out is not None else DeviceArray(self.n, np.float32)
+        st = RunStats()
+        call("g1_ppr", self._h, C.c_uint32(seed), C.c_float(restart), C.c_uint32(max_iter), C.c_double(tol),
+             C.c_void_p(r.ptr), C.byref(st), stream)
+        return RunResult(r.numpy() if to_host else r, st.as_dict())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().g1_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------------------------------- SpMV
+
+def spmv(kind: str, semiring: str, row_offsets: DeviceArray, col_indices: DeviceArray, values, x: DeviceArray,
+         y: DeviceArray, n_rows: int, n_cols: int, nnz: int, workspace: DeviceArray | None = None,
+         lanes: int = 0, stream=None) -> None:
+    """y = A (x) x on a CSR matrix already in device memory.
+
+    kind: 'merge' (merge-path; workspace from merge_workspace() and merge_partition()) or 'csrvec'.
+    semiring: 'minplus_u32' or 'plustimes_f32'. values may be None (unit weights).
+    """
+    vp = None if values is None else C.c_void_p(values.ptr)
+    args = (C.c_uint32(n_rows), C.c_uint32(n_cols), C.c_uint32(nnz), C.c_void_p(row_offsets.ptr),
+            C.c_void_p(col_indices.ptr), vp, C.c_void_p(x.ptr), C.c_void_p(y.ptr))
+    if kind == "merge":
+        assert workspace is not None
+        call(f"g1_spmv_merge_{semiring}", *args, C.c_void_p(workspace.ptr), C.c_size_t(workspace.nbytes), stream)
+    elif kind == "csrvec":
+        call(f"g1_spmv_csrvec_{semiring}", *args, C.c_uint32(lanes), stream)
+    else:
+        raise ValueError(kind)
+
+
+def merge_workspace(n_rows: int, nnz: int, row_offsets: DeviceArray, stream=None) -> DeviceArray:
+    nbytes = lib().g1_spmv_merge_workspace_bytes(n_rows, nnz)
+    ws = DeviceArray((nbytes + 3) // 4, np.uint32)
+    call("g1_spmv_merge_partition", C.c_uint32(n_rows), C.c_uint32(nnz), C.c_void_p(row_offsets.ptr),
+         C.c_void_p(ws.ptr), C.c_size_t(ws.nbytes), stream)
+    return ws
+
+
+__all__ = [
+    "INF", "ENGINES", "DeviceArray", "HostGraph", "Graph", "RunResult", "G1Error", "spmv", "merge_workspace",
+    "synchronize", "device_info",
+]

@@ -1,0 +1,159 @@
+"""ctypes binding of libgraphone_hip.so (declared in include/graphone_hip.h).
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgraphone_hip.so")
+
+G1_OK = 0
+G1_ERR_INVALID_ARG = 1
+G1_ERR_HIP = 2
+G1_ERR_OUT_OF_MEMORY = 3
+G1_ERR_BUDGET = 4
+G1_ERR_NOT_FOUND = 5
+G1_ERR_BUSY = 6
+G1_ERR_CANCELLED = 7
+G1_ERR_INTERNAL = 8
+G1_ERR_NOT_CONVERGED = 9
+G1_INF = 0xFFFFFFFF
+
+G1_GRAPH_TRANSPOSE = 1
+G1_GRAPH_SYMMETRIC = 2
+
+G1_ENGINE_ADVANCE = 0
+G1_ENGINE_BLAS_MERGE = 1
+G1_ENGINE_BLAS_CSRVEC = 2
+G1_ENGINE_DIROPT = 3
+
+G1_PR_L1_F64 = 0
+G1_PR_MAX_ABS = 1
+
+
+class G1Error(RuntimeError):
+    def __init__(self, status: int, where: str, message: str):
+        super().__init__(f"{where}: status {status} ({message})")
+        self.status = status
+
+
+class HostCSR(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32),
+        ("m", C.c_uint32),
+        ("row_offsets", C.POINTER(C.c_uint32)),
+        ("col_indices", C.POINTER(C.c_uint32)),
+        ("weights", C.POINTER(C.c_uint32)),
+    ]
+
+
+class RunStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_uint32),
+        ("push_iterations", C.c_uint32),
+        ("pull_iterations", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("frontier_sum", C.c_uint64),
+        ("edges_sum", C.c_uint64),
+        ("reached", C.c_uint64),
+        ("reached_edges", C.c_uint64),
+        ("elapsed_ms", C.c_double),
+        ("last_delta", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_i32 = C.c_int32
+_vp = C.c_void_p
+_st = C.c_int32
+_pu32 = C.POINTER(C.c_uint32)
+
+_SIGNATURES = {
+    "g1_status_string": (C.c_char_p, [_st]),
+    "g1_last_error_message": (C.c_char_p, []),
+    "g1_abi_version": (_i32, []),
+    "g1_device_info": (_st, [C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64), C.c_char_p, C.c_size_t]),
+    "g1_malloc": (_st, [C.POINTER(_vp), C.c_size_t]),
+    "g1_free": (_st, [_vp]),
+    "g1_memcpy_h2d": (_st, [_vp, _vp, C.c_size_t, _vp]),
+    "g1_memcpy_d2h": (_st, [_vp, _vp, C.c_size_t, _vp]),
+    "g1_memcpy_d2d": (_st, [_vp, _vp, C.c_size_t, _vp]),
+    "g1_fill_u32": (_st, [_vp, _u32, C.c_size_t, _vp]),
+    "g1_fill_f32": (_st, [_vp, C.c_float, C.c_size_t, _vp]),
+    "g1_stream_create": (_st, [C.POINTER(_vp)]),
+    "g1_stream_destroy": (_st, [_vp]),
+    "g1_stream_synchronize": (_st, [_vp]),
+    "g1_event_create": (_st, [C.POINTER(_vp)]),
+    "g1_event_destroy": (_st, [_vp]),
+    "g1_event_record": (_st, [_vp, _vp]),
+    "g1_event_synchronize": (_st, [_vp]),
+    "g1_event_elapsed_ms": (_st, [C.POINTER(C.c_float), _vp, _vp]),
+    "g1_spmv_merge_workspace_bytes": (C.c_size_t, [_u32, _u32]),
+    "g1_spmv_merge_partition": (_st, [_u32, _u32, _vp, _vp, C.c_size_t, _vp]),
+    "g1_spmv_merge_minplus_u32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_spmv_merge_plustimes_f32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_spmv_csrvec_lanes": (_u32, [_u32, _u32]),
+    "g1_spmv_csrvec_minplus_u32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
+    "g1_spmv_csrvec_plustimes_f32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
+    "g1_gen_rmat": (_st, [_u32, _u32, C.c_double, C.c_double, C.c_double, _u64, _i32, C.POINTER(HostCSR)]),
+    "g1_gen_grid": (_st, [_u32, _u32, C.c_double, _u64, C.POINTER(HostCSR)]),
+    "g1_csr_from_edges": (_st, [_u32, _u64, _vp, _vp, _vp, _i32, _i32, _i32, C.POINTER(HostCSR)]),
+    "g1_assign_weights_u32": (_st, [C.POINTER(HostCSR), _u32, _u32, _u64, _i32]),
+    "g1_host_csr_transpose": (_st, [C.POINTER(HostCSR), C.POINTER(HostCSR)]),
+    "g1_host_csr_free": (None, [C.POINTER(HostCSR)]),
+    "g1_graph_create": (_st, [C.POINTER(_vp), C.POINTER(HostCSR), _u32]),
+    "g1_graph_destroy": (_st, [_vp]),
+    "g1_graph_info": (_st, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u64)]),
+    "g1_graph_csr_dev": (_st, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "g1_bfs": (_st, [_vp, _u32, _vp, _i32, C.POINTER(RunStats), _vp]),
+    "g1_sssp": (_st, [_vp, _u32, _vp, _i32, C.POINTER(RunStats), _vp]),
+    "g1_pagerank": (_st, [_vp, C.c_float, _u32, C.c_double, _i32, _i32, _vp, C.POINTER(RunStats), _vp]),
+    "g1_ppr": (_st, [_vp, _u32, C.c_float, _u32, C.c_double, _vp, C.POINTER(RunStats), _vp]),
+}
+
+
+def declared_symbols() -> list[str]:
+    """Names of every function the C header declares (parsed from include/graphone_hip.h)."""
+    import re
+
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "graphone_hip.h")
+    text = open(hdr).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(g1_[a-z0-9_]+)\s*\(", text)))
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        handle = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(status: int, where: str) -> None:
+    if status != G1_OK:
+        L = lib()
+        msg = L.g1_last_error_message().decode(errors="replace")
+        raise G1Error(status, where, msg or L.g1_status_string(status).decode())
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)

@@ -1,0 +1,390 @@
+// Whole-algorithm runs of the C-ABI: the Run<F>(comp, frontier) loop (include/GraphOne/base/run.h:8-39)
+// with the functor fixed to BFS, SSSP, PageRank or personalized PageRank, over the push engine
+// (engine/advance_gc.h) or the pull SpMV engine (engine/blas_gc.h).
+//
+// Host loop: one launch per engine phase, then an 8-byte readback of the device frontier counter
+// (the convergence test the reference does with frontier.IsConvergent()/a D2H copy per iteration).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "g1/apply.hpp"
+#include "g1/push.hpp"
+#include "g1/spmv_merge.hpp"
+#include "graph.hpp"
+#include "spmv_launch.hpp"
+
+namespace g1 {
+namespace {
+
+constexpr uint32_t kMaxIterations = 1u << 20;
+
+uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
+    uint64_t b = (items + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return static_cast<uint32_t>(b);
+}
+
+uint32_t persistent_cap() { return static_cast<uint32_t>(device_cu_count()) * 8; }
+
+g1_status read_ctr(g1_graph* g, hipStream_t s, uint32_t first, uint32_t count) {
+    G1_HIP(hipMemcpyAsync(g->h_ctr + first, g->ctr + first, count * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    G1_HIP(hipStreamSynchronize(s));
+    return G1_OK;
+}
+
+__global__ void set_u32_kernel(uint32_t* p, uint32_t idx, uint32_t value) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[idx] = value;
+}
+
+__global__ void clear_u64_kernel(unsigned long long* p, uint32_t count) {
+    if (threadIdx.x < count && blockIdx.x == 0) p[threadIdx.x] = 0ull;
+}
+
+g1_status finish_stats(g1_graph* g, const uint32_t* d, g1_run_stats* st, hipStream_t s) {
+    clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr + 4, 2);
+    G1_LAUNCHED();
+    reached_stats_kernel<<<grid_for(g->n, 256, persistent_cap()), 256, 0, s>>>(g->n, d, g->row_off, g->ctr + 4);
+    G1_LAUNCHED();
+    G1_TRY(read_ctr(g, s, 4, 2));
+    st->reached = g->h_ctr[4];
+    st->reached_edges = g->h_ctr[5];
+    return G1_OK;
+}
+
+g1_status elapsed(g1_graph* g, g1_run_stats* st) {
+    G1_HIP(hipEventSynchronize(g->ev[1]));
+    float ms = 0.0f;
+    G1_HIP(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+    st->elapsed_ms = ms;
+    return G1_OK;
+}
+
+// ---------------------------------------------------------------- push engines (BFS / SSSP)
+
+PushQueues make_queues(g1_graph* g, uint32_t k) {
+    PushQueues pq;
+    pq.q_in = g->q[k & 1];
+    pq.qoff_in = g->qoff[k & 1];
+    pq.cnt_in = g->ctr + (k % 3);
+    pq.q_out = g->q[(k + 1) & 1];
+    pq.qoff_out = g->qoff[(k + 1) & 1];
+    pq.cnt_out = g->ctr + ((k + 1) % 3);
+    pq.cnt_clear = g->ctr + ((k + 2) % 3);
+    pq.activated = g->ctr + 3;
+    return pq;
+}
+
+g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, g1_run_stats* st, hipStream_t s) {
+    G1_TRY(ensure_push_work(g));
+    if (diropt) G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "g1_bfs: direction-optimizing BFS needs the transpose");
+    const uint32_t n = g->n;
+    const uint32_t words = (n + 31) / 32;
+    G1_TRY(g1_fill_u32(depth, kInf, n, s));
+    set_u32_kernel<<<1, 64, 0, s>>>(depth, src, 0u);
+    G1_LAUNCHED();
+    push_seed_kernel<<<1, 64, 0, s>>>(g->row_off, src, g->q[0], g->qoff[0], g->ctr);
+    G1_LAUNCHED();
+    clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr + 3, 1);
+    G1_LAUNCHED();
+    int fbi = 0;  // bitmap holding the current frontier
+    if (diropt) {
+        G1_TRY(g1_fill_u32(g->fb[0], 0u, words + 1, s));
+        set_u32_kernel<<<1, 64, 0, s>>>(g->fb[0], src >> 5, 1u << (src & 31u));
+        G1_LAUNCHED();
+    }
+    G1_TRY(read_ctr(g, s, 0, 1));
+    G1_HIP(hipEventRecord(g->ev[0], s));
+    unsigned long long cur = g->h_ctr[0];
+    uint64_t explored = 0, activated_prev = 0, frontier_sum = 1;
+    bool pull = false;
+    uint32_t it = 0;
+    const uint32_t cap = persistent_cap();
+    uint64_t nf_active = 1;  // vertices in the current frontier (including those without out-edges)
+    while (static_cast<uint32_t>(cur) != 0 && it < kMaxIterations) {
+        const uint32_t nf = static_cast<uint32_t>(cur);
+        const uint32_t E = static_cast<uint32_t>(cur >> 32);
+        PushQueues pq = make_queues(g, it);
+        if (diropt) {
+            // Beamer's switch: pull when the frontier's out-edges exceed 1/14 of the edges still
+            // unexplored; push again when the frontier falls below n/24 vertices.
+            const uint64_t unexplored = g->m > explored + E ? g->m - explored - E : 0;
+            if (!pull && static_cast<uint64_t>(E) * 14 > unexplored) pull = true;
+            else if (pull && nf_active * 24 < n) pull = false;
+        }
+        if (pull) {
+            bfs_pull_kernel<<<grid_for(n, kPushThreads, cap), kPushThreads, 0, s>>>(
+                n, g->t_row_off, g->t_col, g->row_off, depth, g->fb[fbi], g->fb[fbi ^ 1], it + 1, pq);
+            G1_LAUNCHED();
+            ++st->pull_iterations;
+        } else {
+            if (diropt) G1_TRY(g1_fill_u32(g->fb[fbi ^ 1], 0u, words + 1, s));
+            BfsPush pol{depth, diropt ? g->fb[fbi ^ 1] : nullptr, it + 1};
+            push_advance_kernel<BfsPush><<<grid_for(E, kPushTile, cap), kPushThreads, 0, s>>>(
+                g->row_off, g->col, nullptr, pq, pol);
+            G1_LAUNCHED();
+            ++st->push_iterations;
+            st->edges_sum += E;
+        }
+        fbi ^= 1;
+        explored += E;
+        ++it;
+        G1_HIP(hipMemcpyAsync(g->h_ctr, g->ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        G1_HIP(hipStreamSynchronize(s));
+        cur = g->h_ctr[it % 3];
+        const uint64_t act = g->h_ctr[3];
+        nf_active = act - activated_prev;
+        activated_prev = act;
+        frontier_sum += nf_active;
+        (void)nf;
+    }
+    G1_HIP(hipEventRecord(g->ev[1], s));
+    G1_TRY(elapsed(g, st));
+    st->iterations = it;
+    st->frontier_sum = frontier_sum;
+    return finish_stats(g, depth, st, s);
+}
+
+g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s) {
+    G1_TRY(ensure_push_work(g));
+    G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
+    const uint32_t n = g->n;
+    G1_TRY(g1_fill_u32(dist, kInf, n, s));
+    G1_TRY(g1_fill_u32(g->stamp, 0u, n, s));
+    set_u32_kernel<<<1, 64, 0, s>>>(dist, src, 0u);
+    G1_LAUNCHED();
+    push_seed_kernel<<<1, 64, 0, s>>>(g->row_off, src, g->q[0], g->qoff[0], g->ctr);
+    G1_LAUNCHED();
+    clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr + 3, 1);
+    G1_LAUNCHED();
+    G1_TRY(read_ctr(g, s, 0, 1));
+    G1_HIP(hipEventRecord(g->ev[0], s));
+    unsigned long long cur = g->h_ctr[0];
+    uint32_t it = 0;
+    const uint32_t cap = persistent_cap();
+    while (static_cast<uint32_t>(cur) != 0 && it < kMaxIterations) {
+        const uint32_t E = static_cast<uint32_t>(cur >> 32);
+        PushQueues pq = make_queues(g, it);
+        SsspPush pol{dist, g->stamp, it + 1};
+        push_advance_kernel<SsspPush><<<grid_for(E, kPushTile, cap), kPushThreads, 0, s>>>(
+            g->row_off, g->col, g->w, pq, pol);
+        G1_LAUNCHED();
+        st->edges_sum += E;
+        ++it;
+        G1_HIP(hipMemcpyAsync(g->h_ctr, g->ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        G1_HIP(hipStreamSynchronize(s));
+        cur = g->h_ctr[it % 3];
+    }
+    G1_HIP(hipEventRecord(g->ev[1], s));
+    G1_TRY(elapsed(g, st));
+    st->iterations = it;
+    st->push_iterations = it;
+    st->frontier_sum = 1 + g->h_ctr[3];
+    return finish_stats(g, dist, st, s);
+}
+
+// ---------------------------------------------------------------- pull engines (SpMV)
+
+g1_status run_minplus_pull(g1_graph* g, uint32_t src, uint32_t* d, bool weighted, int engine, g1_run_stats* st,
+                           hipStream_t s) {
+    G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "pull engine needs the graph's transpose");
+    if (weighted) G1_CHECK(g->t_w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
+    G1_TRY(ensure_pull_work(g, false));
+    const uint32_t n = g->n;
+    const uint32_t cap = persistent_cap();
+    minplus_seed_kernel<<<grid_for(n, 256, cap), 256, 0, s>>>(n, src, d, g->xu);
+    G1_LAUNCHED();
+    clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr + 6, 1);
+    G1_LAUNCHED();
+    const uint32_t* vals = weighted ? g->t_w : nullptr;
+    const uint32_t lanes = csrvec_lanes(n, g->m);
+    G1_HIP(hipEventRecord(g->ev[0], s));
+    uint64_t prev = 0, frontier_sum = 1;
+    uint32_t it = 0;
+    for (;;) {
+        if (engine == G1_ENGINE_BLAS_CSRVEC) {
+            G1_TRY(launch_spmv_csrvec<MinPlusU32>(n, g->t_row_off, g->t_col, vals, g->xu, g->yu, lanes, s));
+        } else {
+            G1_TRY(launch_spmv_merge<MinPlusU32>(n, g->m, g->t_row_off, g->t_col, vals, g->xu, g->yu, g->t_ws, s));
+        }
+        apply_minplus_kernel<<<grid_for(n, 256, cap), 256, 0, s>>>(n, g->yu, d, g->xu, g->ctr + 6);
+        G1_LAUNCHED();
+        ++it;
+        st->edges_sum += g->m;
+        G1_TRY(read_ctr(g, s, 6, 1));
+        const uint64_t c = g->h_ctr[6];
+        const uint64_t delta = c - prev;
+        prev = c;
+        if (delta == 0 || it >= kMaxIterations) break;
+        frontier_sum += delta;
+    }
+    G1_HIP(hipEventRecord(g->ev[1], s));
+    G1_TRY(elapsed(g, st));
+    st->iterations = it;
+    st->pull_iterations = it;
+    st->frontier_sum = frontier_sum;
+    return finish_stats(g, d, st, s);
+}
+
+// ---------------------------------------------------------------- PageRank / personalized PageRank
+
+g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, uint32_t max_iter, double tol,
+                       int mode, int engine, float* out, g1_run_stats* st, hipStream_t s) {
+    G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "PageRank needs the graph's transpose");
+    G1_TRY(ensure_pull_work(g, true));
+    const uint32_t n = g->n;
+    const uint32_t blocks = grid_for(n, 256, std::min<uint32_t>(kPartials, persistent_cap()));
+    const float* vals = g->t_wf;  // null for an unweighted graph: every weight is 1
+    const uint32_t lanes = csrvec_lanes(n, g->m);
+    const bool personalized = seed != kInf;
+    // alpha of the propagation: PageRank damping, or 1 - restart for the personalized variant.
+    const float prop = personalized ? 1.0f - restart : alpha;
+    const float r0 = n ? 1.0f / static_cast<float>(n) : 0.0f;
+    pr_init_kernel<<<grid_for(n, 256, persistent_cap()), 256, 0, s>>>(n, g->row_off, g->w, prop, g->iw, g->r[0],
+                                                                      r0, g->xf, seed);
+    G1_LAUNCHED();
+    const float base_all = personalized ? 0.0f : static_cast<float>((1.0 - alpha) / static_cast<double>(n));
+    const float base_seed = personalized ? restart : base_all;
+    G1_HIP(hipEventRecord(g->ev[0], s));
+    uint32_t it = 0;
+    int cur = 0;  // r[cur] holds the latest iterate
+    double delta = 0.0;
+    if (mode == G1_PR_MAX_ABS) {
+        // The reference's loop (apps/pagerank_blas.cu:63-78,93-123): ranks[1] starts at 0; before each
+        // iteration stop if max|ranks[idx^1] - ranks[idx]| <= tol; the iteration writes ranks[idx]
+        // in place (apply_each), then idx ^= 1. The result is ranks[idx] at exit.
+        G1_TRY(g1_fill_f32(g->r[1], 0.0f, n, s));
+        int idx = 0;
+        for (;;) {
+            max_abs_diff_kernel<<<blocks, 256, 0, s>>>(n, g->r[idx ^ 1], g->r[idx], g->partials);
+            G1_LAUNCHED();
+            reduce_partials_kernel<1><<<1, 256, 0, s>>>(g->partials, blocks, g->partials + kPartials);
+            G1_LAUNCHED();
+            G1_HIP(hipMemcpyAsync(g->h_scalar, g->partials + kPartials, sizeof(double), hipMemcpyDeviceToHost, s));
+            G1_HIP(hipStreamSynchronize(s));
+            delta = g->h_scalar[0];
+            if (delta <= tol || it >= max_iter) break;
+            if (engine == G1_ENGINE_BLAS_CSRVEC)
+                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, g->t_row_off, g->t_col, vals, g->xf, g->yf, lanes, s));
+            else
+                G1_TRY(launch_spmv_merge<PlusTimesKahanF32>(n, g->m, g->t_row_off, g->t_col, vals, g->xf, g->yf,
+                                                            g->t_ws, s));
+            // In place on ranks[idx] as apply_each does; the partials of this call are unused.
+            pr_apply_kernel<1><<<blocks, 256, 0, s>>>(n, g->yf, base_all, seed, base_seed, g->iw, g->r[idx ^ 1],
+                                                      g->r[idx], g->xf, g->partials);
+            G1_LAUNCHED();
+            idx ^= 1;
+            ++it;
+        }
+        cur = idx;
+    } else {
+        for (;;) {
+            if (it >= max_iter) break;
+            if (engine == G1_ENGINE_BLAS_CSRVEC)
+                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, g->t_row_off, g->t_col, vals, g->xf, g->yf, lanes, s));
+            else
+                G1_TRY(launch_spmv_merge<PlusTimesKahanF32>(n, g->m, g->t_row_off, g->t_col, vals, g->xf, g->yf,
+                                                            g->t_ws, s));
+            pr_apply_kernel<0><<<blocks, 256, 0, s>>>(n, g->yf, base_all, seed, base_seed, g->iw, g->r[cur],
+                                                      g->r[cur ^ 1], g->xf, g->partials);
+            G1_LAUNCHED();
+            reduce_partials_kernel<0><<<1, 256, 0, s>>>(g->partials, blocks, g->partials + kPartials);
+            G1_LAUNCHED();
+            cur ^= 1;
+            ++it;
+            G1_HIP(hipMemcpyAsync(g->h_scalar, g->partials + kPartials, sizeof(double), hipMemcpyDeviceToHost, s));
+            G1_HIP(hipStreamSynchronize(s));
+            delta = g->h_scalar[0];
+            if (delta < tol) break;
+        }
+    }
+    G1_HIP(hipEventRecord(g->ev[1], s));
+    G1_TRY(elapsed(g, st));
+    G1_HIP(hipMemcpyAsync(out, g->r[cur], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    G1_HIP(hipStreamSynchronize(s));
+    st->iterations = it;
+    st->pull_iterations = it;
+    st->edges_sum = static_cast<uint64_t>(it) * g->m;
+    st->last_delta = delta;
+    st->reached = n;
+    st->reached_edges = g->m;
+    return G1_OK;
+}
+
+g1_status check_run(g1_graph* g, uint32_t src, const void* out, g1_run_stats* st, const char* what) {
+    G1_CHECK(g != nullptr, G1_ERR_INVALID_ARG, "%s: null graph", what);
+    G1_CHECK(out != nullptr, G1_ERR_INVALID_ARG, "%s: null output", what);
+    G1_CHECK(src < g->n, G1_ERR_INVALID_ARG, "%s: source %u out of range (n = %u)", what, src, g->n);
+    if (st) memset(st, 0, sizeof(*st));
+    return G1_OK;
+}
+
+}  // namespace
+}  // namespace g1
+
+extern "C" {
+
+g1_status g1_bfs(g1_graph* g, uint32_t src, uint32_t* depth, int32_t engine, g1_run_stats* stats,
+                 g1_stream_t stream) {
+    using namespace g1;
+    g1_run_stats local;
+    if (!stats) stats = &local;
+    G1_TRY(check_run(g, src, depth, stats, "g1_bfs"));
+    std::lock_guard<std::mutex> lock(g->mu);
+    const hipStream_t s = as_stream(stream);
+    switch (engine) {
+        case G1_ENGINE_ADVANCE: return run_bfs_push(g, src, depth, false, stats, s);
+        case G1_ENGINE_DIROPT: return run_bfs_push(g, src, depth, true, stats, s);
+        case G1_ENGINE_BLAS_MERGE:
+        case G1_ENGINE_BLAS_CSRVEC: return run_minplus_pull(g, src, depth, false, engine, stats, s);
+        default: return set_error(G1_ERR_INVALID_ARG, "g1_bfs: unknown engine %d", engine);
+    }
+}
+
+g1_status g1_sssp(g1_graph* g, uint32_t src, uint32_t* dist, int32_t engine, g1_run_stats* stats,
+                  g1_stream_t stream) {
+    using namespace g1;
+    g1_run_stats local;
+    if (!stats) stats = &local;
+    G1_TRY(check_run(g, src, dist, stats, "g1_sssp"));
+    std::lock_guard<std::mutex> lock(g->mu);
+    const hipStream_t s = as_stream(stream);
+    switch (engine) {
+        case G1_ENGINE_ADVANCE: return run_sssp_push(g, src, dist, stats, s);
+        case G1_ENGINE_BLAS_MERGE:
+        case G1_ENGINE_BLAS_CSRVEC: return run_minplus_pull(g, src, dist, true, engine, stats, s);
+        default: return set_error(G1_ERR_INVALID_ARG, "g1_sssp: unknown engine %d", engine);
+    }
+}
+
+g1_status g1_pagerank(g1_graph* g, float alpha, uint32_t max_iter, double tol, int32_t converge_mode,
+                      int32_t engine, float* rank, g1_run_stats* stats, g1_stream_t stream) {
+    using namespace g1;
+    g1_run_stats local;
+    if (!stats) stats = &local;
+    G1_CHECK(g != nullptr && rank != nullptr, G1_ERR_INVALID_ARG, "g1_pagerank: null argument");
+    G1_CHECK(converge_mode == G1_PR_L1_F64 || converge_mode == G1_PR_MAX_ABS, G1_ERR_INVALID_ARG,
+             "g1_pagerank: unknown convergence mode %d", converge_mode);
+    G1_CHECK(engine == G1_ENGINE_BLAS_MERGE || engine == G1_ENGINE_BLAS_CSRVEC, G1_ERR_INVALID_ARG,
+             "g1_pagerank: engine must be BLAS_MERGE or BLAS_CSRVEC");
+    memset(stats, 0, sizeof(*stats));
+    std::lock_guard<std::mutex> lock(g->mu);
+    return run_pagerank(g, alpha, kInf, 0.0f, max_iter, tol, converge_mode, engine, rank, stats, as_stream(stream));
+}
+
+g1_status g1_ppr(g1_graph* g, uint32_t seed_vertex, float restart, uint32_t max_iter, double tol, float* rank,
+                 g1_run_stats* stats, g1_stream_t stream) {
+    using namespace g1;
+    g1_run_stats local;
+    if (!stats) stats = &local;
+    G1_TRY(check_run(g, seed_vertex, rank, stats, "g1_ppr"));
+    G1_CHECK(restart > 0.0f && restart <= 1.0f, G1_ERR_INVALID_ARG, "g1_ppr: restart must be in (0, 1]");
+    std::lock_guard<std::mutex> lock(g->mu);
+    return run_pagerank(g, 0.0f, seed_vertex, restart, max_iter, tol, G1_PR_L1_F64, G1_ENGINE_BLAS_MERGE, rank, stats,
+                        as_stream(stream));
+}
+
+}  // extern "C"

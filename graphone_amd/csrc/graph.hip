@@ -1,0 +1,191 @@
+// g1_graph: device-resident CSR (+ transpose) and its work areas. Replaces graph::build +
+// Graph::ToArch (include/GraphOne/graph/builder.h:9-46, graph.h:94-113): the host CSR is uploaded once
+// and stays resident in HBM; the in-edge CSR is built on the host from the same edges.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "g1/spmv_merge.hpp"
+#include "graph.hpp"
+#include "spmv_launch.hpp"
+
+namespace g1 {
+
+g1_status graph_alloc(g1_graph* g, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 4;
+    G1_HIP(hipMalloc(p, bytes));
+    g->owned.push_back(*p);
+    g->device_bytes += bytes;
+    return G1_OK;
+}
+
+template <typename T>
+static g1_status alloc_n(g1_graph* g, T** p, size_t count) {
+    void* v = nullptr;
+    G1_TRY(graph_alloc(g, &v, count * sizeof(T)));
+    *p = static_cast<T*>(v);
+    return G1_OK;
+}
+
+g1_status ensure_push_work(g1_graph* g) {
+    if (g->q[0]) return G1_OK;
+    const size_t n = g->n ? g->n : 1;
+    const size_t words = (n + 31) / 32 + 1;
+    for (int i = 0; i < 2; ++i) {
+        G1_TRY(alloc_n(g, &g->q[i], n));
+        G1_TRY(alloc_n(g, &g->qoff[i], n));
+        G1_TRY(alloc_n(g, &g->fb[i], words));
+    }
+    G1_TRY(alloc_n(g, &g->stamp, n));
+    return G1_OK;
+}
+
+g1_status ensure_pull_work(g1_graph* g, bool pagerank) {
+    const size_t n = g->n ? g->n : 1;
+    if (!g->xu) {
+        G1_TRY(alloc_n(g, &g->xu, n));
+        G1_TRY(alloc_n(g, &g->yu, n));
+    }
+    if (pagerank && !g->xf) {
+        G1_TRY(alloc_n(g, &g->xf, n));
+        G1_TRY(alloc_n(g, &g->yf, n));
+        G1_TRY(alloc_n(g, &g->iw, n));
+        G1_TRY(alloc_n(g, &g->r[0], n));
+        G1_TRY(alloc_n(g, &g->r[1], n));
+    }
+    if (pagerank && g->t_w && !g->t_wf) {
+        G1_TRY(alloc_n(g, &g->t_wf, g->m ? g->m : 1));
+        std::vector<uint32_t> hw(g->m);
+        std::vector<float> hf(g->m);
+        if (g->m) {
+            G1_HIP(hipMemcpy(hw.data(), g->t_w, sizeof(uint32_t) * g->m, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < g->m; ++i) hf[i] = static_cast<float>(hw[i]);
+            G1_HIP(hipMemcpy(g->t_wf, hf.data(), sizeof(float) * g->m, hipMemcpyHostToDevice));
+        }
+    }
+    return G1_OK;
+}
+
+static void graph_free_all(g1_graph* g) {
+    for (void* p : g->owned) (void)hipFree(p);
+    g->owned.clear();
+    if (g->h_ctr) (void)hipHostFree(g->h_ctr);
+    if (g->h_scalar) (void)hipHostFree(g->h_scalar);
+    for (auto& e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+}  // namespace g1
+
+extern "C" {
+
+g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags) {
+    using namespace g1;
+    G1_CHECK(out != nullptr && csr != nullptr, G1_ERR_INVALID_ARG, "g1_graph_create: null argument");
+    *out = nullptr;
+    G1_CHECK(csr->row_offsets != nullptr, G1_ERR_INVALID_ARG, "g1_graph_create: null row_offsets");
+    G1_CHECK(csr->m == 0 || csr->col_indices != nullptr, G1_ERR_INVALID_ARG, "g1_graph_create: null col_indices");
+    G1_CHECK(csr->row_offsets[0] == 0 && csr->row_offsets[csr->n] == csr->m, G1_ERR_INVALID_ARG,
+             "g1_graph_create: row_offsets must start at 0 and end at m");
+    G1_CHECK(csr->m < kInf, G1_ERR_INVALID_ARG, "g1_graph_create: too many edges");
+    for (uint32_t v = 0; v < csr->n; ++v)
+        G1_CHECK(csr->row_offsets[v] <= csr->row_offsets[v + 1], G1_ERR_INVALID_ARG,
+                 "g1_graph_create: row_offsets decrease at row %u", v);
+    for (uint32_t e = 0; e < csr->m; ++e)
+        G1_CHECK(csr->col_indices[e] < csr->n, G1_ERR_INVALID_ARG,
+                 "g1_graph_create: column %u of edge %u out of range", csr->col_indices[e], e);
+
+    g1_graph* g = new g1_graph();
+    g->n = csr->n;
+    g->m = csr->m;
+    g->flags = flags;
+    auto fail = [&](g1_status s) {
+        graph_free_all(g);
+        delete g;
+        return s;
+    };
+    g1_status s;
+    const size_t n1 = static_cast<size_t>(csr->n) + 1;
+    if ((s = alloc_n(g, &g->row_off, n1)) != G1_OK) return fail(s);
+    if ((s = alloc_n(g, &g->col, csr->m ? csr->m : 1)) != G1_OK) return fail(s);
+    if (hipMemcpy(g->row_off, csr->row_offsets, n1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        (csr->m && hipMemcpy(g->col, csr->col_indices, size_t(csr->m) * 4, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(set_error(G1_ERR_HIP, "g1_graph_create: upload failed"));
+    if (csr->weights) {
+        if ((s = alloc_n(g, &g->w, csr->m ? csr->m : 1)) != G1_OK) return fail(s);
+        if (csr->m && hipMemcpy(g->w, csr->weights, size_t(csr->m) * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_error(G1_ERR_HIP, "g1_graph_create: weight upload failed"));
+    }
+    if (flags & G1_GRAPH_SYMMETRIC) {
+        g->has_t = true;
+        g->t_alias = true;
+        g->t_row_off = g->row_off;
+        g->t_col = g->col;
+        g->t_w = g->w;
+    } else if (flags & G1_GRAPH_TRANSPOSE) {
+        g1_host_csr t;
+        if ((s = g1_host_csr_transpose(csr, &t)) != G1_OK) return fail(s);
+        g->has_t = true;
+        s = alloc_n(g, &g->t_row_off, n1);
+        if (s == G1_OK) s = alloc_n(g, &g->t_col, csr->m ? csr->m : 1);
+        if (s == G1_OK && t.weights) s = alloc_n(g, &g->t_w, csr->m ? csr->m : 1);
+        if (s == G1_OK) {
+            if (hipMemcpy(g->t_row_off, t.row_offsets, n1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                (csr->m && hipMemcpy(g->t_col, t.col_indices, size_t(csr->m) * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+                (t.weights && csr->m &&
+                 hipMemcpy(g->t_w, t.weights, size_t(csr->m) * 4, hipMemcpyHostToDevice) != hipSuccess))
+                s = set_error(G1_ERR_HIP, "g1_graph_create: transpose upload failed");
+        }
+        g1_host_csr_free(&t);
+        if (s != G1_OK) return fail(s);
+    }
+    if (g->has_t) {
+        g->t_ws_bytes = merge_workspace_bytes(g->n, g->m);
+        if ((s = graph_alloc(g, &g->t_ws, g->t_ws_bytes)) != G1_OK) return fail(s);
+        if ((s = launch_merge_partition(g->n, g->m, g->t_row_off, g->t_ws, nullptr)) != G1_OK) return fail(s);
+    }
+    if ((s = alloc_n(g, &g->ctr, 16)) != G1_OK) return fail(s);
+    if ((s = alloc_n(g, &g->partials, kPartials + 8)) != G1_OK) return fail(s);
+    if (hipHostMalloc(reinterpret_cast<void**>(&g->h_ctr), 16 * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&g->h_scalar), 8 * sizeof(double)) != hipSuccess)
+        return fail(set_error(G1_ERR_HIP, "g1_graph_create: pinned host allocation failed"));
+    if (hipEventCreate(&g->ev[0]) != hipSuccess || hipEventCreate(&g->ev[1]) != hipSuccess)
+        return fail(set_error(G1_ERR_HIP, "g1_graph_create: event creation failed"));
+    if (hipDeviceSynchronize() != hipSuccess) return fail(set_error(G1_ERR_HIP, "g1_graph_create: sync failed"));
+    *out = g;
+    return G1_OK;
+}
+
+g1_status g1_graph_destroy(g1_graph* g) {
+    if (!g) return G1_OK;
+    (void)hipDeviceSynchronize();
+    g1::graph_free_all(g);
+    delete g;
+    return G1_OK;
+}
+
+g1_status g1_graph_info(const g1_graph* g, uint32_t* n, uint32_t* m, uint64_t* device_bytes) {
+    G1_CHECK(g != nullptr, G1_ERR_INVALID_ARG, "g1_graph_info: null graph");
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (device_bytes) *device_bytes = g->device_bytes;
+    return G1_OK;
+}
+
+g1_status g1_graph_csr_dev(const g1_graph* g, int32_t transpose, const uint32_t** row_offsets,
+                           const uint32_t** col_indices, const uint32_t** weights) {
+    G1_CHECK(g != nullptr, G1_ERR_INVALID_ARG, "g1_graph_csr_dev: null graph");
+    if (transpose) {
+        G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "g1_graph_csr_dev: graph was created without its transpose");
+        if (row_offsets) *row_offsets = g->t_row_off;
+        if (col_indices) *col_indices = g->t_col;
+        if (weights) *weights = g->t_w;
+    } else {
+        if (row_offsets) *row_offsets = g->row_off;
+        if (col_indices) *col_indices = g->col;
+        if (weights) *weights = g->w;
+    }
+    return G1_OK;
+}
+
+}  // extern "C"

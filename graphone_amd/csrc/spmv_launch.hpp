@@ -1,0 +1,88 @@
+// Host-side launchers of the SpMV kernels, shared by the C-ABI exports and the engines.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "g1/spmv_merge.hpp"
+#include "internal.hpp"
+
+namespace g1 {
+
+inline uint32_t csrvec_lanes(uint32_t n_rows, uint32_t nnz) {
+    // Average row length rounded up to a power of two, at most one wave per row.
+    const uint32_t avg = n_rows ? nnz / n_rows : 0;
+    uint32_t lanes = 1;
+    while (lanes < avg && lanes < 64) lanes <<= 1;
+    return lanes;
+}
+
+inline g1_status launch_merge_partition(uint32_t n_rows, uint32_t nnz, const uint32_t* row_offsets, void* ws,
+                                        hipStream_t stream) {
+    if (n_rows == 0) return G1_OK;
+    const uint32_t tiles = merge_num_tiles(n_rows, nnz);
+    const uint32_t blocks = (tiles + 1 + 255) / 256;
+    merge_partition_kernel<<<dim3(blocks), dim3(256), 0, stream>>>(row_offsets, n_rows, nnz, tiles,
+                                                                   merge_coords(ws));
+    G1_LAUNCHED();
+    return G1_OK;
+}
+
+template <typename S>
+g1_status launch_spmv_merge(uint32_t n_rows, uint32_t nnz, const uint32_t* row_offsets, const uint32_t* col,
+                            const typename S::value_t* vals, const typename S::x_t* x, typename S::y_t* y,
+                            void* ws, hipStream_t stream) {
+    if (n_rows == 0) return G1_OK;
+    const uint32_t tiles = merge_num_tiles(n_rows, nnz);
+    MergeCoord* coords = merge_coords(ws);
+    MergeTileRec<S>* recs = merge_records<S>(ws, n_rows, nnz);
+    if (vals) {
+        spmv_merge_tile_kernel<S, true><<<dim3(tiles), dim3(kMergeThreads), 0, stream>>>(
+            n_rows, nnz, row_offsets, col, vals, x, y, coords, recs);
+    } else {
+        spmv_merge_tile_kernel<S, false><<<dim3(tiles), dim3(kMergeThreads), 0, stream>>>(
+            n_rows, nnz, row_offsets, col, vals, x, y, coords, recs);
+    }
+    G1_LAUNCHED();
+    if (tiles > 1) {
+        spmv_merge_fixup_kernel<S><<<dim3((tiles + 255) / 256), dim3(256), 0, stream>>>(tiles, coords, recs, y);
+        G1_LAUNCHED();
+    }
+    return G1_OK;
+}
+
+template <typename S, bool V, int L>
+inline void csrvec_launch_one(uint32_t n_rows, const uint32_t* row_offsets, const uint32_t* col,
+                              const typename S::value_t* vals, const typename S::x_t* x, typename S::y_t* y,
+                              hipStream_t stream) {
+    const uint64_t threads = static_cast<uint64_t>(n_rows) * L;
+    const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256);
+    spmv_csrvec_kernel<S, V, L><<<dim3(blocks), dim3(256), 0, stream>>>(n_rows, row_offsets, col, vals, x, y);
+}
+
+template <typename S, bool V>
+inline void csrvec_dispatch(uint32_t lanes, uint32_t n_rows, const uint32_t* row_offsets, const uint32_t* col,
+                            const typename S::value_t* vals, const typename S::x_t* x, typename S::y_t* y,
+                            hipStream_t stream) {
+    switch (lanes) {
+        case 1: csrvec_launch_one<S, V, 1>(n_rows, row_offsets, col, vals, x, y, stream); break;
+        case 2: csrvec_launch_one<S, V, 2>(n_rows, row_offsets, col, vals, x, y, stream); break;
+        case 4: csrvec_launch_one<S, V, 4>(n_rows, row_offsets, col, vals, x, y, stream); break;
+        case 8: csrvec_launch_one<S, V, 8>(n_rows, row_offsets, col, vals, x, y, stream); break;
+        case 16: csrvec_launch_one<S, V, 16>(n_rows, row_offsets, col, vals, x, y, stream); break;
+        case 32: csrvec_launch_one<S, V, 32>(n_rows, row_offsets, col, vals, x, y, stream); break;
+        default: csrvec_launch_one<S, V, 64>(n_rows, row_offsets, col, vals, x, y, stream); break;
+    }
+}
+
+template <typename S>
+g1_status launch_spmv_csrvec(uint32_t n_rows, const uint32_t* row_offsets, const uint32_t* col,
+                             const typename S::value_t* vals, const typename S::x_t* x, typename S::y_t* y,
+                             uint32_t lanes, hipStream_t stream) {
+    if (n_rows == 0) return G1_OK;
+    if (vals) csrvec_dispatch<S, true>(lanes, n_rows, row_offsets, col, vals, x, y, stream);
+    else csrvec_dispatch<S, false>(lanes, n_rows, row_offsets, col, vals, x, y, stream);
+    G1_LAUNCHED();
+    return G1_OK;
+}
+
+}  // namespace g1

@@ -1,0 +1,182 @@
+// Apply/construct kernels of the pull (BLAS) engine, engine/blas_gc.h:21-76 and :171-435 redesigned:
+// the reference's Construct (fill + transform + scatter) and Apply (transform + copy_if) passes are
+// fused into one kernel that reads y once and writes the next x, the per-vertex state and a count.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "g1/device.hpp"
+
+namespace g1 {
+
+// Min-plus apply (apps/sssp_blas.cu:75-82 + blas_gc.h:43-55):
+//   pred = y[v] != INF && y[v] < d[v];  if pred: d[v] = y[v];  x[v] = pred ? d[v] : INF
+// count += pred (order free, so the count is exact).
+static __global__ void __launch_bounds__(256) apply_minplus_kernel(uint32_t n, const uint32_t* __restrict__ y,
+                                                            uint32_t* __restrict__ d, uint32_t* __restrict__ x,
+                                                            unsigned long long* __restrict__ count) {
+    uint32_t local = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const uint32_t res = y[v];
+        uint32_t xv = kInf;
+        if (res != kInf) {
+            const uint32_t dv = d[v];
+            if (res < dv) {
+                d[v] = res;
+                xv = res;
+                ++local;
+            }
+        }
+        x[v] = xv;
+    }
+    const uint32_t w = wave_sum(local);
+    if (lane_id() == 0 && w) atomicAdd(count, static_cast<unsigned long long>(w));
+}
+
+// x = INF everywhere except x[src] = 0 (the sparse engine's Construct for the frontier {src}),
+// d likewise (the component's Init: fill(dists, INF); dists.set(src, 0)).
+static __global__ void __launch_bounds__(256) minplus_seed_kernel(uint32_t n, uint32_t src, uint32_t* __restrict__ d,
+                                                           uint32_t* __restrict__ x) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const uint32_t val = (v == src) ? 0u : kInf;
+        d[v] = val;
+        if (x) x[v] = val;
+    }
+}
+
+// iw[u] = alpha / sum_out w(u) (0 when u has no out-edge), summed in float left to right as
+// apps/pagerank_blas.cu:41-53 does; r[u] = r0; x[u] = r[u] * iw[u].
+static __global__ void __launch_bounds__(256) pr_init_kernel(uint32_t n, const uint32_t* __restrict__ row_off,
+                                                      const uint32_t* __restrict__ wts, float alpha,
+                                                      float* __restrict__ iw, float* __restrict__ r, float r0,
+                                                      float* __restrict__ x, uint32_t seed_vertex) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += stride) {
+        float s = 0.0f;
+        const uint32_t b = row_off[u], e = row_off[u + 1];
+        if (wts) {
+            for (uint32_t k = b; k < e; ++k) s += static_cast<float>(wts[k]);
+        } else {
+            s = static_cast<float>(e - b);
+        }
+        const float inv = (s != 0.0f) ? alpha / s : 0.0f;
+        iw[u] = inv;
+        const float rv = (seed_vertex == kInf) ? r0 : (u == seed_vertex ? 1.0f : 0.0f);
+        r[u] = rv;
+        x[u] = rv * inv;
+    }
+}
+
+// r_new[v] = base(v) + y[v]; x[v] = r_new[v] * iw[v]; per-block FP64 partial of |r_new - r_old|
+// (mode 0: sum, the L1 norm) or max |r_new - r_old| in float (mode 1, the reference's test).
+// base(v) = base_all, plus base_seed when v == seed (personalized PageRank).
+template <int kMode>
+__global__ void __launch_bounds__(256) pr_apply_kernel(uint32_t n, const float* __restrict__ y, float base_all,
+                                                       uint32_t seed, float base_seed, const float* __restrict__ iw,
+                                                       const float* __restrict__ r_old, float* __restrict__ r_new,
+                                                       float* __restrict__ x, double* __restrict__ partials) {
+    __shared__ double s_part[256 / kWave];
+    double acc = 0.0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const float b = (v == seed) ? base_seed : base_all;
+        const float rn = b + y[v];
+        r_new[v] = rn;
+        x[v] = rn * iw[v];
+        if (kMode == 0) {
+            acc += fabs(static_cast<double>(rn) - static_cast<double>(r_old[v]));
+        } else {
+            const double dlt = static_cast<double>(fabsf(rn - r_old[v]));
+            acc = dlt > acc ? dlt : acc;
+        }
+    }
+    // Fixed-shape reduction: lanes, then waves in order.
+#pragma unroll
+    for (int off = kWave / 2; off >= 1; off >>= 1) {
+        const double o = __shfl_down(acc, off);
+        if (kMode == 0) acc += o; else acc = o > acc ? o : acc;
+    }
+    if (lane_id() == 0) s_part[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = s_part[0];
+        for (int w = 1; w < 256 / kWave; ++w) {
+            if (kMode == 0) t += s_part[w]; else t = s_part[w] > t ? s_part[w] : t;
+        }
+        partials[blockIdx.x] = t;
+    }
+}
+
+// Folds per-block partials in block order into out[0] (sum or max); one block.
+template <int kMode>
+__global__ void __launch_bounds__(256) reduce_partials_kernel(const double* __restrict__ partials, uint32_t count,
+                                                              double* __restrict__ out) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    // Each thread folds a contiguous run, then the runs are folded in thread order.
+    const uint32_t per = (count + 255) / 256;
+    const uint32_t b = threadIdx.x * per;
+    for (uint32_t i = b; i < b + per && i < count; ++i) {
+        if (kMode == 0) acc += partials[i]; else acc = partials[i] > acc ? partials[i] : acc;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < 256; ++i) {
+            if (kMode == 0) t += s[i]; else t = s[i] > t ? s[i] : t;
+        }
+        out[0] = t;
+    }
+}
+
+// Max |a - b| over float vectors (the reference's IsConvergent transform_reduce,
+// apps/pagerank_blas.cu:63-78), written as per-block partials for reduce_partials_kernel<1>.
+static __global__ void __launch_bounds__(256) max_abs_diff_kernel(uint32_t n, const float* __restrict__ a,
+                                                           const float* __restrict__ b,
+                                                           double* __restrict__ partials) {
+    __shared__ double s_part[256 / kWave];
+    double acc = 0.0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const double dlt = static_cast<double>(fabsf(a[v] - b[v]));
+        acc = dlt > acc ? dlt : acc;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off >= 1; off >>= 1) {
+        const double o = __shfl_down(acc, off);
+        acc = o > acc ? o : acc;
+    }
+    if (lane_id() == 0) s_part[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = s_part[0];
+        for (int w = 1; w < 256 / kWave; ++w) t = s_part[w] > t ? s_part[w] : t;
+        partials[blockIdx.x] = t;
+    }
+}
+
+// Reached-vertex statistics: count of d[v] != INF and the sum of their out-degrees.
+static __global__ void __launch_bounds__(256) reached_stats_kernel(uint32_t n, const uint32_t* __restrict__ d,
+                                                            const uint32_t* __restrict__ row_off,
+                                                            unsigned long long* __restrict__ out2) {
+    unsigned long long cnt = 0, edges = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        if (d[v] != kInf) {
+            ++cnt;
+            edges += row_off[v + 1] - row_off[v];
+        }
+    }
+    cnt = wave_sum(cnt);
+    edges = wave_sum(edges);
+    if (lane_id() == 0) {
+        if (cnt) atomicAdd(&out2[0], cnt);
+        if (edges) atomicAdd(&out2[1], edges);
+    }
+}
+
+}  // namespace g1

@@ -1,0 +1,241 @@
+// Merge-path semiring SpMV for gfx950 (Merrill & Garland, "Merge-based parallel sparse matrix-vector
+// multiplication", SC16), the algorithm the reference takes from CUB
+// (include/GraphOne/archi/blas/spmv/cuda/merge_based/*), designed here for wave64 and LDS.
+//
+// The merge path walks the two sorted lists A = row_end[0..n_rows) (row_end[r] = row_offsets[r+1])
+// and B = 0..nnz-1 (nonzero indices). Step (i, j) consumes nonzero j when j < row_end[i], otherwise
+// it closes row i. The n_rows + nnz steps are cut into tiles of kMergeTile steps (one workgroup of
+// kMergeThreads threads, kMergeItems consecutive steps per thread). Tile start coordinates are found
+// once per matrix (merge_partition_kernel, the reference's SegmentSearch run once in the dispatcher
+// constructor, cuda/cuda.cuh:89-91) and reused by every multiply.
+//
+// Fold order (what makes plus-times results reproducible): inside a thread a row's products are
+// added left to right; a row that crosses thread boundaries is finished by the thread that closes it
+// by merging, left to right, the tails of the threads before it back to the last thread that closed
+// a row; a row that crosses tiles is finished by the fixup kernel, which merges the carries of the
+// tiles it crosses, left to right, followed by the closing tile's head. The order therefore depends
+// only on the matrix and the constants below, never on scheduling, so repeated runs and the batched
+// SpMM (which uses the same partition) give the same bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "g1/device.hpp"
+#include "g1/semiring.hpp"
+
+namespace g1 {
+
+constexpr int kMergeThreads = 256;
+constexpr int kMergeItems = 8;
+constexpr int kMergeTile = kMergeThreads * kMergeItems;  // 2048 merge steps per workgroup
+
+struct MergeCoord {
+    uint32_t x;  // rows closed before this point
+    uint32_t y;  // nonzeros consumed before this point
+};
+
+// Per-tile record left by the tile kernel for the fixup kernel.
+template <typename S>
+struct MergeTileRec {
+    typename S::acc_t carry;  // fold of the row still open at the tile end (after its last row end)
+    typename S::acc_t head;   // fold, inside the tile, of the first row it closes (when deferred)
+    uint32_t flags;           // bit 0: the tile closes at least one row; bit 1: head is deferred
+};
+
+constexpr uint32_t kTileHasEnd = 1u;
+constexpr uint32_t kTileDeferred = 2u;
+
+__host__ __device__ inline uint32_t merge_num_tiles(uint32_t n_rows, uint32_t nnz) {
+    const uint64_t items = static_cast<uint64_t>(n_rows) + nnz;
+    return static_cast<uint32_t>((items + kMergeTile - 1) / kMergeTile);
+}
+
+// Workspace: coords[num_tiles + 1] then records[num_tiles] (sized for the largest record type).
+constexpr size_t kMergeRecBytes = 32;
+inline size_t merge_workspace_bytes(uint32_t n_rows, uint32_t nnz) {
+    const size_t t = merge_num_tiles(n_rows, nnz);
+    const size_t coords = ((t + 1) * sizeof(MergeCoord) + 255) & ~size_t(255);
+    return coords + t * kMergeRecBytes + 256;
+}
+inline MergeCoord* merge_coords(void* ws) { return static_cast<MergeCoord*>(ws); }
+template <typename S>
+inline MergeTileRec<S>* merge_records(void* ws, uint32_t n_rows, uint32_t nnz) {
+    static_assert(sizeof(MergeTileRec<S>) <= kMergeRecBytes, "tile record too large");
+    const size_t t = merge_num_tiles(n_rows, nnz);
+    const size_t coords = ((t + 1) * sizeof(MergeCoord) + 255) & ~size_t(255);
+    return reinterpret_cast<MergeTileRec<S>*>(static_cast<char*>(ws) + coords);
+}
+
+// Merge-path search on diagonal `diag`: the smallest i with row_end[i] > diag - i - 1
+// (reference: merge_based/thread_search.cuh:11-52).
+__device__ __forceinline__ MergeCoord merge_path_search(uint64_t diag, const uint32_t* __restrict__ row_end,
+                                                        uint32_t n_rows, uint32_t nnz) {
+    uint32_t lo = diag > nnz ? static_cast<uint32_t>(diag - nnz) : 0u;
+    uint32_t hi = diag < n_rows ? static_cast<uint32_t>(diag) : n_rows;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (static_cast<uint64_t>(row_end[mid]) + mid + 1 <= diag) lo = mid + 1; else hi = mid;
+    }
+    return MergeCoord{lo, static_cast<uint32_t>(diag - lo)};
+}
+
+static __global__ void __launch_bounds__(256) merge_partition_kernel(const uint32_t* __restrict__ row_offsets,
+                                                              uint32_t n_rows, uint32_t nnz,
+                                                              uint32_t num_tiles,
+                                                              MergeCoord* __restrict__ coords) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > num_tiles) return;
+    const uint64_t total = static_cast<uint64_t>(n_rows) + nnz;
+    uint64_t diag = static_cast<uint64_t>(t) * kMergeTile;
+    if (diag > total) diag = total;
+    coords[t] = merge_path_search(diag, row_offsets + 1, n_rows, nnz);
+}
+
+// One workgroup per tile. LDS: row ends and products of the tile, then one tail per thread.
+template <typename S, bool kHasValues>
+__global__ void __launch_bounds__(kMergeThreads)
+spmv_merge_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row_offsets,
+                       const uint32_t* __restrict__ col, const typename S::value_t* __restrict__ vals,
+                       const typename S::x_t* __restrict__ x, typename S::y_t* __restrict__ y,
+                       const MergeCoord* __restrict__ coords, MergeTileRec<S>* __restrict__ recs) {
+    using acc_t = typename S::acc_t;
+    using y_t = typename S::y_t;
+    __shared__ uint32_t s_rowend[kMergeTile + 1];
+    __shared__ y_t s_prod[kMergeTile];
+    __shared__ acc_t s_tail[kMergeThreads];
+    __shared__ uint32_t s_hasend[kMergeThreads];
+
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const MergeCoord c0 = coords[tile];
+    const MergeCoord c1 = coords[tile + 1];
+    const uint32_t rows = c1.x - c0.x;
+    const uint32_t nzs = c1.y - c0.y;
+    const uint32_t total = rows + nzs;
+
+    for (uint32_t k = tid; k < rows; k += kMergeThreads) s_rowend[k] = row_offsets[c0.x + 1 + k];
+    if (tid == 0) s_rowend[rows] = kInf;  // no further row end inside this tile
+    for (uint32_t k = tid; k < nzs; k += kMergeThreads) {
+        const uint32_t e = c0.y + k;
+        const typename S::value_t w = kHasValues ? vals[e] : typename S::value_t(1);
+        s_prod[k] = S::combine(w, x[col[e]]);
+    }
+    __syncthreads();
+
+    // This thread's first step inside the tile, then its kMergeItems steps.
+    uint32_t d = tid * kMergeItems;
+    if (d > total) d = total;
+    uint32_t d_end = d + kMergeItems;
+    if (d_end > total) d_end = total;
+    uint32_t lo = d > nzs ? d - nzs : 0u;
+    uint32_t hi = d < rows ? d : rows;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (static_cast<uint64_t>(s_rowend[mid]) + 1 <= static_cast<uint64_t>(c0.y) + (d - mid)) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t i = lo;
+    uint32_t j = d - lo;
+
+    acc_t acc = S::identity();
+    acc_t head = S::identity();
+    bool hasend = false;
+    uint32_t first_row = 0;
+    for (uint32_t k = d; k < d_end; ++k) {
+        if (c0.y + j < s_rowend[i]) {
+            S::add(acc, s_prod[j]);
+            ++j;
+        } else {
+            if (!hasend) {
+                head = acc;
+                hasend = true;
+                first_row = c0.x + i;
+            } else {
+                y[c0.x + i] = S::finalize(acc);
+            }
+            acc = S::identity();
+            ++i;
+        }
+    }
+    s_tail[tid] = acc;
+    s_hasend[tid] = hasend ? 1u : 0u;
+    __syncthreads();
+
+    if (hasend) {
+        int u = static_cast<int>(tid) - 1;
+        while (u >= 0 && s_hasend[u] == 0u) --u;
+        acc_t chain = S::identity();
+        for (int v = (u >= 0 ? u : 0); v < static_cast<int>(tid); ++v) chain = S::merge(chain, s_tail[v]);
+        const acc_t whole = S::merge(chain, head);
+        // The tile's first closed row began in an earlier tile: leave it for the fixup kernel.
+        if (u < 0 && row_offsets[first_row] < c0.y) {
+            recs[tile].head = whole;
+        } else {
+            y[first_row] = S::finalize(whole);
+        }
+    }
+    if (tid == kMergeThreads - 1) {
+        int u = kMergeThreads - 1;
+        while (u >= 0 && s_hasend[u] == 0u) --u;
+        acc_t carry = S::identity();
+        for (int v = (u >= 0 ? u : 0); v < kMergeThreads; ++v) carry = S::merge(carry, s_tail[v]);
+        recs[tile].carry = carry;
+        uint32_t flags = (u >= 0) ? kTileHasEnd : 0u;
+        // Deferred exactly when the tile's first closed row (row c0.x) began in an earlier tile; the
+        // first closing thread made the same test above and stored its head instead of writing y.
+        if (u >= 0 && row_offsets[c0.x] < c0.y) flags |= kTileDeferred;
+        recs[tile].flags = flags;
+    }
+}
+
+// One thread per tile: finish the first closed row of every deferred tile by folding, left to right,
+// the carries of the earlier tiles the row crosses and then the tile's head (the reference folds
+// these carries with CAS atomics in arbitrary order, agent_segment_fixup.cuh:50-80,257-302).
+template <typename S>
+__global__ void __launch_bounds__(256) spmv_merge_fixup_kernel(uint32_t num_tiles,
+                                                              const MergeCoord* __restrict__ coords,
+                                                              const MergeTileRec<S>* __restrict__ recs,
+                                                              typename S::y_t* __restrict__ y) {
+    using acc_t = typename S::acc_t;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tiles || t == 0) return;
+    if ((recs[t].flags & kTileDeferred) == 0u) return;
+    const uint32_t row = coords[t].x;
+    uint32_t u = t - 1;
+    while (u > 0 && (recs[u].flags & kTileHasEnd) == 0u) --u;
+    acc_t acc = S::identity();
+    for (uint32_t v = u; v < t; ++v) acc = S::merge(acc, recs[v].carry);
+    acc = S::merge(acc, recs[t].head);
+    y[row] = S::finalize(acc);
+}
+
+// CSR-vector: L lanes per row (L a power of two dividing 64), lanes stride over the row, then a
+// shuffle tree folds lane results with lower lanes on the left.
+template <typename S, bool kHasValues, int L>
+__global__ void __launch_bounds__(256)
+spmv_csrvec_kernel(uint32_t n_rows, const uint32_t* __restrict__ row_offsets, const uint32_t* __restrict__ col,
+                   const typename S::value_t* __restrict__ vals, const typename S::x_t* __restrict__ x,
+                   typename S::y_t* __restrict__ y) {
+    using acc_t = typename S::acc_t;
+    const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t row = gid / L;
+    const uint32_t lane = static_cast<uint32_t>(gid % L);
+    acc_t acc = S::identity();
+    if (row < n_rows) {
+        const uint32_t beg = row_offsets[row];
+        const uint32_t end = row_offsets[row + 1];
+        for (uint32_t e = beg + lane; e < end; e += L) {
+            const typename S::value_t w = kHasValues ? vals[e] : typename S::value_t(1);
+            S::add(acc, S::combine(w, x[col[e]]));
+        }
+    }
+#pragma unroll
+    for (int off = L / 2; off >= 1; off >>= 1) {
+        const acc_t other = S::shfl_down(acc, off, L);
+        if (lane + off < static_cast<uint32_t>(L)) acc = S::merge(acc, other);
+    }
+    if (row < n_rows && lane == 0) y[row] = S::finalize(acc);
+}
+
+}  // namespace g1

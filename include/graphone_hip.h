@@ -1,0 +1,224 @@
+/*
+ * graphone_hip.h -- C-ABI of libgraphone_hip.so, the gfx950 (MI355X) graph-computation hot path.
+ *
+ * The reference (GraphOne) is a header-only C++/CUDA template library with no ABI of its own: its
+ * engines call the SpMV dispatcher, LaunchKernel and Thrust wrappers directly. Every entry point below
+ * replaces one of those internal boundaries; the comment on each names the reference interface it
+ * stands in for (paths relative to the reference root).
+ *
+ * Conventions
+ *   - Every function returns g1_status (0 = G1_OK). On failure g1_last_error_message() holds a
+ *     thread-local description. Errors are always checked (the reference's checkCudaErrors is a no-op
+ *     unless DEBUG_CUDA, include/GraphOne/archi/check/cuda.cuh:10-16).
+ *   - Pointers named *_dev are device (HBM) pointers; *_host are host pointers. Callers own every
+ *     buffer they pass. Functions never allocate device memory, except the graph object (which owns
+ *     the resident CSR and its per-graph work areas) and the serving registry.
+ *   - Vertex ids, edge offsets and distances are uint32 (vid_t/eid_t, include/GraphOne/type.hpp:8-9).
+ *     G1_INF = 0xFFFFFFFF is the unreached distance and the invalid vertex
+ *     (include/GraphOne/utils/limits.hpp:54-73).
+ *   - All work is stream-ordered on the given stream (NULL = the default stream).
+ */
+#ifndef GRAPHONE_HIP_H
+#define GRAPHONE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t g1_status;
+#define G1_OK 0
+#define G1_ERR_INVALID_ARG 1
+#define G1_ERR_HIP 2
+#define G1_ERR_OUT_OF_MEMORY 3
+#define G1_ERR_BUDGET 4
+#define G1_ERR_NOT_FOUND 5
+#define G1_ERR_BUSY 6
+#define G1_ERR_CANCELLED 7
+#define G1_ERR_INTERNAL 8
+#define G1_ERR_NOT_CONVERGED 9
+
+#define G1_INF 0xFFFFFFFFu
+
+typedef void* g1_stream_t; /* a hipStream_t; NULL is the default stream */
+typedef void* g1_event_t;  /* a hipEvent_t */
+
+/* ---------------------------------------------------------------------------------------------
+ * Status and runtime.  Replaces archi::memalloc/memfree/memset/memcpy
+ * (include/GraphOne/archi/mem/cuda.cuh:11-59), archi::fill (include/GraphOne/archi/thrust/functions.h:28-31)
+ * and archi::LaunchSync (include/GraphOne/archi/kernel/cuda.cuh:93-95).
+ * ------------------------------------------------------------------------------------------- */
+const char* g1_status_string(g1_status status);
+const char* g1_last_error_message(void);
+int32_t g1_abi_version(void);
+
+g1_status g1_device_info(int32_t* device_count, int32_t* cu_count, uint64_t* hbm_bytes,
+                         char* name, size_t name_len);
+g1_status g1_malloc(void** ptr_dev, size_t bytes);
+g1_status g1_free(void* ptr_dev);
+g1_status g1_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes, g1_stream_t stream);
+g1_status g1_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes, g1_stream_t stream);
+g1_status g1_memcpy_d2d(void* dst_dev, const void* src_dev, size_t bytes, g1_stream_t stream);
+/* Fill kernels (not hipMemset), so they are safe inside captured HIP graphs. */
+g1_status g1_fill_u32(uint32_t* dst_dev, uint32_t value, size_t count, g1_stream_t stream);
+g1_status g1_fill_f32(float* dst_dev, float value, size_t count, g1_stream_t stream);
+g1_status g1_stream_create(g1_stream_t* stream);
+g1_status g1_stream_destroy(g1_stream_t stream);
+g1_status g1_stream_synchronize(g1_stream_t stream);
+g1_status g1_event_create(g1_event_t* event);
+g1_status g1_event_destroy(g1_event_t event);
+g1_status g1_event_record(g1_event_t event, g1_stream_t stream);
+g1_status g1_event_synchronize(g1_event_t event);
+g1_status g1_event_elapsed_ms(float* ms, g1_event_t start, g1_event_t stop);
+
+/* ---------------------------------------------------------------------------------------------
+ * Semiring SpMV  y = A (x) x  on a CSR matrix A (n_rows x n_cols, nnz nonzeros).
+ *   min-plus u32 : y[r] = min_j ( x[c_j] == INF ? INF : A[r,c_j] + x[c_j] ), empty row -> INF
+ *                  (the SSSP semiring of apps/sssp_blas.cu:55-73 with initialize = default_result)
+ *   plus-times f32: y[r] = sum_j A[r,c_j] * x[c_j] with compensated (Kahan-Babuska) accumulation,
+ *                  empty row -> 0 (the PageRank semiring of apps/pagerank_blas.cu:82-111)
+ * `values_dev` may be NULL: every nonzero then has value 1 (an unweighted graph).
+ * Results are deterministic: the plus-times fold order is a function of the matrix only.
+ *
+ * Merge-path (Merrill & Garland): replaces SpmvDispatcher<SpmvCudaCsrMergeBased>
+ * (include/GraphOne/archi/blas/spmv/cuda/cuda.cuh:196-246): GetBufferSize -> workspace_bytes,
+ * SegmentSearch (run once per matrix) -> partition, SpmvMerge -> spmv_merge_*.
+ * ------------------------------------------------------------------------------------------- */
+size_t g1_spmv_merge_workspace_bytes(uint32_t n_rows, uint32_t nnz);
+g1_status g1_spmv_merge_partition(uint32_t n_rows, uint32_t nnz, const uint32_t* row_offsets_dev,
+                                  void* workspace_dev, size_t workspace_bytes, g1_stream_t stream);
+g1_status g1_spmv_merge_minplus_u32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz,
+                                    const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                                    const uint32_t* values_dev, const uint32_t* x_dev, uint32_t* y_dev,
+                                    void* workspace_dev, size_t workspace_bytes, g1_stream_t stream);
+g1_status g1_spmv_merge_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz,
+                                      const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                                      const float* values_dev, const float* x_dev, float* y_dev,
+                                      void* workspace_dev, size_t workspace_bytes, g1_stream_t stream);
+
+/* CSR-vector: a vector of L lanes (L in 1..64, a power of two) per row inside a wave64, L chosen from
+ * nnz / n_rows unless lanes > 0 forces it. Replaces SpmvDispatcher<SpmvCudaCsrVector>
+ * (include/GraphOne/archi/blas/spmv/cuda/cuda.cuh:142-193, csr_vector.cuh:20-68), written from its
+ * definition (the reference kernel does not compile and is never instantiated). */
+uint32_t g1_spmv_csrvec_lanes(uint32_t n_rows, uint32_t nnz);
+g1_status g1_spmv_csrvec_minplus_u32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz,
+                                     const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                                     const uint32_t* values_dev, const uint32_t* x_dev, uint32_t* y_dev,
+                                     uint32_t lanes, g1_stream_t stream);
+g1_status g1_spmv_csrvec_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz,
+                                       const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                                       const float* values_dev, const float* x_dev, float* y_dev,
+                                       uint32_t lanes, g1_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Host-side synthetic graphs (seeded, splitmix64; the RNG stream is specified in DESIGN notes of the
+ * repository).  They produce the CSR invariants of GraphLoader + EdgeCache::ToCsr
+ * (include/GraphOne/loader/graph_loader.h:254-296, include/GraphOne/loader/edge_cache.h:50-105):
+ * no self loops, no duplicate (src,dst), columns ascending within a row, n = number of vertex ids.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct g1_host_csr {
+    uint32_t n;
+    uint32_t m;
+    uint32_t* row_offsets; /* n + 1 */
+    uint32_t* col_indices; /* m */
+    uint32_t* weights;     /* m, or NULL */
+} g1_host_csr;
+
+/* R-MAT: 2^scale vertices, edge_factor * 2^scale raw directed edges, quadrant probabilities a,b,c
+ * (d = 1-a-b-c); self loops and duplicates removed. undirected != 0 adds every reverse edge first. */
+g1_status g1_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                      uint64_t seed, int32_t undirected, g1_host_csr* out);
+/* rows x cols 4-neighbour grid, vertex id r*cols + c; each undirected grid edge kept with probability
+ * p_keep, stored in both directions. */
+g1_status g1_gen_grid(uint32_t rows, uint32_t cols, double p_keep, uint64_t seed, g1_host_csr* out);
+/* Builds a CSR from an edge list (src[i] -> dst[i]) with the EdgeCache rules: self loops dropped unless
+ * keep_self, reverse edges added unless directed, (src,dst) duplicates dropped unless keep_dup (the
+ * first occurrence in input order keeps its weight). weights may be NULL. */
+g1_status g1_csr_from_edges(uint32_t n, uint64_t num_edges, const uint32_t* src, const uint32_t* dst,
+                            const uint32_t* weights, int32_t directed, int32_t keep_self,
+                            int32_t keep_dup, g1_host_csr* out);
+/* u32 weights uniform in [lo, hi] from a hash of (seed, src, dst); symmetric != 0 hashes the unordered
+ * pair so both directions of an undirected edge get the same weight. Allocates out->weights. */
+g1_status g1_assign_weights_u32(g1_host_csr* csr, uint32_t lo, uint32_t hi, uint64_t seed,
+                                int32_t symmetric);
+/* Transpose (in-edge CSR): rows are destinations, columns ascending source ids; weights follow. */
+g1_status g1_host_csr_transpose(const g1_host_csr* in, g1_host_csr* out);
+void g1_host_csr_free(g1_host_csr* csr);
+
+/* ---------------------------------------------------------------------------------------------
+ * Device-resident graph.  Replaces graph::build<arch, views> + Graph/GraphArchRef
+ * (include/GraphOne/graph/builder.h:9-46, graph.h:13-140, graph_arch_ref.h:19-116): the out-edge CSR
+ * ("normal" view) and optionally its transpose (the view BlasViews uses, engine/blas_gc.h:438).
+ * The graph also owns the work areas its engines reuse across runs; runs on one graph object must
+ * therefore not overlap in time.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct g1_graph g1_graph;
+#define G1_GRAPH_TRANSPOSE 1u  /* also keep the in-edge CSR (needed by pull / BLAS engines) */
+#define G1_GRAPH_SYMMETRIC 2u  /* the CSR is symmetric: the transpose aliases it            */
+
+g1_status g1_graph_create(g1_graph** graph, const g1_host_csr* csr, uint32_t flags);
+g1_status g1_graph_destroy(g1_graph* graph);
+g1_status g1_graph_info(const g1_graph* graph, uint32_t* n, uint32_t* m, uint64_t* device_bytes);
+g1_status g1_graph_csr_dev(const g1_graph* graph, int32_t transpose, const uint32_t** row_offsets_dev,
+                           const uint32_t** col_indices_dev, const uint32_t** weights_dev);
+
+/* ---------------------------------------------------------------------------------------------
+ * Whole-algorithm runs (the Run<F>(comp, frontier) loop, include/GraphOne/base/run.h:8-39, with the
+ * functor fixed).  Engines:
+ *   G1_ENGINE_ADVANCE      push advance + fused filter/compaction (engine/advance_gc.h:162-283)
+ *   G1_ENGINE_BLAS_MERGE   pull semiring SpMV engine, merge-path kernel (engine/blas_gc.h:171-351)
+ *   G1_ENGINE_BLAS_CSRVEC  the same engine over the CSR-vector kernel
+ *   G1_ENGINE_DIROPT       BFS only: queue/bitmap switching push<->pull by frontier density
+ * ------------------------------------------------------------------------------------------- */
+#define G1_ENGINE_ADVANCE 0
+#define G1_ENGINE_BLAS_MERGE 1
+#define G1_ENGINE_BLAS_CSRVEC 2
+#define G1_ENGINE_DIROPT 3
+
+typedef struct g1_run_stats {
+    uint32_t iterations;       /* engine iterations (Forward calls)                         */
+    uint32_t push_iterations;  /* iterations run top-down (advance)                          */
+    uint32_t pull_iterations;  /* iterations run bottom-up / SpMV                            */
+    uint32_t reserved;
+    uint64_t frontier_sum;     /* sum over iterations of |F_i| (vertices activated)          */
+    uint64_t edges_sum;        /* sum over iterations of E(F_i) (push) or nnz scanned (pull) */
+    uint64_t reached;          /* vertices with a finite result                              */
+    uint64_t reached_edges;    /* sum of out-degrees of reached vertices (GTEPS numerator)   */
+    double elapsed_ms;         /* device time of the run                                     */
+    double last_delta;         /* PageRank: last L1 (or max) delta                           */
+} g1_run_stats;
+
+/* BFS depths (hops), depth_dev[src] = 0, unreached = G1_INF. */
+g1_status g1_bfs(g1_graph* graph, uint32_t src, uint32_t* depth_dev, int32_t engine,
+                 g1_run_stats* stats, g1_stream_t stream);
+/* SSSP distances over u32 weights (min-plus), dist_dev[src] = 0, unreached = G1_INF.
+ * Relaxation is atomicMin on uint32 (apps/sssp_adv.cu:49-60). */
+g1_status g1_sssp(g1_graph* graph, uint32_t src, uint32_t* dist_dev, int32_t engine,
+                  g1_run_stats* stats, g1_stream_t stream);
+
+/* PageRank (apps/pagerank_blas.cu:16-125): iw[u] = alpha / sum_out w (0 if none), r0 = 1/N,
+ * r' = (1-alpha)/N + sum_{u->v} w * r[u] * iw[u]; no dangling redistribution.
+ * FP32 values, Kahan row sums. converge_mode:
+ *   G1_PR_L1_F64   stop when the FP64 L1 norm of r'-r is < tol or after max_iter iterations
+ *   G1_PR_MAX_ABS  the reference's test: before each iteration stop if max|r_cur - r_pre| <= tol
+ *                  (the first test compares r0 with a zero vector, as apps/pagerank_blas.cu:63-78)
+ * rank_dev receives the final iterate (for G1_PR_MAX_ABS: the array the reference prints,
+ * ranks[opt_idx], i.e. the iterate before the last). */
+#define G1_PR_L1_F64 0
+#define G1_PR_MAX_ABS 1
+g1_status g1_pagerank(g1_graph* graph, float alpha, uint32_t max_iter, double tol,
+                      int32_t converge_mode, int32_t engine, float* rank_dev, g1_run_stats* stats,
+                      g1_stream_t stream);
+
+/* Personalized PageRank from one seed: r0 = e_seed, r' = restart * e_seed + sum_{u->v} w*r[u]*iw[u]
+ * with iw[u] = (1-restart)/sum_out w. Stops when FP64 L1(r'-r) < tol or after max_iter. */
+g1_status g1_ppr(g1_graph* graph, uint32_t seed_vertex, float restart, uint32_t max_iter, double tol,
+                 float* rank_dev, g1_run_stats* stats, g1_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAPHONE_HIP_H */

@@ -1,0 +1,59 @@
+"""The C-ABI library loads and exports every symbol include/graphone_hip.h declares; argument errors are
+reported with status codes before any GPU work (no GPU needed)."""
+import ctypes as C
+import subprocess
+
+import numpy as np
+import pytest
+
+import graphone_amd as g1
+from graphone_amd import _lib
+
+
+def test_every_declared_symbol_is_exported():
+    declared = _lib.declared_symbols()
+    assert len(declared) >= 40
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    # and every binding of the Python layer names a declared symbol
+    assert set(_lib._SIGNATURES) <= set(declared)
+
+
+def test_status_strings_and_version():
+    L = _lib.lib()
+    assert L.g1_abi_version() == 1
+    assert L.g1_status_string(0) == b"ok"
+    assert L.g1_status_string(4) == b"HBM budget exceeded"
+
+
+def test_generator_argument_errors():
+    L = _lib.lib()
+    csr = _lib.HostCSR()
+    assert L.g1_gen_rmat(0, 16, 0.57, 0.19, 0.19, 1, 0, C.byref(csr)) == _lib.G1_ERR_INVALID_ARG
+    assert b"scale" in L.g1_last_error_message()
+    assert L.g1_gen_rmat(5, 16, 0.7, 0.3, 0.3, 1, 0, C.byref(csr)) == _lib.G1_ERR_INVALID_ARG
+    assert L.g1_gen_grid(0, 5, 0.5, 1, C.byref(csr)) == _lib.G1_ERR_INVALID_ARG
+
+
+def test_graph_create_validates_csr_on_host():
+    bad = g1.HostGraph.from_csr(np.array([0, 2, 1], np.uint32), np.array([1, 0], np.uint32))
+    with pytest.raises(g1.G1Error) as e:
+        g1.Graph(bad)
+    assert e.value.status == _lib.G1_ERR_INVALID_ARG
+    bad_col = g1.HostGraph.from_csr(np.array([0, 1, 1], np.uint32), np.array([7], np.uint32))
+    with pytest.raises(g1.G1Error):
+        g1.Graph(bad_col)
+
+
+def test_merge_workspace_size_formula():
+    L = _lib.lib()
+    assert L.g1_spmv_merge_workspace_bytes(0, 0) > 0
+    a = L.g1_spmv_merge_workspace_bytes(4194304, 65000000)
+    tiles = (4194304 + 65000000 + 2047) // 2048
+    assert a >= (tiles + 1) * 8 + tiles * 32
+    assert L.g1_spmv_csrvec_lanes(4194304, 65000000) == 16
+    assert L.g1_spmv_csrvec_lanes(45369, 108374) == 2  # integer average 2
+    assert L.g1_spmv_csrvec_lanes(10, 5000) == 64
