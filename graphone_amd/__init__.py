@@ -240,6 +240,23 @@ class Graph:
         call("g1_graph_csr_dev", self._h, int(transpose), C.byref(ro), C.byref(ci), C.byref(w))
         return ro.value, ci.value, w.value
 
+    def profile(self, enable: bool = True) -> None:
+        """Per-kernel HIP-event timing of every engine launch (see read_profile)."""
+        call("g1_profile_enable", self._h, int(enable))
+
+    def read_profile(self) -> list[dict]:
+        """Kernel entries accumulated since the last read: launches, device ms, algorithmic bytes."""
+        cap = 32
+        arr = (_lib.KernelProfile * cap)()
+        cnt = C.c_uint32()
+        call("g1_profile_read", self._h, arr, cap, C.byref(cnt))
+        out = []
+        for i in range(min(cnt.value, cap)):
+            e = arr[i]
+            out.append({"name": e.name.decode(), "launches": e.launches, "total_ms": e.total_ms,
+                        "algorithmic_bytes": e.algorithmic_bytes})
+        return out
+
     def _run_u32(self, fn: str, src: int, engine, out: DeviceArray | None, stream):
         d = out if out is not None else DeviceArray(self.n, np.uint32)
         st = RunStats()

@@ -68,6 +68,15 @@ class RunStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class KernelProfile(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 40),
+        ("launches", C.c_uint64),
+        ("total_ms", C.c_double),
+        ("algorithmic_bytes", C.c_double),
+    ]
+
+
 _u32 = C.c_uint32
 _u64 = C.c_uint64
 _i32 = C.c_int32
@@ -112,6 +121,8 @@ _SIGNATURES = {
     "g1_graph_destroy": (_st, [_vp]),
     "g1_graph_info": (_st, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u64)]),
     "g1_graph_csr_dev": (_st, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "g1_profile_enable": (_st, [_vp, _i32]),
+    "g1_profile_read": (_st, [_vp, C.POINTER(KernelProfile), _u32, C.POINTER(_u32)]),
     "g1_bfs": (_st, [_vp, _u32, _vp, _i32, C.POINTER(RunStats), _vp]),
     "g1_sssp": (_st, [_vp, _u32, _vp, _i32, C.POINTER(RunStats), _vp]),
     "g1_pagerank": (_st, [_vp, C.c_float, _u32, C.c_double, _i32, _i32, _vp, C.POINTER(RunStats), _vp]),

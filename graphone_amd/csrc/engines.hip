@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "g1/apply.hpp"
 #include "g1/push.hpp"
@@ -45,9 +46,15 @@ __global__ void clear_u64_kernel(unsigned long long* p, uint32_t count) {
 }
 
 g1_status finish_stats(g1_graph* g, const uint32_t* d, g1_run_stats* st, hipStream_t s) {
+    G1_TRY(ensure_push_work(g));  // the stats slots live with the push work areas
     clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr + 4, 2);
     G1_LAUNCHED();
-    reached_stats_kernel<<<grid_for(g->n, 256, persistent_cap()), 256, 0, s>>>(g->n, d, g->row_off, g->ctr + 4);
+    reached_stats_kernel<<<grid_for(g->n, 256, kMaxPushBlocks), 256, 0, s>>>(g->n, d, g->row_off, g->slots,
+                                                                            g->slots + kMaxPushBlocks);
+    G1_LAUNCHED();
+    sum_slots_kernel<<<1, 256, 0, s>>>(g->slots, kMaxPushBlocks, g->ctr + 4);
+    G1_LAUNCHED();
+    sum_slots_kernel<<<1, 256, 0, s>>>(g->slots + kMaxPushBlocks, kMaxPushBlocks, g->ctr + 5);
     G1_LAUNCHED();
     G1_TRY(read_ctr(g, s, 4, 2));
     st->reached = g->h_ctr[4];
@@ -69,13 +76,100 @@ PushQueues make_queues(g1_graph* g, uint32_t k) {
     PushQueues pq;
     pq.q_in = g->q[k & 1];
     pq.qoff_in = g->qoff[k & 1];
-    pq.cnt_in = g->ctr + (k % 3);
+    pq.cnt_in = g->qcnt + (k % 3) * kSegments;
     pq.q_out = g->q[(k + 1) & 1];
     pq.qoff_out = g->qoff[(k + 1) & 1];
-    pq.cnt_out = g->ctr + ((k + 1) % 3);
-    pq.cnt_clear = g->ctr + ((k + 2) % 3);
-    pq.activated = g->ctr + 3;
+    pq.cnt_out = g->qcnt + ((k + 1) % 3) * kSegments;
+    pq.cnt_clear = g->qcnt + ((k + 2) % 3) * kSegments;
+    pq.stats = g->slots;
+    pq.overflow = g->overflow;
+    pq.cap = g->seg_cap;
     return pq;
+}
+
+// Reads the kSegments counters of counter set `set` and returns (E << 32) | nf of that frontier.
+g1_status read_frontier(g1_graph* g, uint32_t set, hipStream_t s, unsigned long long* packed) {
+    G1_HIP(hipMemcpyAsync(g->h_ctr + 16, g->qcnt + set * kSegments, kSegments * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    G1_HIP(hipStreamSynchronize(s));
+    unsigned long long nf = 0, E = 0;
+    for (int i = 0; i < kSegments; ++i) {
+        nf += static_cast<uint32_t>(g->h_ctr[16 + i]);
+        E += g->h_ctr[16 + i] >> 32;
+    }
+    G1_CHECK(E < 0xFFFFFFFFull, G1_ERR_INTERNAL, "frontier edge count overflow");
+    *packed = (E << 32) | nf;
+    return G1_OK;
+}
+
+// Folds the per-workgroup stats slots into ctr[3] (activated) and ctr[7] (scanned) and reads them.
+g1_status fold_slots(g1_graph* g, hipStream_t s, bool read) {
+    sum_slots_kernel<<<1, 256, 0, s>>>(g->slots, kMaxPushBlocks, g->ctr + 3);
+    G1_LAUNCHED();
+    sum_slots_kernel<<<1, 256, 0, s>>>(g->slots + kMaxPushBlocks, kMaxPushBlocks, g->ctr + 7);
+    G1_LAUNCHED();
+    if (read) G1_TRY(read_ctr(g, s, 0, 8));
+    return G1_OK;
+}
+
+g1_status check_overflow(g1_graph* g, hipStream_t s) {
+    uint32_t flag = 0;
+    G1_HIP(hipMemcpyAsync(g->h_ctr + 8, g->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    G1_HIP(hipStreamSynchronize(s));
+    memcpy(&flag, g->h_ctr + 8, sizeof(flag));
+    G1_CHECK(flag == 0, G1_ERR_INTERNAL, "frontier queue segment overflow");
+    return G1_OK;
+}
+
+// Device-driven iteration loop of the push engines. Every launch reads its input frontier's counters
+// on the device and is a no-op once the frontier is empty, so the host enqueues iterations in batches
+// (2, 4, 8, ... up to 32) and reads back only the output counters of the last launch of a batch.
+// With profiling on, the batch is one launch so each launch's counts can be attributed to it.
+template <typename Launch, typename AfterLaunch>
+g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& after, uint32_t* launched) {
+    uint32_t it = 0, batch = 2;
+    for (;;) {
+        const uint32_t b = g->profile ? 1u : batch;
+        G1_CHECK(it + b <= kHistIters, G1_ERR_INTERNAL, "push engine exceeded %u iterations", kHistIters);
+        for (uint32_t k = 0; k < b; ++k) {
+            G1_TRY(launch(it));
+            G1_TRY(after(it));
+            ++it;
+        }
+        unsigned long long next = 0;
+        G1_TRY(read_frontier(g, it % 3, s, &next));
+        if (static_cast<uint32_t>(next) == 0) break;
+        if (batch < 32) batch *= 2;
+    }
+    *launched = it;
+    return G1_OK;
+}
+
+// Reads the history of `launched` launches and fills iteration statistics: the iterations that had a
+// non-empty input, their frontier entries and edges.
+g1_status read_history(g1_graph* g, uint32_t launched, hipStream_t s, g1_run_stats* st, bool count_edges) {
+    std::vector<unsigned long long> h(2ull * launched);
+    G1_HIP(hipMemcpyAsync(h.data(), g->hist, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    G1_HIP(hipStreamSynchronize(s));
+    uint32_t iters = 0;
+    for (uint32_t i = 0; i < launched; ++i) {
+        const uint32_t nf = static_cast<uint32_t>(h[2 * i]);
+        if (nf == 0) break;
+        ++iters;
+        const bool pull = (h[2 * i + 1] >> 63) != 0;
+        if (pull) ++st->pull_iterations; else ++st->push_iterations;
+        if (count_edges && !pull) st->edges_sum += h[2 * i] >> 32;
+    }
+    st->iterations = iters;
+    return G1_OK;
+}
+
+g1_status seed_push(g1_graph* g, uint32_t src, hipStream_t s) {
+    push_seed_kernel<<<1, 256, 0, s>>>(g->row_off, src, g->q[0], g->qoff[0], g->qcnt);
+    G1_LAUNCHED();
+    clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr, 8);
+    G1_LAUNCHED();
+    return read_ctr(g, s, 0, 8);  // also syncs the host mirror with the cleared counters
 }
 
 g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, g1_run_stats* st, hipStream_t s) {
@@ -86,102 +180,114 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
     G1_TRY(g1_fill_u32(depth, kInf, n, s));
     set_u32_kernel<<<1, 64, 0, s>>>(depth, src, 0u);
     G1_LAUNCHED();
-    push_seed_kernel<<<1, 64, 0, s>>>(g->row_off, src, g->q[0], g->qoff[0], g->ctr);
-    G1_LAUNCHED();
-    clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr + 3, 1);
-    G1_LAUNCHED();
-    int fbi = 0;  // bitmap holding the current frontier
     if (diropt) {
-        G1_TRY(g1_fill_u32(g->fb[0], 0u, words + 1, s));
-        set_u32_kernel<<<1, 64, 0, s>>>(g->fb[0], src >> 5, 1u << (src & 31u));
+        G1_TRY(g1_fill_u32(g->fb3, 0u, 2ull * words, s));
+        set_u32_kernel<<<1, 64, 0, s>>>(g->fb3, src >> 5, 1u << (src & 31u));
         G1_LAUNCHED();
     }
-    G1_TRY(read_ctr(g, s, 0, 1));
+    G1_TRY(seed_push(g, src, s));
     G1_HIP(hipEventRecord(g->ev[0], s));
-    unsigned long long cur = g->h_ctr[0];
-    uint64_t explored = 0, activated_prev = 0, frontier_sum = 1;
-    bool pull = false;
-    uint32_t it = 0;
-    const uint32_t cap = persistent_cap();
-    uint64_t nf_active = 1;  // vertices in the current frontier (including those without out-edges)
-    while (static_cast<uint32_t>(cur) != 0 && it < kMaxIterations) {
-        const uint32_t nf = static_cast<uint32_t>(cur);
-        const uint32_t E = static_cast<uint32_t>(cur >> 32);
+    int tok = -1;
+    auto launch = [&](uint32_t it) -> g1_status {
         PushQueues pq = make_queues(g, it);
+        tok = prof_begin(g, diropt ? "bfs_step_diropt" : "bfs_advance", s);
         if (diropt) {
-            // Beamer's switch: pull when the frontier's out-edges exceed 1/14 of the edges still
-            // unexplored; push again when the frontier falls below n/24 vertices.
-            const uint64_t unexplored = g->m > explored + E ? g->m - explored - E : 0;
-            if (!pull && static_cast<uint64_t>(E) * 14 > unexplored) pull = true;
-            else if (pull && nf_active * 24 < n) pull = false;
-        }
-        if (pull) {
-            bfs_pull_kernel<<<grid_for(n, kPushThreads, cap), kPushThreads, 0, s>>>(
-                n, g->t_row_off, g->t_col, g->row_off, depth, g->fb[fbi], g->fb[fbi ^ 1], it + 1, pq);
-            G1_LAUNCHED();
-            ++st->pull_iterations;
+            bfs_step_kernel<true><<<kMaxPushBlocks, kPushThreads, 0, s>>>(
+                n, g->m, g->row_off, g->col, g->t_row_off, g->t_col, depth, g->fb3, words, pq, it, g->hist);
         } else {
-            if (diropt) G1_TRY(g1_fill_u32(g->fb[fbi ^ 1], 0u, words + 1, s));
-            BfsPush pol{depth, diropt ? g->fb[fbi ^ 1] : nullptr, it + 1};
-            push_advance_kernel<BfsPush><<<grid_for(E, kPushTile, cap), kPushThreads, 0, s>>>(
-                g->row_off, g->col, nullptr, pq, pol);
-            G1_LAUNCHED();
-            ++st->push_iterations;
-            st->edges_sum += E;
+            bfs_step_kernel<false><<<kMaxPushBlocks, kPushThreads, 0, s>>>(
+                n, g->m, g->row_off, g->col, g->t_row_off, g->t_col, depth, g->fb3, words, pq, it, g->hist);
         }
-        fbi ^= 1;
-        explored += E;
-        ++it;
-        G1_HIP(hipMemcpyAsync(g->h_ctr, g->ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-        G1_HIP(hipStreamSynchronize(s));
-        cur = g->h_ctr[it % 3];
-        const uint64_t act = g->h_ctr[3];
-        nf_active = act - activated_prev;
-        activated_prev = act;
-        frontier_sum += nf_active;
-        (void)nf;
-    }
+        G1_LAUNCHED();
+        prof_end(g, tok, s, 0.0);
+        return G1_OK;
+    };
+    auto after = [&](uint32_t it) -> g1_status {
+        if (!g->profile) return G1_OK;
+        // attribute this launch's algorithmic bytes from its history record and stats slots
+        const uint64_t act0 = g->h_ctr[3], scan0 = g->h_ctr[7];
+        G1_TRY(fold_slots(g, s, true));
+        unsigned long long h[2];
+        G1_HIP(hipMemcpy(h, g->hist + 2 * it, sizeof(h), hipMemcpyDeviceToHost));
+        const double nf = static_cast<double>(static_cast<uint32_t>(h[0]));
+        const double E = static_cast<double>(h[0] >> 32);
+        const double act = static_cast<double>(g->h_ctr[3] - act0);
+        const double scanned = static_cast<double>(g->h_ctr[7] - scan0);
+        if (nf == 0.0) prof_set_bytes(g, tok, 0.0);
+        else if (h[1] >> 63)  // bottom-up: depth and in-edge offsets of every vertex, scanned in-edges,
+                              // the next bitmap, the new depths
+            prof_set_bytes(g, tok, 4.0 * n + 4.0 * (n + 1.0) + 4.0 * scanned + n / 8.0 + 4.0 * act);
+        else
+            prof_set_bytes(g, tok, 16.0 * nf + 8.0 * E + 4.0 * act);
+        return G1_OK;
+    };
+    uint32_t launched = 0;
+    G1_TRY(drive_push(g, s, launch, after, &launched));
     G1_HIP(hipEventRecord(g->ev[1], s));
     G1_TRY(elapsed(g, st));
-    st->iterations = it;
-    st->frontier_sum = frontier_sum;
+    G1_TRY(prof_flush(g));
+    G1_TRY(fold_slots(g, s, true));
+    G1_TRY(check_overflow(g, s));
+    G1_TRY(read_history(g, launched, s, st, true));
+    st->frontier_sum = 1 + g->h_ctr[3];
+    st->edges_sum += g->h_ctr[7];
     return finish_stats(g, depth, st, s);
+}
+
+__global__ void dstamp_init_kernel(uint32_t n, uint32_t src, unsigned long long* __restrict__ ds) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        ds[v] = (v == src) ? 0ull : (static_cast<unsigned long long>(kInf) << 32);
+}
+
+__global__ void dstamp_extract_kernel(uint32_t n, const unsigned long long* __restrict__ ds,
+                                      uint32_t* __restrict__ dist) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        dist[v] = static_cast<uint32_t>(ds[v] >> 32);
 }
 
 g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s) {
     G1_TRY(ensure_push_work(g));
     G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
     const uint32_t n = g->n;
-    G1_TRY(g1_fill_u32(dist, kInf, n, s));
-    G1_TRY(g1_fill_u32(g->stamp, 0u, n, s));
-    set_u32_kernel<<<1, 64, 0, s>>>(dist, src, 0u);
+    const uint32_t fill_blocks = grid_for(n, 256, kMaxPushBlocks);
+    dstamp_init_kernel<<<fill_blocks, 256, 0, s>>>(n, src, g->dstamp);
     G1_LAUNCHED();
-    push_seed_kernel<<<1, 64, 0, s>>>(g->row_off, src, g->q[0], g->qoff[0], g->ctr);
-    G1_LAUNCHED();
-    clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr + 3, 1);
-    G1_LAUNCHED();
-    G1_TRY(read_ctr(g, s, 0, 1));
+    G1_TRY(seed_push(g, src, s));
     G1_HIP(hipEventRecord(g->ev[0], s));
-    unsigned long long cur = g->h_ctr[0];
-    uint32_t it = 0;
-    const uint32_t cap = persistent_cap();
-    while (static_cast<uint32_t>(cur) != 0 && it < kMaxIterations) {
-        const uint32_t E = static_cast<uint32_t>(cur >> 32);
+    int tok = -1;
+    auto launch = [&](uint32_t it) -> g1_status {
         PushQueues pq = make_queues(g, it);
-        SsspPush pol{dist, g->stamp, it + 1};
-        push_advance_kernel<SsspPush><<<grid_for(E, kPushTile, cap), kPushThreads, 0, s>>>(
-            g->row_off, g->col, g->w, pq, pol);
+        SsspPush pol{g->dstamp, it + 1};
+        tok = prof_begin(g, "sssp_advance", s);
+        push_advance_kernel<SsspPush><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq, pol, it,
+                                                                             g->hist);
         G1_LAUNCHED();
-        st->edges_sum += E;
-        ++it;
-        G1_HIP(hipMemcpyAsync(g->h_ctr, g->ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-        G1_HIP(hipStreamSynchronize(s));
-        cur = g->h_ctr[it % 3];
-    }
+        prof_end(g, tok, s, 0.0);
+        return G1_OK;
+    };
+    auto after = [&](uint32_t it) -> g1_status {
+        if (!g->profile) return G1_OK;
+        const uint64_t act0 = g->h_ctr[3];
+        G1_TRY(fold_slots(g, s, true));
+        unsigned long long h[1];
+        G1_HIP(hipMemcpy(h, g->hist + 2 * it, sizeof(h), hipMemcpyDeviceToHost));
+        const double nf = static_cast<double>(static_cast<uint32_t>(h[0]));
+        const double E = static_cast<double>(h[0] >> 32);
+        prof_set_bytes(g, tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E + 4.0 * static_cast<double>(g->h_ctr[3] - act0));
+        return G1_OK;
+    };
+    uint32_t launched = 0;
+    G1_TRY(drive_push(g, s, launch, after, &launched));
+    dstamp_extract_kernel<<<fill_blocks, 256, 0, s>>>(n, g->dstamp, dist);
+    G1_LAUNCHED();
     G1_HIP(hipEventRecord(g->ev[1], s));
     G1_TRY(elapsed(g, st));
-    st->iterations = it;
-    st->push_iterations = it;
+    G1_TRY(prof_flush(g));
+    G1_TRY(fold_slots(g, s, true));
+    G1_TRY(check_overflow(g, s));
+    G1_TRY(read_history(g, launched, s, st, true));
     st->frontier_sum = 1 + g->h_ctr[3];
     return finish_stats(g, dist, st, s);
 }
@@ -193,8 +299,9 @@ g1_status run_minplus_pull(g1_graph* g, uint32_t src, uint32_t* d, bool weighted
     G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "pull engine needs the graph's transpose");
     if (weighted) G1_CHECK(g->t_w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
     G1_TRY(ensure_pull_work(g, false));
+    G1_TRY(ensure_push_work(g));  // stats slots
     const uint32_t n = g->n;
-    const uint32_t cap = persistent_cap();
+    const uint32_t cap = std::min<uint32_t>(persistent_cap(), kMaxPushBlocks);
     minplus_seed_kernel<<<grid_for(n, 256, cap), 256, 0, s>>>(n, src, d, g->xu);
     G1_LAUNCHED();
     clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr + 6, 1);
@@ -204,13 +311,22 @@ g1_status run_minplus_pull(g1_graph* g, uint32_t src, uint32_t* d, bool weighted
     G1_HIP(hipEventRecord(g->ev[0], s));
     uint64_t prev = 0, frontier_sum = 1;
     uint32_t it = 0;
+    const double spmv_bytes = 4.0 * (n + 1.0) + 4.0 * g->m * (weighted ? 2.0 : 1.0) + 8.0 * n;
     for (;;) {
         if (engine == G1_ENGINE_BLAS_CSRVEC) {
+            const int tok = prof_begin(g, "spmv_csrvec_minplus", s);
             G1_TRY(launch_spmv_csrvec<MinPlusU32>(n, g->t_row_off, g->t_col, vals, g->xu, g->yu, lanes, s));
+            prof_end(g, tok, s, spmv_bytes);
         } else {
+            const int tok = prof_begin(g, "spmv_merge_minplus", s);
             G1_TRY(launch_spmv_merge<MinPlusU32>(n, g->m, g->t_row_off, g->t_col, vals, g->xu, g->yu, g->t_ws, s));
+            prof_end(g, tok, s, spmv_bytes);
         }
-        apply_minplus_kernel<<<grid_for(n, 256, cap), 256, 0, s>>>(n, g->yu, d, g->xu, g->ctr + 6);
+        const int tok = prof_begin(g, "apply_minplus", s);
+        apply_minplus_kernel<<<grid_for(n, 256, cap), 256, 0, s>>>(n, g->yu, d, g->xu, g->slots);
+        G1_LAUNCHED();
+        prof_end(g, tok, s, 12.0 * n);
+        sum_slots_kernel<<<1, 256, 0, s>>>(g->slots, kMaxPushBlocks, g->ctr + 6);
         G1_LAUNCHED();
         ++it;
         st->edges_sum += g->m;
@@ -223,6 +339,7 @@ g1_status run_minplus_pull(g1_graph* g, uint32_t src, uint32_t* d, bool weighted
     }
     G1_HIP(hipEventRecord(g->ev[1], s));
     G1_TRY(elapsed(g, st));
+    G1_TRY(prof_flush(g));
     st->iterations = it;
     st->pull_iterations = it;
     st->frontier_sum = frontier_sum;
@@ -281,16 +398,24 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
         }
         cur = idx;
     } else {
+        const double spmv_bytes = 4.0 * (n + 1.0) + 4.0 * g->m * (vals ? 2.0 : 1.0) + 8.0 * n;
         for (;;) {
             if (it >= max_iter) break;
-            if (engine == G1_ENGINE_BLAS_CSRVEC)
+            int tok;
+            if (engine == G1_ENGINE_BLAS_CSRVEC) {
+                tok = prof_begin(g, "spmv_csrvec_plustimes", s);
                 G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, g->t_row_off, g->t_col, vals, g->xf, g->yf, lanes, s));
-            else
+            } else {
+                tok = prof_begin(g, "spmv_merge_plustimes", s);
                 G1_TRY(launch_spmv_merge<PlusTimesKahanF32>(n, g->m, g->t_row_off, g->t_col, vals, g->xf, g->yf,
                                                             g->t_ws, s));
+            }
+            prof_end(g, tok, s, spmv_bytes);
+            tok = prof_begin(g, "pr_apply", s);
             pr_apply_kernel<0><<<blocks, 256, 0, s>>>(n, g->yf, base_all, seed, base_seed, g->iw, g->r[cur],
                                                       g->r[cur ^ 1], g->xf, g->partials);
             G1_LAUNCHED();
+            prof_end(g, tok, s, 20.0 * n);
             reduce_partials_kernel<0><<<1, 256, 0, s>>>(g->partials, blocks, g->partials + kPartials);
             G1_LAUNCHED();
             cur ^= 1;
@@ -303,6 +428,7 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
     }
     G1_HIP(hipEventRecord(g->ev[1], s));
     G1_TRY(elapsed(g, st));
+    G1_TRY(prof_flush(g));
     G1_HIP(hipMemcpyAsync(out, g->r[cur], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     G1_HIP(hipStreamSynchronize(s));
     st->iterations = it;

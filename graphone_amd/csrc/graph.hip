@@ -2,8 +2,10 @@
 // Graph::ToArch (include/GraphOne/graph/builder.h:9-46, graph.h:94-113): the host CSR is uploaded once
 // and stays resident in HBM; the in-edge CSR is built on the host from the same edges.
 #include <hip/hip_runtime.h>
+#include <stdio.h>
 #include <string.h>
 
+#include "g1/push.hpp"
 #include "g1/spmv_merge.hpp"
 #include "graph.hpp"
 #include "spmv_launch.hpp"
@@ -31,12 +33,24 @@ g1_status ensure_push_work(g1_graph* g) {
     if (g->q[0]) return G1_OK;
     const size_t n = g->n ? g->n : 1;
     const size_t words = (n + 31) / 32 + 1;
+    const uint64_t items = g->m > g->n ? g->m : g->n;
+    const uint64_t cap = push_segment_capacity(n, items);
+    G1_CHECK(cap * kSegments < (1ull << 34), G1_ERR_INVALID_ARG, "push queues too large");
+    g->seg_cap = static_cast<uint32_t>(cap);
     for (int i = 0; i < 2; ++i) {
-        G1_TRY(alloc_n(g, &g->q[i], n));
-        G1_TRY(alloc_n(g, &g->qoff[i], n));
+        G1_TRY(alloc_n(g, &g->q[i], cap * kSegments));
+        G1_TRY(alloc_n(g, &g->qoff[i], cap * kSegments));
         G1_TRY(alloc_n(g, &g->fb[i], words));
     }
     G1_TRY(alloc_n(g, &g->stamp, n));
+    G1_TRY(alloc_n(g, &g->fb3, 3 * words));
+    G1_TRY(alloc_n(g, &g->dstamp, n));
+    G1_TRY(alloc_n(g, &g->hist, 2ull * kHistIters));
+    G1_TRY(alloc_n(g, &g->qcnt, 3 * kSegments));
+    G1_TRY(alloc_n(g, &g->slots, 2 * kMaxPushBlocks));
+    G1_TRY(alloc_n(g, &g->overflow, 4));
+    G1_HIP(hipMemset(g->slots, 0, sizeof(unsigned long long) * 2 * kMaxPushBlocks));
+    G1_HIP(hipMemset(g->overflow, 0, 16));
     return G1_OK;
 }
 
@@ -66,7 +80,59 @@ g1_status ensure_pull_work(g1_graph* g, bool pagerank) {
     return G1_OK;
 }
 
+static hipEvent_t pool_event(g1_graph* g) {
+    if (g->ev_used == g->ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        g->ev_pool.push_back(e);
+    }
+    return g->ev_pool[g->ev_used++];
+}
+
+int prof_begin(g1_graph* g, const char* name, hipStream_t s) {
+    if (!g->profile) return -1;
+    int entry = -1;
+    for (size_t i = 0; i < g->prof.size(); ++i)
+        if (g->prof[i].name == name) entry = static_cast<int>(i);
+    if (entry < 0) {
+        g->prof.push_back(g1_graph::ProfEntry{name, 0, 0.0, 0.0});
+        entry = static_cast<int>(g->prof.size() - 1);
+    }
+    hipEvent_t a = pool_event(g), b = pool_event(g);
+    if (!a || !b) return -1;
+    (void)hipEventRecord(a, s);
+    g->pending.push_back(g1_graph::ProfPending{entry, a, b, 0.0});
+    return static_cast<int>(g->pending.size() - 1);
+}
+
+void prof_end(g1_graph* g, int token, hipStream_t s, double bytes) {
+    if (token < 0) return;
+    (void)hipEventRecord(g->pending[token].b, s);
+    g->pending[token].bytes = bytes;
+}
+
+void prof_set_bytes(g1_graph* g, int token, double bytes) {
+    if (token >= 0) g->pending[token].bytes = bytes;
+}
+
+g1_status prof_flush(g1_graph* g) {
+    for (auto& p : g->pending) {
+        G1_HIP(hipEventSynchronize(p.b));
+        float ms = 0.0f;
+        G1_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        auto& e = g->prof[p.entry];
+        e.launches += 1;
+        e.ms += ms;
+        e.bytes += p.bytes;
+    }
+    g->pending.clear();
+    g->ev_used = 0;
+    return G1_OK;
+}
+
 static void graph_free_all(g1_graph* g) {
+    for (auto e : g->ev_pool) (void)hipEventDestroy(e);
+    g->ev_pool.clear();
     for (void* p : g->owned) (void)hipFree(p);
     g->owned.clear();
     if (g->h_ctr) (void)hipHostFree(g->h_ctr);
@@ -146,7 +212,7 @@ g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags
     }
     if ((s = alloc_n(g, &g->ctr, 16)) != G1_OK) return fail(s);
     if ((s = alloc_n(g, &g->partials, kPartials + 8)) != G1_OK) return fail(s);
-    if (hipHostMalloc(reinterpret_cast<void**>(&g->h_ctr), 16 * sizeof(unsigned long long)) != hipSuccess ||
+    if (hipHostMalloc(reinterpret_cast<void**>(&g->h_ctr), 256 * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&g->h_scalar), 8 * sizeof(double)) != hipSuccess)
         return fail(set_error(G1_ERR_HIP, "g1_graph_create: pinned host allocation failed"));
     if (hipEventCreate(&g->ev[0]) != hipSuccess || hipEventCreate(&g->ev[1]) != hipSuccess)
@@ -161,6 +227,30 @@ g1_status g1_graph_destroy(g1_graph* g) {
     (void)hipDeviceSynchronize();
     g1::graph_free_all(g);
     delete g;
+    return G1_OK;
+}
+
+g1_status g1_profile_enable(g1_graph* g, int32_t enable) {
+    G1_CHECK(g != nullptr, G1_ERR_INVALID_ARG, "g1_profile_enable: null graph");
+    std::lock_guard<std::mutex> lock(g->mu);
+    g->profile = enable != 0;
+    return G1_OK;
+}
+
+g1_status g1_profile_read(g1_graph* g, g1_kernel_profile* out, uint32_t capacity, uint32_t* count) {
+    G1_CHECK(g != nullptr, G1_ERR_INVALID_ARG, "g1_profile_read: null graph");
+    std::lock_guard<std::mutex> lock(g->mu);
+    G1_TRY(g1::prof_flush(g));
+    const uint32_t k = static_cast<uint32_t>(g->prof.size());
+    if (count) *count = k;
+    for (uint32_t i = 0; i < k && i < capacity && out; ++i) {
+        memset(&out[i], 0, sizeof(out[i]));
+        snprintf(out[i].name, sizeof(out[i].name), "%s", g->prof[i].name.c_str());
+        out[i].launches = g->prof[i].launches;
+        out[i].total_ms = g->prof[i].ms;
+        out[i].algorithmic_bytes = g->prof[i].bytes;
+    }
+    g->prof.clear();
     return G1_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "internal.hpp"
@@ -31,7 +32,14 @@ struct g1_graph {
     uint32_t* q[2] = {nullptr, nullptr};
     uint32_t* qoff[2] = {nullptr, nullptr};
     uint32_t* stamp = nullptr;
+    uint32_t seg_cap = 0;                       // entries per queue segment
+    unsigned long long* qcnt = nullptr;         // 3 rotating sets of kSegments queue counters
+    unsigned long long* slots = nullptr;        // per-workgroup stats slots (2 * kMaxPushBlocks)
+    uint32_t* overflow = nullptr;
     uint32_t* fb[2] = {nullptr, nullptr};
+    uint32_t* fb3 = nullptr;                    // three rotating frontier bitmaps (device-driven BFS)
+    unsigned long long* dstamp = nullptr;       // SSSP (dist << 32 | stamp) words
+    unsigned long long* hist = nullptr;         // per-iteration history of the push engines
     unsigned long long* ctr = nullptr;  // [0..2] frontier counters, [3] activated, [4..5] reached stats, [6] pull count
     // pull work areas
     uint32_t* xu = nullptr;
@@ -41,9 +49,27 @@ struct g1_graph {
     float* iw = nullptr;
     float* r[2] = {nullptr, nullptr};
     double* partials = nullptr;  // per-block partial sums, then the folded scalar at [kPartials]
-    unsigned long long* h_ctr = nullptr;  // pinned host mirror of ctr
+    unsigned long long* h_ctr = nullptr;  // pinned host mirror of ctr (16) and a counter set (at [16])
     double* h_scalar = nullptr;           // pinned host scalar
     hipEvent_t ev[2] = {nullptr, nullptr};
+
+    // per-kernel profile (g1_profile_enable / g1_profile_read)
+    struct ProfEntry {
+        std::string name;
+        uint64_t launches = 0;
+        double ms = 0.0;
+        double bytes = 0.0;
+    };
+    struct ProfPending {
+        int entry;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    bool profile = false;
+    std::vector<ProfEntry> prof;
+    std::vector<ProfPending> pending;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
 
     uint64_t device_bytes = 0;
     std::vector<void*> owned;  // every device allocation, freed by g1_graph_destroy
@@ -52,10 +78,19 @@ struct g1_graph {
 
 namespace g1 {
 
-constexpr uint32_t kPartials = 4096;  // max blocks of the apply kernels (and partial slots)
+constexpr uint32_t kPartials = 4096;     // max blocks of the apply kernels (and partial slots)
+constexpr uint32_t kHistIters = 1u << 17;  // iterations a push run may take
 
 g1_status graph_alloc(g1_graph* g, void** p, size_t bytes);
 g1_status ensure_push_work(g1_graph* g);
 g1_status ensure_pull_work(g1_graph* g, bool pagerank);
+
+// Profiling: prof_begin records a start event (returns -1 when profiling is off), prof_end the stop
+// event with the launch's algorithmic bytes; prof_flush (after the stream is synchronized) adds the
+// elapsed times to the per-kernel entries.
+int prof_begin(g1_graph* g, const char* name, hipStream_t s);
+void prof_end(g1_graph* g, int token, hipStream_t s, double bytes);
+void prof_set_bytes(g1_graph* g, int token, double bytes);
+g1_status prof_flush(g1_graph* g);
 
 }  // namespace g1

@@ -12,10 +12,12 @@ namespace g1 {
 
 // Min-plus apply (apps/sssp_blas.cu:75-82 + blas_gc.h:43-55):
 //   pred = y[v] != INF && y[v] < d[v];  if pred: d[v] = y[v];  x[v] = pred ? d[v] : INF
-// count += pred (order free, so the count is exact).
+// The number of applied vertices goes to this block's slot (slots[blockIdx.x] += count), never to a
+// shared counter: device-scope atomics on one address serialize across the whole GPU.
 static __global__ void __launch_bounds__(256) apply_minplus_kernel(uint32_t n, const uint32_t* __restrict__ y,
-                                                            uint32_t* __restrict__ d, uint32_t* __restrict__ x,
-                                                            unsigned long long* __restrict__ count) {
+                                                                   uint32_t* __restrict__ d, uint32_t* __restrict__ x,
+                                                                   unsigned long long* __restrict__ slots) {
+    __shared__ uint32_t s_red[256 / kWave];
     uint32_t local = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
@@ -32,7 +34,12 @@ static __global__ void __launch_bounds__(256) apply_minplus_kernel(uint32_t n, c
         x[v] = xv;
     }
     const uint32_t w = wave_sum(local);
-    if (lane_id() == 0 && w) atomicAdd(count, static_cast<unsigned long long>(w));
+    if (lane_id() == 0) s_red[threadIdx.x / kWave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+        if (t) slots[blockIdx.x] += t;
+    }
 }
 
 // x = INF everywhere except x[src] = 0 (the sparse engine's Construct for the frontier {src}),
@@ -159,10 +166,13 @@ static __global__ void __launch_bounds__(256) max_abs_diff_kernel(uint32_t n, co
     }
 }
 
-// Reached-vertex statistics: count of d[v] != INF and the sum of their out-degrees.
+// Reached-vertex statistics: per block, the count of d[v] != INF and the sum of their out-degrees, added
+// to slots_cnt[blockIdx.x] and slots_edges[blockIdx.x].
 static __global__ void __launch_bounds__(256) reached_stats_kernel(uint32_t n, const uint32_t* __restrict__ d,
-                                                            const uint32_t* __restrict__ row_off,
-                                                            unsigned long long* __restrict__ out2) {
+                                                                   const uint32_t* __restrict__ row_off,
+                                                                   unsigned long long* __restrict__ slots_cnt,
+                                                                   unsigned long long* __restrict__ slots_edges) {
+    __shared__ unsigned long long s_red[2][256 / kWave];
     unsigned long long cnt = 0, edges = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
@@ -174,8 +184,13 @@ static __global__ void __launch_bounds__(256) reached_stats_kernel(uint32_t n, c
     cnt = wave_sum(cnt);
     edges = wave_sum(edges);
     if (lane_id() == 0) {
-        if (cnt) atomicAdd(&out2[0], cnt);
-        if (edges) atomicAdd(&out2[1], edges);
+        s_red[0][threadIdx.x / kWave] = cnt;
+        s_red[1][threadIdx.x / kWave] = edges;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        slots_cnt[blockIdx.x] += s_red[0][0] + s_red[0][1] + s_red[0][2] + s_red[0][3];
+        slots_edges[blockIdx.x] += s_red[1][0] + s_red[1][1] + s_red[1][2] + s_red[1][3];
     }
 }
 

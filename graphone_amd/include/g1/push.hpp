@@ -1,22 +1,27 @@
 // Push-style advance + filter for gfx950, the engine of engine/advance_gc.h:24-283 redesigned.
 //
-// Frontier representation. A frontier is a queue of vertex ids q[0..nf) plus, for every entry, the
-// exclusive prefix sum of the out-degrees qoff[i] = sum_{k<i} deg(q[k]) (so the queue is also the
-// row-end list of a merge path over its E = sum deg edges), plus one packed 64-bit counter
-// cnt = (E << 32) | nf. Entries are appended with ONE 64-bit atomicAdd per workgroup round that
-// reserves queue slots and edge offsets together, so qoff stays sorted in slot order without any scan
-// kernel. Vertices with no out-edge are counted but not queued: they have nothing to advance.
+// Frontier representation (a segmented queue). A frontier is kSegments independent queue segments.
+// Segment s holds vertex ids q[s*cap + 0 .. n_s) and, for each entry, the exclusive prefix sum of the
+// out-degrees inside the segment (qoff); its packed counter cnt[s] = (E_s << 32) | n_s. The frontier is
+// the concatenation of the segments in order, so entry i of the frontier has the global edge offset
+// ebase[s] + qoff[...]: the queue is the row-end list of a merge path over its E = sum E_s edges.
+// Producers append to segment (blockIdx.x % kSegments) with ONE 64-bit atomicAdd per workgroup round that
+// reserves slots and edge offsets together, so no scan kernel is ever needed; spreading the counters
+// over kSegments addresses keeps the device-scope atomics (which serialize per address) off the
+// critical path. Vertices with no out-edge are counted but not queued: they have nothing to advance.
 //
 // Advance. The E edges of the input frontier are cut into tiles of kPushTile consecutive edge slots;
-// a workgroup finds the first queue entry of its tile with one wave-wide 64-way search over qoff,
-// stages the tile's sources (id, first edge, per-source value) in LDS, then maps each edge slot to
-// its source with a binary search in LDS. Every tile holds the same number of edges whatever the
-// degree distribution (a 97k-edge hub is spread over ~95 workgroups), unlike the reference's
-// per-block balancing (advance_gc.h:119-157).
+// each workgroup takes a contiguous run of tiles. A tile finds its first queue entry with a search over
+// the segment bases (LDS) and a wave-wide 64-way search inside the segment, stages the tile's sources in
+// LDS, then maps each edge slot to its source with a binary search in LDS. Every tile holds the same
+// number of edges whatever the degree distribution (a 97k-edge hub is spread over ~48 tiles), unlike the
+// reference's per-block balancing (advance_gc.h:119-157).
 //
-// Filter fused with compaction. The user filter (advance_gc.h:212-244) becomes the policy's relax():
-// it returns true exactly once per vertex and iteration (atomicMin + iteration stamp for SSSP,
-// atomicCAS on the depth for BFS), so the output needs no INVALID slots and no second pass.
+// Filter fused with compaction. The user filter (advance_gc.h:212-244) becomes the policy's relax(): it
+// returns true exactly once per vertex and iteration (atomicMin + iteration stamp for SSSP, atomicCAS on
+// the depth for BFS), so the output needs no INVALID slots and no second pass.
+//
+// Statistics go to per-workgroup slots (stats[blockIdx.x] += ...), never to a shared counter.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,52 +32,107 @@
 namespace g1 {
 
 constexpr int kPushThreads = 256;
-constexpr int kPushEpt = 4;                             // edge slots per thread per tile
-constexpr int kPushTile = kPushThreads * kPushEpt;      // 1024 edge slots per tile
+constexpr int kPushEpt = 8;                          // edge slots per thread per tile
+constexpr int kPushTile = kPushThreads * kPushEpt;   // 2048 edge slots per tile
+constexpr int kSegments = 64;                        // queue segments (one wave scans their counters)
+constexpr uint32_t kMaxPushBlocks = 2048;            // producers per launch (bounds segment fill)
+
+// Entries one segment can receive from one launch of at most kMaxPushBlocks workgroups that each take a
+// contiguous share of `items` (edges for an advance, vertices for a bottom-up step): a segment is fed by
+// at most ceil(G/kSegments) workgroups, each emitting at most one entry per item of its share
+// (ceil(items/G) rounded up to whole tiles), and no segment holds a vertex twice.
+__host__ __device__ inline uint64_t push_segment_capacity(uint64_t n, uint64_t items) {
+    const uint64_t per = (items + kSegments - 1) / kSegments + (items + kMaxPushBlocks - 1) / kMaxPushBlocks +
+                         40ull * kPushTile;
+    return per < n ? per : n;
+}
 
 struct PushQueues {
     const uint32_t* q_in;
     const uint32_t* qoff_in;
-    const unsigned long long* cnt_in;   // (E << 32) | nf of the input frontier
+    const unsigned long long* cnt_in;  // kSegments packed counters of the input frontier
     uint32_t* q_out;
     uint32_t* qoff_out;
-    unsigned long long* cnt_out;        // must be zero on entry
-    unsigned long long* cnt_clear;      // set to zero by the kernel (the counter of the next round)
-    unsigned long long* activated;      // += vertices activated (including those with no out-edge)
+    unsigned long long* cnt_out;       // kSegments counters, zero on entry
+    unsigned long long* cnt_clear;     // kSegments counters zeroed by block 0 (the next round's output)
+    unsigned long long* stats;         // per-workgroup slots: [b] activated, [kMaxPushBlocks + b] scanned
+    uint32_t* overflow;                // set to 1 if a segment would overflow (never expected)
+    uint32_t cap;                      // entries per segment
 };
 
-// Appends the calling thread's candidates (cand[r] != kInf, degree cdeg[r]) to the output queue.
-// Block-uniform control flow; every thread of the block must call it.
+// Segment bases of the input frontier in LDS: vb[s], eb[s] exclusive prefixes of the segment counts
+// and edge totals, vb[kSegments] = nf, eb[kSegments] = E. Called by wave 0 only.
+__device__ __forceinline__ void load_segment_bases(const unsigned long long* __restrict__ cnt, uint32_t* vb,
+                                                   uint32_t* eb) {
+    const uint32_t lane = lane_id();
+    const unsigned long long c = cnt[lane];
+    const uint32_t nv = static_cast<uint32_t>(c), ne = static_cast<uint32_t>(c >> 32);
+    const uint32_t iv = wave_inclusive_sum(nv), ie = wave_inclusive_sum(ne);
+    vb[lane] = iv - nv;
+    eb[lane] = ie - ne;
+    if (lane == kWave - 1) {
+        vb[kSegments] = iv;
+        eb[kSegments] = ie;
+    }
+}
+
+// Appends the calling thread's candidates (cand[r] != kInf, degree cdeg[r]) to segment `seg` of the
+// output queue. Block-uniform control flow; every thread of the block must call it.
 template <int N>
 __device__ __forceinline__ void push_enqueue(const uint32_t (&cand)[N], const uint32_t (&cdeg)[N],
-                                             uint32_t my_cnt, uint32_t my_deg, const PushQueues& pq,
+                                             uint32_t my_cnt, uint32_t my_deg, const PushQueues& pq, uint32_t seg,
                                              unsigned long long* s_scan, unsigned long long* s_base) {
     const unsigned long long mine = (static_cast<unsigned long long>(my_deg) << 32) | my_cnt;
     unsigned long long tot;
     const unsigned long long excl = block_exclusive_sum<kPushThreads>(mine, tot, s_scan);
     if (tot != 0ull) {
-        if (threadIdx.x == 0) *s_base = atomicAdd(pq.cnt_out, tot);
+        if (threadIdx.x == 0) {
+            unsigned long long b = atomicAdd(pq.cnt_out + seg, tot);
+            if (static_cast<uint64_t>(static_cast<uint32_t>(b)) + static_cast<uint32_t>(tot) > pq.cap) {
+                atomicExch(pq.overflow, 1u);
+                b = ~0ull;
+            }
+            *s_base = b;
+        }
         __syncthreads();
-        const unsigned long long base = *s_base + excl;
-        uint32_t slot = static_cast<uint32_t>(base);
-        uint32_t eoff = static_cast<uint32_t>(base >> 32);
+        const unsigned long long sb = *s_base;
+        if (sb != ~0ull) {
+            const unsigned long long base = sb + excl;
+            uint32_t slot = static_cast<uint32_t>(base);
+            uint32_t eoff = static_cast<uint32_t>(base >> 32);
+            uint32_t* qo = pq.q_out + static_cast<size_t>(seg) * pq.cap;
+            uint32_t* oo = pq.qoff_out + static_cast<size_t>(seg) * pq.cap;
 #pragma unroll
-        for (int r = 0; r < N; ++r) {
-            if (cand[r] != kInf) {
-                pq.q_out[slot] = cand[r];
-                pq.qoff_out[slot] = eoff;
-                ++slot;
-                eoff += cdeg[r];
+            for (int r = 0; r < N; ++r) {
+                if (cand[r] != kInf) {
+                    qo[slot] = cand[r];
+                    oo[slot] = eoff;
+                    ++slot;
+                    eoff += cdeg[r];
+                }
             }
         }
         __syncthreads();
     }
 }
 
+// Adds a per-thread count to this workgroup's stats slot (no shared counter).
+__device__ __forceinline__ void block_stat_add(unsigned long long* slot, uint32_t v, uint32_t* s_red) {
+    const uint32_t w = wave_sum(v);
+    if (lane_id() == 0) s_red[threadIdx.x / kWave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < kPushThreads / kWave; ++i) t += s_red[i];
+        if (t) *slot += t;
+    }
+}
+
+// Processes this workgroup's share of the input frontier's edge tiles (see the file comment).
 template <typename P>
-static __global__ void __launch_bounds__(kPushThreads)
-push_advance_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
-                    const uint32_t* __restrict__ wts, PushQueues pq, P pol) {
+__device__ __forceinline__ void advance_tiles(const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+                                              const uint32_t* __restrict__ wts, const PushQueues& pq, const P& pol,
+                                              const uint32_t* s_vb, const uint32_t* s_eb) {
     __shared__ uint32_t s_qoff[kPushTile];
     __shared__ uint32_t s_rs[kPushTile];
     __shared__ uint32_t s_u[kPushTile];
@@ -82,19 +142,24 @@ push_advance_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __rest
     __shared__ unsigned long long s_base;
     __shared__ uint32_t s_i0;
 
-    const unsigned long long packed = *pq.cnt_in;
-    const uint32_t nf = static_cast<uint32_t>(packed);
-    const uint32_t E = static_cast<uint32_t>(packed >> 32);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *pq.cnt_clear = 0ull;
+    const uint32_t nf = s_vb[kSegments];
+    const uint32_t E = s_eb[kSegments];
     const uint32_t tiles = (E + kPushTile - 1) / kPushTile;
+    const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t t_beg = blockIdx.x * per;
+    const uint32_t t_end = (t_beg + per < tiles) ? t_beg + per : tiles;
+    const uint32_t seg_out = blockIdx.x % kSegments;
     uint32_t activated = 0;
 
-    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    for (uint32_t tile = t_beg; tile < t_end; ++tile) {
         const uint32_t e0 = tile * kPushTile;
         const uint32_t e1 = (E - e0 < static_cast<uint32_t>(kPushTile)) ? E : e0 + kPushTile;
         if (threadIdx.x < kWave) {
-            const uint32_t i0 = wave_search_last_le(pq.qoff_in, nf, e0);
-            if (threadIdx.x == 0) s_i0 = i0;
+            const uint32_t s = search_last_le(s_eb, kSegments, e0);
+            const uint32_t* qo = pq.qoff_in + static_cast<size_t>(s) * pq.cap;
+            const uint32_t nseg = s_vb[s + 1] - s_vb[s];
+            const uint32_t loc = wave_search_last_le(qo, nseg, e0 - s_eb[s]);
+            if (threadIdx.x == 0) s_i0 = s_vb[s] + loc;
         }
         __syncthreads();
         const uint32_t i0 = s_i0;
@@ -105,10 +170,12 @@ push_advance_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __rest
             const uint32_t i = i0 + k;
             bool valid = false;
             if (i < nf) {
-                const uint32_t qo = pq.qoff_in[i];
+                const uint32_t s = search_last_le(s_vb, kSegments, i);
+                const size_t at = static_cast<size_t>(s) * pq.cap + (i - s_vb[s]);
+                const uint32_t qo = s_eb[s] + pq.qoff_in[at];
                 valid = (k == 0) || (qo < e1);
                 if (valid) {
-                    const uint32_t u = pq.q_in[i];
+                    const uint32_t u = pq.q_in[at];
                     s_qoff[k] = qo;
                     s_u[k] = u;
                     s_rs[k] = row_off[u];
@@ -145,28 +212,59 @@ push_advance_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __rest
                 }
             }
         }
-        push_enqueue(cand, cdeg, my_cnt, my_deg, pq, s_scan, &s_base);
+        push_enqueue(cand, cdeg, my_cnt, my_deg, pq, seg_out, s_scan, &s_base);
         __syncthreads();
     }
-    const uint32_t wsum = wave_sum(activated);
-    if (lane_id() == 0 && wsum) atomicAdd(pq.activated, static_cast<unsigned long long>(wsum));
+    block_stat_add(pq.stats + blockIdx.x, activated, s_red);
+}
+
+// Common prologue of every push-engine iteration: segment bases of the input frontier into LDS, the
+// counters of the round after next cleared (block 0), and (block 0) the iteration's history record
+// hist[2*it] = (E << 32) | nf. Returns false when the input frontier is empty (the launch is a no-op,
+// which lets the host enqueue iterations ahead without reading the counters back).
+__device__ __forceinline__ bool push_prologue(const PushQueues& pq, uint32_t* s_vb, uint32_t* s_eb) {
+    if (threadIdx.x < kWave) load_segment_bases(pq.cnt_in, s_vb, s_eb);
+    if (blockIdx.x == 0 && threadIdx.x < kSegments) pq.cnt_clear[threadIdx.x] = 0ull;
+    __syncthreads();
+    return s_vb[kSegments] != 0;
+}
+
+template <typename P>
+static __global__ void __launch_bounds__(kPushThreads)
+push_advance_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+                    const uint32_t* __restrict__ wts, PushQueues pq, P pol, uint32_t it,
+                    unsigned long long* __restrict__ hist) {
+    __shared__ uint32_t s_vb[kSegments + 1];
+    __shared__ uint32_t s_eb[kSegments + 1];
+    const bool active = push_prologue(pq, s_vb, s_eb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        hist[2 * it] = (static_cast<unsigned long long>(s_eb[kSegments]) << 32) | s_vb[kSegments];
+        hist[2 * it + 1] = 0ull;
+    }
+    if (!active) return;
+    advance_tiles(row_off, col, wts, pq, pol, s_vb, s_eb);
 }
 
 // SSSP relaxation (apps/sssp_adv.cu:49-71): nd = d[u] + w; atomicMin(&d[v], nd) and keep v when
 // nd < old; the stamp makes v enter the next frontier once per iteration (the visited filter).
+// Distance and stamp share one 64-bit word (dist << 32 | stamp), so ONE atomicMin both lowers the
+// distance and claims the enqueue: a strict improvement installs (nd, iter); the first improvement of
+// this iteration is the one that saw an older stamp. An equal distance never replaces the word (the
+// older stamp is smaller), so a non-improving write cannot claim the vertex.
 struct SsspPush {
     using src_t = uint32_t;
     static constexpr bool kWeighted = true;
-    uint32_t* dist;
-    uint32_t* stamp;
+    unsigned long long* dstamp;
     uint32_t iter;
-    __device__ __forceinline__ uint32_t source_value(uint32_t u) const { return dist[u]; }
+    __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
+        return static_cast<uint32_t>(dstamp[u] >> 32);
+    }
     __device__ __forceinline__ bool relax(uint32_t, uint32_t du, uint32_t v, uint32_t w, uint32_t) const {
         const uint32_t nd = du + w;
-        if (nd >= dist[v]) return false;
-        const uint32_t old = atomicMin(&dist[v], nd);
-        if (nd >= old) return false;
-        return atomicMax(&stamp[v], iter) < iter;
+        if (nd >= static_cast<uint32_t>(dstamp[v] >> 32)) return false;
+        const unsigned long long old = atomicMin(&dstamp[v], (static_cast<unsigned long long>(nd) << 32) | iter);
+        if (nd >= static_cast<uint32_t>(old >> 32)) return false;
+        return static_cast<uint32_t>(old) < iter;
     }
 };
 
@@ -190,24 +288,30 @@ struct BfsPush {
 // BFS bottom-up step over the in-edge CSR: every unvisited vertex looks for an in-neighbour in the
 // current frontier bitmap and stops at the first one. The next frontier is written both as a bitmap
 // (every word fully written by one wave, so it needs no clearing) and as a queue for a following
-// top-down step. Blocks cover contiguous runs of 256 vertices.
-static __global__ void __launch_bounds__(kPushThreads)
-bfs_pull_kernel(uint32_t n, const uint32_t* __restrict__ t_row_off, const uint32_t* __restrict__ t_col,
-                const uint32_t* __restrict__ row_off, uint32_t* __restrict__ depth,
-                const uint32_t* __restrict__ fb_cur, uint32_t* __restrict__ fb_next, uint32_t next_level,
-                PushQueues pq) {
+// top-down step. Each workgroup takes a contiguous run of 256-vertex groups.
+__device__ __forceinline__ void pull_groups(uint32_t n, const uint32_t* __restrict__ t_row_off,
+                                            const uint32_t* __restrict__ t_col, const uint32_t* __restrict__ row_off,
+                                            uint32_t* __restrict__ depth, const uint32_t* __restrict__ fb_cur,
+                                            uint32_t* __restrict__ fb_next, uint32_t next_level, const PushQueues& pq) {
     __shared__ unsigned long long s_scan[kPushThreads / kWave];
     __shared__ unsigned long long s_base;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *pq.cnt_clear = 0ull;
+    __shared__ uint32_t s_red[kPushThreads / kWave];
     uint32_t activated = 0;
+    uint32_t scanned = 0;
     const uint32_t words = (n + 31) / 32;
-    for (uint32_t base = blockIdx.x * kPushThreads; base < n; base += gridDim.x * kPushThreads) {
-        const uint32_t v = base + threadIdx.x;
+    const uint32_t groups = (n + kPushThreads - 1) / kPushThreads;
+    const uint32_t per = (groups + gridDim.x - 1) / gridDim.x;
+    const uint32_t g_beg = blockIdx.x * per;
+    const uint32_t g_end = (g_beg + per < groups) ? g_beg + per : groups;
+    const uint32_t seg_out = blockIdx.x % kSegments;
+    for (uint32_t grp = g_beg; grp < g_end; ++grp) {
+        const uint32_t v = grp * kPushThreads + threadIdx.x;
         bool found = false;
         if (v < n && depth[v] == kInf) {
             const uint32_t b = t_row_off[v], e = t_row_off[v + 1];
             for (uint32_t k = b; k < e; ++k) {
                 const uint32_t u = t_col[k];
+                ++scanned;
                 if ((fb_cur[u >> 5] >> (u & 31u)) & 1u) { found = true; break; }
             }
         }
@@ -225,23 +329,92 @@ bfs_pull_kernel(uint32_t n, const uint32_t* __restrict__ t_row_off, const uint32
             const uint32_t dg = row_off[v + 1] - row_off[v];
             if (dg > 0) { cand[0] = v; cdeg[0] = dg; my_cnt = 1; my_deg = dg; }
         }
-        push_enqueue(cand, cdeg, my_cnt, my_deg, pq, s_scan, &s_base);
+        push_enqueue(cand, cdeg, my_cnt, my_deg, pq, seg_out, s_scan, &s_base);
     }
-    const uint32_t wsum = wave_sum(activated);
-    if (lane_id() == 0 && wsum) atomicAdd(pq.activated, static_cast<unsigned long long>(wsum));
+    block_stat_add(pq.stats + blockIdx.x, activated, s_red);
+    __syncthreads();
+    block_stat_add(pq.stats + kMaxPushBlocks + blockIdx.x, scanned, s_red);
 }
 
-// Sets up a one-vertex frontier {src} for a push engine: q[0] = src, qoff[0] = 0,
-// cnt = (deg(src) << 32) | 1 (or 0 when src has no out-edge), clears the other two counters.
-static __global__ void push_seed_kernel(const uint32_t* __restrict__ row_off, uint32_t src, uint32_t* q, uint32_t* qoff,
-                                 unsigned long long* cnt3) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+// One BFS level. With kDirOpt the level runs top-down (advance) or bottom-up (pull) by Beamer's rule,
+// decided identically by every workgroup from the input counters and the previous level's history:
+// pull when the frontier's out-edges exceed 1/14 of the edges not yet explored, push again when the
+// frontier falls below n/24 vertices. History: hist[2*it] = (E << 32) | nf, hist[2*it + 1] = edges
+// explored through this level | (pull << 63). Frontier bitmaps rotate over three buffers fb0..fb2
+// (`words` words each): read fb[it%3], write fb[(it+1)%3], clear fb[(it+2)%3].
+template <bool kDirOpt>
+static __global__ void __launch_bounds__(kPushThreads)
+bfs_step_kernel(uint32_t n, uint32_t m, const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+                const uint32_t* __restrict__ t_row_off, const uint32_t* __restrict__ t_col,
+                uint32_t* __restrict__ depth, uint32_t* __restrict__ fb, uint32_t words, PushQueues pq, uint32_t it,
+                unsigned long long* __restrict__ hist) {
+    __shared__ uint32_t s_vb[kSegments + 1];
+    __shared__ uint32_t s_eb[kSegments + 1];
+    const bool active = push_prologue(pq, s_vb, s_eb);
+    const uint32_t nf = s_vb[kSegments];
+    const uint32_t E = s_eb[kSegments];
+    bool pull = false;
+    unsigned long long explored = 0;
+    if (kDirOpt) {
+        uint32_t* fb_clear = fb + static_cast<size_t>((it + 2) % 3) * words;
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
+            fb_clear[i] = 0u;
+        bool pull_prev = false;
+        if (it > 0) {
+            const unsigned long long h = hist[2 * (it - 1) + 1];
+            explored = h & ~(1ull << 63);
+            pull_prev = (h >> 63) != 0;
+        }
+        const unsigned long long unexplored = (m > explored + E) ? m - explored - E : 0ull;
+        pull = pull_prev ? (static_cast<unsigned long long>(nf) * 24 >= n)
+                         : (static_cast<unsigned long long>(E) * 14 > unexplored);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        hist[2 * it] = (static_cast<unsigned long long>(E) << 32) | nf;
+        hist[2 * it + 1] = (explored + E) | (pull ? (1ull << 63) : 0ull);
+    }
+    if (!active) return;
+    uint32_t* fb_cur = fb + static_cast<size_t>(it % 3) * words;
+    uint32_t* fb_next = fb + static_cast<size_t>((it + 1) % 3) * words;
+    if (kDirOpt && pull) {
+        pull_groups(n, t_row_off, t_col, row_off, depth, fb_cur, fb_next, it + 1, pq);
+    } else {
+        BfsPush pol{depth, kDirOpt ? fb_next : nullptr, it + 1};
+        advance_tiles(row_off, col, nullptr, pq, pol, s_vb, s_eb);
+    }
+}
+
+// Sets up a one-vertex frontier {src} for a push engine: segment 0 holds src (when it has out-edges),
+// every counter of the three rotating counter sets is cleared otherwise.
+static __global__ void push_seed_kernel(const uint32_t* __restrict__ row_off, uint32_t src, uint32_t* q,
+                                        uint32_t* qoff, unsigned long long* cnt3) {
+    const uint32_t t = threadIdx.x;
+    if (blockIdx.x != 0) return;
+    for (uint32_t i = t; i < 3u * kSegments; i += blockDim.x) cnt3[i] = 0ull;
+    __syncthreads();
+    if (t == 0) {
         const uint32_t dg = row_off[src + 1] - row_off[src];
         q[0] = src;
         qoff[0] = 0;
         cnt3[0] = dg ? ((static_cast<unsigned long long>(dg) << 32) | 1ull) : 0ull;
-        cnt3[1] = 0ull;
-        cnt3[2] = 0ull;
+    }
+}
+
+// Adds per-workgroup stats slots [0, count) to out[0] (one block, fixed order) and clears them.
+static __global__ void __launch_bounds__(256) sum_slots_kernel(unsigned long long* __restrict__ slots, uint32_t count,
+                                                                unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long s[256];
+    unsigned long long acc = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += 256) {
+        acc += slots[i];
+        slots[i] = 0ull;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < 256; ++i) t += s[i];
+        out[0] += t;
     }
 }
 

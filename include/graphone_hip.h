@@ -113,8 +113,8 @@ g1_status g1_spmv_csrvec_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_
                                        uint32_t lanes, g1_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------------
- * Host-side synthetic graphs (seeded, splitmix64; the RNG stream is specified in DESIGN notes of the
- * repository).  They produce the CSR invariants of GraphLoader + EdgeCache::ToCsr
+ * Host-side synthetic graphs (seeded, splitmix64; the random streams are spelled out at the top of
+ * graphone_amd/csrc/gen.cpp).  They produce the CSR invariants of GraphLoader + EdgeCache::ToCsr
  * (include/GraphOne/loader/graph_loader.h:254-296, include/GraphOne/loader/edge_cache.h:50-105):
  * no self loops, no duplicate (src,dst), columns ascending within a row, n = number of vertex ids.
  * ------------------------------------------------------------------------------------------- */
@@ -189,6 +189,19 @@ typedef struct g1_run_stats {
     double elapsed_ms;         /* device time of the run                                     */
     double last_delta;         /* PageRank: last L1 (or max) delta                           */
 } g1_run_stats;
+
+/* Per-kernel profile of the runs on one graph (HIP events around every engine launch, recorded on the
+ * run's stream). While enabled, every launch adds its device time and its algorithmic bytes (the bytes
+ * the kernel must move by definition of its work, not the bytes it happens to move) to the entry of
+ * its kernel. g1_profile_read copies up to `capacity` entries and clears them. */
+typedef struct g1_kernel_profile {
+    char name[40];
+    uint64_t launches;
+    double total_ms;
+    double algorithmic_bytes;
+} g1_kernel_profile;
+g1_status g1_profile_enable(g1_graph* graph, int32_t enable);
+g1_status g1_profile_read(g1_graph* graph, g1_kernel_profile* out, uint32_t capacity, uint32_t* count);
 
 /* BFS depths (hops), depth_dev[src] = 0, unreached = G1_INF. */
 g1_status g1_bfs(g1_graph* graph, uint32_t src, uint32_t* depth_dev, int32_t engine,

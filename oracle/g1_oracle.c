@@ -298,7 +298,13 @@ int orc_pagerank_f64(uint32_t n, const uint32_t* rp, const uint32_t* w, const ui
     double delta = 0.0;
     int cur = 0;
     if (mode == 1) {
+        /* x is built once from ranks[0] (the first-iteration Construct, engine/blas_gc.h:388-400), then
+         * by apply_construct from the rank just written (blas_gc.h:64-74 with construct_each of
+         * apps/pagerank_blas.cu:94-100). */
         int idx = 0;
+        for (uint32_t u = 0; u < n; ++u) x[u] = ranks[0][u] * iw[u];
+        double* y = (double*)malloc(sizeof(double) * (n ? n : 1));
+        if (!y) { free(iw); free(r0); free(r1); free(x); return -1; }
         for (;;) {
             delta = 0.0;
             for (uint32_t v = 0; v < n; ++v) {
@@ -306,15 +312,19 @@ int orc_pagerank_f64(uint32_t n, const uint32_t* rp, const uint32_t* w, const ui
                 if (dl > delta) delta = dl;
             }
             if (delta <= tol || it >= max_iter) break;
-            for (uint32_t u = 0; u < n; ++u) x[u] = ranks[idx][u] * iw[u];
             for (uint32_t v = 0; v < n; ++v) {
                 double s = 0.0;
                 for (uint32_t k = tp[v]; k < tp[v + 1]; ++k) s += x[tj[k]] * (tw ? (double)tw[k] : 1.0);
-                ranks[idx][v] = base + s + ((personalized && v == seed) ? restart : 0.0);
+                y[v] = s;
+            }
+            for (uint32_t v = 0; v < n; ++v) {
+                ranks[idx][v] = base + y[v] + ((personalized && v == seed) ? restart : 0.0);
+                x[v] = ranks[idx][v] * iw[v];
             }
             idx ^= 1;
             ++it;
         }
+        free(y);
         cur = idx;
     } else {
         while (it < max_iter) {

@@ -277,6 +277,19 @@ def test_pagerank_reference_convergence_quirk():
     assert st["iterations"] > 5
 
 
+def test_pagerank_reference_mode_iterates_are_the_power_iteration():
+    """Mode 1 updates in place and alternates buffers, but its iterates must be exactly the power
+    iteration of mode 0: after K iterations it returns ranks[opt_idx], the iterate K-1."""
+    rng = np.random.default_rng(9)
+    rp, ci, _ = random_csr(rng, 70, 4, weighted=False)
+    tp, tj, _ = transpose(rp, ci, None)
+    for k in (2, 3, 8, 9):
+        r1, st1 = oracle.pagerank_f64(rp, None, tp, tj, None, 0.85, max_iter=k, tol=0.0, mode=1)
+        r0, st0 = oracle.pagerank_f64(rp, None, tp, tj, None, 0.85, max_iter=k - 1, tol=0.0, mode=0)
+        assert st1["iterations"] == k and st0["iterations"] == k - 1
+        assert np.array_equal(r1, r0)
+
+
 # ------------------------------------------------------------------------- golden vectors
 
 def test_golden_vectors():
