@@ -114,12 +114,33 @@ spmv_merge_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict
     const uint32_t nzs = c1.y - c0.y;
     const uint32_t total = rows + nzs;
 
-    for (uint32_t k = tid; k < rows; k += kMergeThreads) s_rowend[k] = row_offsets[c0.x + 1 + k];
-    if (tid == 0) s_rowend[rows] = kInf;  // no further row end inside this tile
-    for (uint32_t k = tid; k < nzs; k += kMergeThreads) {
-        const uint32_t e = c0.y + k;
-        const typename S::value_t w = kHasValues ? vals[e] : typename S::value_t(1);
-        s_prod[k] = S::combine(w, x[col[e]]);
+    // Stage the tile with every load of a thread in flight at once (rows + nzs <= kMergeTile, so each
+    // thread owns at most kMergeItems row ends and kMergeItems nonzeros).
+    {
+        uint32_t re[kMergeItems], ci[kMergeItems];
+        typename S::value_t wv[kMergeItems];
+        typename S::x_t xv[kMergeItems];
+#pragma unroll
+        for (int r = 0; r < kMergeItems; ++r) {
+            const uint32_t k = r * kMergeThreads + tid;
+            if (k < rows) re[r] = row_offsets[c0.x + 1 + k];
+            if (k < nzs) {
+                ci[r] = col[c0.y + k];
+                wv[r] = kHasValues ? vals[c0.y + k] : typename S::value_t(1);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kMergeItems; ++r) {
+            const uint32_t k = r * kMergeThreads + tid;
+            if (k < nzs) xv[r] = x[ci[r]];
+        }
+#pragma unroll
+        for (int r = 0; r < kMergeItems; ++r) {
+            const uint32_t k = r * kMergeThreads + tid;
+            if (k < rows) s_rowend[k] = re[r];
+            if (k < nzs) s_prod[k] = S::combine(wv[r], xv[r]);
+        }
+        if (tid == 0) s_rowend[rows] = kInf;  // no further row end inside this tile
     }
     __syncthreads();
 
