@@ -18,8 +18,9 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (G1_ENGINE_ADVANCE, G1_ENGINE_BLAS_CSRVEC, G1_ENGINE_BLAS_MERGE, G1_ENGINE_DIROPT, G1_INF,
-                   G1_PR_L1_F64, G1_PR_MAX_ABS, G1Error, HostCSR, RunStats, call, lib)
+from ._lib import (G1_BATCH_MINPLUS, G1_BATCH_PPR, G1_ENGINE_ADVANCE, G1_ENGINE_BLAS_CSRVEC, G1_ENGINE_BLAS_MERGE,
+                   G1_ENGINE_DIROPT, G1_INF, G1_PR_L1_F64, G1_PR_MAX_ABS, BatchStats, G1Error, HostCSR, RunStats,
+                   call, lib)
 
 INF = G1_INF
 
@@ -289,6 +290,44 @@ class Graph:
         call("g1_ppr", self._h, C.c_uint32(seed), C.c_float(restart), C.c_uint32(max_iter), C.c_double(tol),
              C.c_void_p(r.ptr), C.byref(st), stream)
         return RunResult(r.numpy() if to_host else r, st.as_dict())
+
+    # ---- batched multi-source engines (k sources as the columns of one SpMM)
+
+    @staticmethod
+    def _sources(sources) -> np.ndarray:
+        return np.ascontiguousarray(np.asarray(sources, dtype=np.uint32).reshape(-1))
+
+    def autotune(self, kind: str, sources, weighted: bool = True, stream=None) -> int:
+        """Times every column width on the graph and keeps the best for kind 'minplus' or 'ppr'."""
+        src = self._sources(sources)
+        best = C.c_uint32()
+        k = G1_BATCH_PPR if kind == "ppr" else G1_BATCH_MINPLUS
+        call("g1_batch_autotune", self._h, k, src.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_uint32(src.size),
+             int(weighted), C.byref(best), stream)
+        return best.value
+
+    def sssp_batch(self, sources, k: int = 0, weighted: bool = True, out: DeviceArray | None = None, stream=None,
+                   to_host: bool = True):
+        """Distances (weighted) or BFS depths of every source: values[c] is the run from sources[c]."""
+        src = self._sources(sources)
+        d = out if out is not None else DeviceArray(max(1, src.size * self.n), np.uint32)
+        st = BatchStats()
+        call("g1_sssp_batch", self._h, src.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_uint32(src.size),
+             C.c_uint32(k), int(weighted), C.c_void_p(d.ptr), C.byref(st), stream)
+        vals = d.numpy()[: src.size * self.n].reshape(src.size, self.n) if to_host else d
+        return RunResult(vals, st.as_dict())
+
+    def ppr_batch(self, seeds, restart: float = 0.15, max_iter: int = 50, tol: float = 1e-7, k: int = 0,
+                  out: DeviceArray | None = None, stream=None, to_host: bool = True):
+        """Personalized PageRank of every seed: values[c] equals ppr(seeds[c]) bit for bit."""
+        src = self._sources(seeds)
+        r = out if out is not None else DeviceArray(max(1, src.size * self.n), np.float32)
+        st = BatchStats()
+        call("g1_ppr_batch", self._h, src.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_uint32(src.size),
+             C.c_float(restart), C.c_uint32(max_iter), C.c_double(tol), C.c_uint32(k), C.c_void_p(r.ptr),
+             C.byref(st), stream)
+        vals = r.numpy()[: src.size * self.n].reshape(src.size, self.n) if to_host else r
+        return RunResult(vals, st.as_dict())
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

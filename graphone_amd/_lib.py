@@ -33,6 +33,9 @@ G1_ENGINE_DIROPT = 3
 G1_PR_L1_F64 = 0
 G1_PR_MAX_ABS = 1
 
+G1_BATCH_MINPLUS = 0
+G1_BATCH_PPR = 1
+
 
 class G1Error(RuntimeError):
     def __init__(self, status: int, where: str, message: str):
@@ -66,6 +69,18 @@ class RunStats(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("k", C.c_uint32),
+        ("tiles", C.c_uint32),
+        ("iterations_sum", C.c_uint64),
+        ("elapsed_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class KernelProfile(C.Structure):
@@ -127,6 +142,9 @@ _SIGNATURES = {
     "g1_sssp": (_st, [_vp, _u32, _vp, _i32, C.POINTER(RunStats), _vp]),
     "g1_pagerank": (_st, [_vp, C.c_float, _u32, C.c_double, _i32, _i32, _vp, C.POINTER(RunStats), _vp]),
     "g1_ppr": (_st, [_vp, _u32, C.c_float, _u32, C.c_double, _vp, C.POINTER(RunStats), _vp]),
+    "g1_batch_autotune": (_st, [_vp, _i32, _pu32, _u32, _i32, _pu32, _vp]),
+    "g1_sssp_batch": (_st, [_vp, _pu32, _u32, _u32, _i32, _vp, C.POINTER(BatchStats), _vp]),
+    "g1_ppr_batch": (_st, [_vp, _pu32, _u32, C.c_float, _u32, C.c_double, _u32, _vp, C.POINTER(BatchStats), _vp]),
 }
 
 

@@ -1,0 +1,462 @@
+// Batched multi-source engines: k sources run together as k columns of one semiring SpMM over the
+// in-edge CSR (g1/spmm.hpp), min-plus for SSSP/BFS and compensated plus-times for personalized PageRank.
+//
+// A batch of S sources is cut into ceil(S/k) column tiles of k in {8, 16, 32, 64}; k comes from the
+// caller or from the autotuner (g1_batch_autotune), which times one tile of every width on the graph and
+// keeps the fastest per source. Min-plus iterations are captured once per (graph, k) as a HIP graph of
+// two iterations (the change masks swap back) and replayed until no column changes.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "g1/apply.hpp"
+#include "g1/spmm.hpp"
+#include "graph.hpp"
+#include "spmv_launch.hpp"
+
+struct BatchWork {
+    uint32_t K = 0;
+    uint32_t* D = nullptr;                                 // n x K distances
+    unsigned long long* mask[2] = {nullptr, nullptr};      // changed-column masks
+    void* recs = nullptr;                                  // SpMM tile records
+    uint32_t* tile_counts = nullptr;
+    unsigned long long* active = nullptr;                  // rows changed by the last iteration
+    float* R[2] = {nullptr, nullptr};                      // PPR ranks, n x K
+    float* X = nullptr;
+    float* Y = nullptr;
+    hipGraphExec_t minplus_exec[2] = {nullptr, nullptr};   // two captured iterations, [weighted]
+};
+
+namespace g1 {
+namespace {
+
+int k_slot(uint32_t k) { return k == 8 ? 0 : k == 16 ? 1 : k == 32 ? 2 : k == 64 ? 3 : -1; }
+
+template <typename T>
+g1_status alloc_n(g1_graph* g, T** p, size_t count) {
+    void* v = nullptr;
+    G1_TRY(graph_alloc(g, &v, count * sizeof(T)));
+    *p = static_cast<T*>(v);
+    return G1_OK;
+}
+
+g1_status ensure_batch(g1_graph* g, uint32_t K, bool ppr, BatchWork** out) {
+    const int slot = k_slot(K);
+    G1_CHECK(slot >= 0, G1_ERR_INVALID_ARG, "column tile k must be 8, 16, 32 or 64 (got %u)", K);
+    G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "batched engines need the graph's transpose");
+    if (!g->batch[slot]) {
+        BatchWork* b = new BatchWork();
+        b->K = K;
+        g->batch[slot] = b;
+    }
+    BatchWork* b = g->batch[slot];
+    const size_t n = g->n ? g->n : 1;
+    const uint32_t tiles = merge_num_tiles(g->n, g->m);
+    if (!b->recs) {
+        const size_t rb = std::max(spmm_records_bytes<MinPlusU32, 64>(g->n, g->m),
+                                   spmm_records_bytes<PlusTimesKahanF32, 64>(g->n, g->m));
+        void* v = nullptr;
+        G1_TRY(graph_alloc(g, &v, rb + 256));
+        b->recs = v;
+        G1_TRY(alloc_n(g, &b->tile_counts, tiles + 1));
+        G1_TRY(alloc_n(g, &b->active, 4));
+    }
+    if (!ppr && !b->D) {
+        G1_TRY(alloc_n(g, &b->D, n * K));
+        G1_TRY(alloc_n(g, &b->mask[0], n));
+        G1_TRY(alloc_n(g, &b->mask[1], n));
+    }
+    if (ppr && !b->X) {
+        G1_TRY(alloc_n(g, &b->R[0], n * K));
+        G1_TRY(alloc_n(g, &b->R[1], n * K));
+        G1_TRY(alloc_n(g, &b->X, n * K));
+        G1_TRY(alloc_n(g, &b->Y, n * K));
+        G1_TRY(ensure_rank_partials(g));
+        G1_TRY(ensure_pull_work(g, true));  // iw
+    }
+    *out = b;
+    return G1_OK;
+}
+
+// D = INF, D[src_j][j] = 0 for the tile's valid columns; mask[v] = columns whose source is v.
+template <int K>
+__global__ void minplus_batch_init_kernel(uint32_t n, uint32_t* __restrict__ D, unsigned long long* __restrict__ mask,
+                                          const uint32_t* __restrict__ srcs, uint32_t cols) {
+    const size_t total = static_cast<size_t>(n) * K;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) D[i] = kInf;
+    for (size_t v = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) mask[v] = 0ull;
+}
+
+template <int K>
+__global__ void minplus_batch_seed_kernel(uint32_t* __restrict__ D, unsigned long long* __restrict__ mask,
+                                          const uint32_t* __restrict__ srcs, uint32_t cols) {
+    const uint32_t j = threadIdx.x;
+    if (j < cols) {
+        const uint32_t s = srcs[j];
+        D[static_cast<size_t>(s) * K + j] = 0u;
+        atomicOr(&mask[s], 1ull << j);
+    }
+}
+
+// R0 = e_seed per column and X0 = R0 * iw (pr_init_kernel's seed variant, per column); padding columns
+// (seed kInf) start at zero.
+template <int K>
+__global__ void ppr_batch_init_kernel(uint32_t n, const float* __restrict__ iw, const uint32_t* __restrict__ seeds,
+                                      float* __restrict__ R, float* __restrict__ X) {
+    const size_t total = static_cast<size_t>(n) * K;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t v = static_cast<uint32_t>(i / K);
+        const uint32_t j = static_cast<uint32_t>(i % K);
+        const float rv = (v == seeds[j]) ? 1.0f : 0.0f;
+        R[i] = rv;
+        X[i] = rv * iw[v];
+    }
+}
+
+__global__ void sum_counts_kernel(const uint32_t* __restrict__ c, uint32_t count, unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long s[256];
+    unsigned long long acc = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += 256) acc += c[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < 256; ++i) t += s[i];
+        out[0] = t;
+    }
+}
+
+// out[(col0 + j) * n + v] = D[v * K + j] for the tile's valid columns.
+template <int K, typename T>
+__global__ void batch_columns_out_kernel(uint32_t n, const T* __restrict__ D, uint32_t cols, T* __restrict__ out) {
+    const size_t total = static_cast<size_t>(n) * cols;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t j = static_cast<uint32_t>(i / n);
+        const uint32_t v = static_cast<uint32_t>(i % n);
+        out[i] = D[static_cast<size_t>(v) * K + j];
+    }
+}
+
+uint32_t fill_blocks(size_t items) {
+    size_t b = (items + 255) / 256;
+    if (b > 8192) b = 8192;
+    return static_cast<uint32_t>(b ? b : 1);
+}
+
+// One min-plus iteration: SpMM over the in-edges with the change masks (prev -> next), its fixup, and the
+// count of changed rows into b->active.
+template <int K>
+g1_status minplus_iteration(g1_graph* g, BatchWork* b, int prev, bool weighted, hipStream_t s) {
+    const uint32_t tiles = merge_num_tiles(g->n, g->m);
+    MinPlusBatchPolicy<K> pol{b->D, b->mask[prev], b->mask[prev ^ 1]};
+    auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(b->recs);
+    const uint32_t* vals = weighted ? g->t_w : nullptr;
+    spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+        g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, b->tile_counts);
+    G1_LAUNCHED();
+    if (tiles > 1) {
+        const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
+        spmm_fixup_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<blocks, 256, 0, s>>>(tiles, merge_coords(g->t_ws),
+                                                                                     recs, pol, b->tile_counts);
+        G1_LAUNCHED();
+    }
+    sum_counts_kernel<<<1, 256, 0, s>>>(b->tile_counts, tiles, b->active);
+    G1_LAUNCHED();
+    return G1_OK;
+}
+
+template <int K>
+g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host, uint32_t cols, bool weighted,
+                           uint32_t* out, uint32_t* iterations, hipStream_t s) {
+    const uint32_t n = g->n;
+    G1_HIP(hipMemcpyAsync(g->seeds_dev, srcs_host, cols * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    minplus_batch_init_kernel<K><<<fill_blocks(static_cast<size_t>(n) * K), 256, 0, s>>>(n, b->D, b->mask[0],
+                                                                                          g->seeds_dev, cols);
+    G1_LAUNCHED();
+    minplus_batch_seed_kernel<K><<<1, 64, 0, s>>>(b->D, b->mask[0], g->seeds_dev, cols);
+    G1_LAUNCHED();
+    // Two iterations per replay: mask[0] -> mask[1] -> mask[0].
+    hipGraphExec_t& exec = b->minplus_exec[weighted ? 1 : 0];
+    if (!exec) {
+        // Captured once per (graph, k, weighted) on a private stream; the pointers it holds never change.
+        hipStream_t cs = nullptr;
+        G1_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipGraph_t graph = nullptr;
+        g1_status st = G1_OK;
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            st = minplus_iteration<K>(g, b, 0, weighted, cs);
+            if (st == G1_OK) st = minplus_iteration<K>(g, b, 1, weighted, cs);
+            e = hipStreamEndCapture(cs, &graph);
+        }
+        (void)hipStreamDestroy(cs);
+        G1_TRY(st);
+        G1_HIP(e);
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        G1_HIP(e);
+    }
+    uint32_t it = 0;
+    for (;;) {
+        G1_HIP(hipGraphLaunch(exec, s));
+        it += 2;
+        G1_HIP(hipMemcpyAsync(g->h_ctr + 12, b->active, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        G1_HIP(hipStreamSynchronize(s));
+        if (g->h_ctr[12] == 0 || it >= kHistIters) break;
+    }
+    *iterations = it;
+    batch_columns_out_kernel<K, uint32_t><<<fill_blocks(static_cast<size_t>(n) * cols), 256, 0, s>>>(n, b->D, cols, out);
+    G1_LAUNCHED();
+    return G1_OK;
+}
+
+template <int K>
+g1_status ppr_tile_run(g1_graph* g, BatchWork* b, const uint32_t* seeds_host, uint32_t cols, float restart,
+                       uint32_t max_iter, double tol, float* out, uint32_t* iterations, hipStream_t s) {
+    const uint32_t n = g->n;
+    const uint32_t tiles = merge_num_tiles(g->n, g->m);
+    const uint32_t nb = rank_blocks(n);
+    const float* vals = g->t_wf;
+    std::vector<uint32_t> seeds(K, kInf);
+    for (uint32_t j = 0; j < cols; ++j) seeds[j] = seeds_host[j];
+    G1_HIP(hipMemcpyAsync(g->seeds_dev, seeds.data(), K * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    // iw from the out-edges, then R0 = e_seed per column, X0 = R0 * iw
+    pr_init_kernel<<<fill_blocks(n), 256, 0, s>>>(n, g->row_off, g->w, 1.0f - restart, g->iw, g->r[0], 0.0f, g->xf,
+                                                  kInf);
+    G1_LAUNCHED();
+    ppr_batch_init_kernel<K><<<fill_blocks(static_cast<size_t>(n) * K), 256, 0, s>>>(n, g->iw, g->seeds_dev, b->R[0],
+                                                                                      b->X);
+    G1_LAUNCHED();
+    unsigned long long frozen = 0;
+    for (uint32_t j = cols; j < K; ++j) frozen |= 1ull << j;  // padding columns never run
+    std::vector<uint32_t> col_iters(K, 0);
+    std::vector<double> deltas(K);
+    int cur = 0;
+    uint32_t it = 0;
+    auto* recs = static_cast<SpmmTileRec<PlusTimesKahanF32, K>*>(b->recs);
+    const unsigned long long all = (K == 64) ? ~0ull : ((1ull << K) - 1ull);
+    while (frozen != all && it < max_iter) {
+        PlusTimesBatchPolicy<K> pol{b->X, b->Y};
+        spmm_tile_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, nullptr);
+        G1_LAUNCHED();
+        if (tiles > 1) {
+            const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
+            spmm_fixup_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<blocks, 256, 0, s>>>(
+                tiles, merge_coords(g->t_ws), recs, pol, nullptr);
+            G1_LAUNCHED();
+        }
+        rank_apply_kernel<K><<<nb, 256, 0, s>>>(n, b->Y, 0.0f, g->seeds_dev, restart, g->iw, b->R[cur], b->R[cur ^ 1],
+                                                b->X, g->rpart, nb, frozen);
+        G1_LAUNCHED();
+        reduce_partials_kernel<0><<<dim3(1, K), 256, 0, s>>>(g->rpart, nb, g->rpart + static_cast<size_t>(nb) * kBatchCols);
+        G1_LAUNCHED();
+        G1_HIP(hipMemcpyAsync(deltas.data(), g->rpart + static_cast<size_t>(nb) * kBatchCols, K * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+        G1_HIP(hipStreamSynchronize(s));
+        cur ^= 1;
+        ++it;
+        for (uint32_t j = 0; j < K; ++j) {
+            if ((frozen >> j) & 1ull) continue;
+            col_iters[j] = it;
+            if (deltas[j] < tol) frozen |= 1ull << j;  // the single-source run stops here too
+        }
+    }
+    // A column frozen at iteration t kept its rank in both buffers since, so R[cur] holds every column.
+    *iterations = it;
+    batch_columns_out_kernel<K, float><<<fill_blocks(static_cast<size_t>(n) * cols), 256, 0, s>>>(n, b->R[cur], cols, out);
+    G1_LAUNCHED();
+    return G1_OK;
+}
+
+template <int K>
+g1_status sssp_batch_k(g1_graph* g, const uint32_t* srcs, uint32_t count, bool weighted, uint32_t* out,
+                       g1_batch_stats* st, hipStream_t s) {
+    BatchWork* b = nullptr;
+    G1_TRY(ensure_batch(g, K, false, &b));
+    G1_TRY(ensure_rank_partials(g));  // seeds_dev
+    for (uint32_t c0 = 0; c0 < count; c0 += K) {
+        const uint32_t cols = std::min<uint32_t>(K, count - c0);
+        uint32_t iters = 0;
+        G1_TRY(minplus_tile_run<K>(g, b, srcs + c0, cols, weighted, out + static_cast<size_t>(c0) * g->n, &iters, s));
+        st->iterations_sum += iters;
+        st->tiles += 1;
+    }
+    return G1_OK;
+}
+
+template <int K>
+g1_status ppr_batch_k(g1_graph* g, const uint32_t* seeds, uint32_t count, float restart, uint32_t max_iter, double tol,
+                      float* out, g1_batch_stats* st, hipStream_t s) {
+    BatchWork* b = nullptr;
+    G1_TRY(ensure_batch(g, K, true, &b));
+    for (uint32_t c0 = 0; c0 < count; c0 += K) {
+        const uint32_t cols = std::min<uint32_t>(K, count - c0);
+        uint32_t iters = 0;
+        G1_TRY(ppr_tile_run<K>(g, b, seeds + c0, cols, restart, max_iter, tol, out + static_cast<size_t>(c0) * g->n,
+                               &iters, s));
+        st->iterations_sum += iters;
+        st->tiles += 1;
+    }
+    return G1_OK;
+}
+
+g1_status check_sources(g1_graph* g, const uint32_t* srcs, uint32_t count, const void* out, const char* what) {
+    G1_CHECK(g != nullptr, G1_ERR_INVALID_ARG, "%s: null graph", what);
+    G1_CHECK(out != nullptr && (count == 0 || srcs != nullptr), G1_ERR_INVALID_ARG, "%s: null argument", what);
+    for (uint32_t i = 0; i < count; ++i)
+        G1_CHECK(srcs[i] < g->n, G1_ERR_INVALID_ARG, "%s: source %u (index %u) out of range (n = %u)", what, srcs[i], i,
+                 g->n);
+    return G1_OK;
+}
+
+g1_status timed(g1_graph* g, hipStream_t s, g1_batch_stats* st) {
+    G1_HIP(hipEventRecord(g->ev[1], s));
+    G1_HIP(hipEventSynchronize(g->ev[1]));
+    float ms = 0.0f;
+    G1_HIP(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+    st->elapsed_ms = ms;
+    return G1_OK;
+}
+
+g1_status sssp_batch_dispatch(g1_graph* g, const uint32_t* srcs, uint32_t count, uint32_t k, bool weighted,
+                              uint32_t* out, g1_batch_stats* st, hipStream_t s) {
+    switch (k) {
+        case 8: return sssp_batch_k<8>(g, srcs, count, weighted, out, st, s);
+        case 16: return sssp_batch_k<16>(g, srcs, count, weighted, out, st, s);
+        case 32: return sssp_batch_k<32>(g, srcs, count, weighted, out, st, s);
+        case 64: return sssp_batch_k<64>(g, srcs, count, weighted, out, st, s);
+        default: return set_error(G1_ERR_INVALID_ARG, "column tile k must be 8, 16, 32 or 64 (got %u)", k);
+    }
+}
+
+g1_status ppr_batch_dispatch(g1_graph* g, const uint32_t* seeds, uint32_t count, uint32_t k, float restart,
+                             uint32_t max_iter, double tol, float* out, g1_batch_stats* st, hipStream_t s) {
+    switch (k) {
+        case 8: return ppr_batch_k<8>(g, seeds, count, restart, max_iter, tol, out, st, s);
+        case 16: return ppr_batch_k<16>(g, seeds, count, restart, max_iter, tol, out, st, s);
+        case 32: return ppr_batch_k<32>(g, seeds, count, restart, max_iter, tol, out, st, s);
+        case 64: return ppr_batch_k<64>(g, seeds, count, restart, max_iter, tol, out, st, s);
+        default: return set_error(G1_ERR_INVALID_ARG, "column tile k must be 8, 16, 32 or 64 (got %u)", k);
+    }
+}
+
+// Times one full column tile of every width on sources drawn from `srcs` (repeated as needed) and
+// returns the width with the lowest device time per source.
+g1_status autotune(g1_graph* g, int kind, const uint32_t* srcs, uint32_t count, bool weighted, uint32_t* best,
+                   hipStream_t s) {
+    G1_CHECK(count > 0, G1_ERR_INVALID_ARG, "autotune needs at least one source");
+    std::vector<uint32_t> sample(64);
+    for (uint32_t i = 0; i < 64; ++i) sample[i] = srcs[i % count];
+    void* out = nullptr;
+    const size_t elem = kind == 1 ? sizeof(float) : sizeof(uint32_t);
+    G1_HIP(hipMalloc(&out, elem * static_cast<size_t>(g->n) * 64));
+    double best_per = 1e300;
+    uint32_t best_k = 64;
+    g1_status status = G1_OK;
+    for (uint32_t k : {8u, 16u, 32u, 64u}) {
+        g1_batch_stats st;
+        memset(&st, 0, sizeof(st));
+        // untimed first pass (allocation, graph capture), then the timed pass
+        for (int pass = 0; pass < 2 && status == G1_OK; ++pass) {
+            memset(&st, 0, sizeof(st));
+            if (hipEventRecord(g->ev[0], s) != hipSuccess) { status = set_error(G1_ERR_HIP, "event"); break; }
+            status = kind == 1 ? ppr_batch_dispatch(g, sample.data(), k, k, 0.15f, 50, 1e-7, static_cast<float*>(out), &st, s)
+                               : sssp_batch_dispatch(g, sample.data(), k, k, weighted,
+                                                     static_cast<uint32_t*>(out), &st, s);
+            if (status == G1_OK) status = timed(g, s, &st);
+        }
+        if (status != G1_OK) break;
+        const double per = st.elapsed_ms / k;
+        if (per < best_per) {
+            best_per = per;
+            best_k = k;
+        }
+    }
+    (void)hipFree(out);
+    G1_TRY(status);
+    *best = best_k;
+    g->best_k[kind] = best_k;
+    return G1_OK;
+}
+
+}  // namespace
+
+void batch_release(g1_graph* g) {
+    for (BatchWork*& b : g->batch) {
+        if (!b) continue;
+        for (hipGraphExec_t& e : b->minplus_exec)
+            if (e) (void)hipGraphExecDestroy(e);
+        delete b;  // device areas belong to the graph's allocation list
+        b = nullptr;
+    }
+}
+
+}  // namespace g1
+
+extern "C" {
+
+g1_status g1_batch_autotune(g1_graph* g, int32_t kind, const uint32_t* sources, uint32_t count, int32_t weighted,
+                            uint32_t* best_k, g1_stream_t stream) {
+    using namespace g1;
+    G1_CHECK(g != nullptr && best_k != nullptr, G1_ERR_INVALID_ARG, "g1_batch_autotune: null argument");
+    G1_CHECK(kind == G1_BATCH_MINPLUS || kind == G1_BATCH_PPR, G1_ERR_INVALID_ARG, "g1_batch_autotune: unknown kind");
+    G1_TRY(check_sources(g, sources, count, best_k, "g1_batch_autotune"));
+    std::lock_guard<std::mutex> lock(g->mu);
+        if (weighted && kind == G1_BATCH_MINPLUS)
+        G1_CHECK(g->t_w != nullptr, G1_ERR_INVALID_ARG, "g1_batch_autotune: graph has no weights");
+    return autotune(g, kind, sources, count, weighted != 0 && kind == G1_BATCH_MINPLUS, best_k, as_stream(stream));
+}
+
+g1_status g1_sssp_batch(g1_graph* g, const uint32_t* sources, uint32_t count, uint32_t k, int32_t weighted,
+                        uint32_t* dist_dev, g1_batch_stats* stats, g1_stream_t stream) {
+    using namespace g1;
+    g1_batch_stats local;
+    if (!stats) stats = &local;
+    memset(stats, 0, sizeof(*stats));
+    G1_TRY(check_sources(g, sources, count, dist_dev, "g1_sssp_batch"));
+    if (weighted) G1_CHECK(g->t_w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp_batch: graph has no weights");
+    std::lock_guard<std::mutex> lock(g->mu);
+    const hipStream_t s = as_stream(stream);
+    if (k == 0) {
+        if (!g->best_k[0]) {
+            uint32_t best = 0;
+            G1_TRY(autotune(g, G1_BATCH_MINPLUS, sources, count, weighted != 0, &best, s));
+        }
+        k = g->best_k[0];
+    }
+    stats->k = k;
+    G1_HIP(hipEventRecord(g->ev[0], s));
+    G1_TRY(sssp_batch_dispatch(g, sources, count, k, weighted != 0, dist_dev, stats, s));
+    return timed(g, s, stats);
+}
+
+g1_status g1_ppr_batch(g1_graph* g, const uint32_t* seeds, uint32_t count, float restart, uint32_t max_iter, double tol,
+                       uint32_t k, float* rank_dev, g1_batch_stats* stats, g1_stream_t stream) {
+    using namespace g1;
+    g1_batch_stats local;
+    if (!stats) stats = &local;
+    memset(stats, 0, sizeof(*stats));
+    G1_TRY(check_sources(g, seeds, count, rank_dev, "g1_ppr_batch"));
+    G1_CHECK(restart > 0.0f && restart <= 1.0f, G1_ERR_INVALID_ARG, "g1_ppr_batch: restart must be in (0, 1]");
+    std::lock_guard<std::mutex> lock(g->mu);
+    const hipStream_t s = as_stream(stream);
+    if (k == 0) {
+        if (!g->best_k[1]) {
+            uint32_t best = 0;
+            G1_TRY(autotune(g, G1_BATCH_PPR, seeds, count, false, &best, s));
+        }
+        k = g->best_k[1];
+    }
+    stats->k = k;
+    G1_HIP(hipEventRecord(g->ev[0], s));
+    G1_TRY(ppr_batch_dispatch(g, seeds, count, k, restart, max_iter, tol, rank_dev, stats, s));
+    return timed(g, s, stats);
+}
+
+}  // extern "C"

@@ -399,6 +399,9 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
         cur = idx;
     } else {
         const double spmv_bytes = 4.0 * (n + 1.0) + 4.0 * g->m * (vals ? 2.0 : 1.0) + 8.0 * n;
+        G1_TRY(ensure_rank_partials(g));
+        const uint32_t nb = rank_blocks(n);
+        G1_HIP(hipMemcpyAsync(g->seeds_dev, &seed, sizeof(uint32_t), hipMemcpyHostToDevice, s));
         for (;;) {
             if (it >= max_iter) break;
             int tok;
@@ -412,15 +415,16 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
             }
             prof_end(g, tok, s, spmv_bytes);
             tok = prof_begin(g, "pr_apply", s);
-            pr_apply_kernel<0><<<blocks, 256, 0, s>>>(n, g->yf, base_all, seed, base_seed, g->iw, g->r[cur],
-                                                      g->r[cur ^ 1], g->xf, g->partials);
+            rank_apply_kernel<1><<<nb, 256, 0, s>>>(n, g->yf, base_all, g->seeds_dev, base_seed, g->iw, g->r[cur],
+                                                    g->r[cur ^ 1], g->xf, g->rpart, nb, 0ull);
             G1_LAUNCHED();
             prof_end(g, tok, s, 20.0 * n);
-            reduce_partials_kernel<0><<<1, 256, 0, s>>>(g->partials, blocks, g->partials + kPartials);
+            reduce_partials_kernel<0><<<dim3(1, 1), 256, 0, s>>>(g->rpart, nb, g->rpart + static_cast<size_t>(nb) * kBatchCols);
             G1_LAUNCHED();
             cur ^= 1;
             ++it;
-            G1_HIP(hipMemcpyAsync(g->h_scalar, g->partials + kPartials, sizeof(double), hipMemcpyDeviceToHost, s));
+            G1_HIP(hipMemcpyAsync(g->h_scalar, g->rpart + static_cast<size_t>(nb) * kBatchCols, sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
             G1_HIP(hipStreamSynchronize(s));
             delta = g->h_scalar[0];
             if (delta < tol) break;

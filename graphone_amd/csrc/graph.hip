@@ -80,6 +80,14 @@ g1_status ensure_pull_work(g1_graph* g, bool pagerank) {
     return G1_OK;
 }
 
+g1_status ensure_rank_partials(g1_graph* g) {
+    if (g->rpart) return G1_OK;
+    const size_t nb = rank_blocks(g->n ? g->n : 1);
+    G1_TRY(alloc_n(g, &g->rpart, nb * kBatchCols + kBatchCols + 8));
+    G1_TRY(alloc_n(g, &g->seeds_dev, kBatchCols));
+    return G1_OK;
+}
+
 static hipEvent_t pool_event(g1_graph* g) {
     if (g->ev_used == g->ev_pool.size()) {
         hipEvent_t e = nullptr;
@@ -225,6 +233,7 @@ g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags
 g1_status g1_graph_destroy(g1_graph* g) {
     if (!g) return G1_OK;
     (void)hipDeviceSynchronize();
+    g1::batch_release(g);
     g1::graph_free_all(g);
     delete g;
     return G1_OK;

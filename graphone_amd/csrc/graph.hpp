@@ -49,6 +49,10 @@ struct g1_graph {
     float* iw = nullptr;
     float* r[2] = {nullptr, nullptr};
     double* partials = nullptr;  // per-block partial sums, then the folded scalar at [kPartials]
+    double* rpart = nullptr;     // canonical L1 partials: kBatchCols columns x ceil(n/256) blocks (+ sums)
+    uint32_t* seeds_dev = nullptr;  // seed vertex per column (kBatchCols)
+    struct BatchWork* batch[4] = {nullptr, nullptr, nullptr, nullptr};  // per column count 8, 16, 32, 64
+    uint32_t best_k[2] = {0, 0};                                        // autotuned k: [0] min-plus, [1] PPR
     unsigned long long* h_ctr = nullptr;  // pinned host mirror of ctr (16) and a counter set (at [16])
     double* h_scalar = nullptr;           // pinned host scalar
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -80,6 +84,10 @@ namespace g1 {
 
 constexpr uint32_t kPartials = 4096;     // max blocks of the apply kernels (and partial slots)
 constexpr uint32_t kHistIters = 1u << 17;  // iterations a push run may take
+constexpr uint32_t kBatchCols = 64;        // widest column tile of the batched engines
+inline uint32_t rank_blocks(uint32_t n) { return (n + 255) / 256; }
+g1_status ensure_rank_partials(g1_graph* g);
+void batch_release(g1_graph* g);  // captured graphs and host records of the batched engines
 
 g1_status graph_alloc(g1_graph* g, void** p, size_t bytes);
 g1_status ensure_push_work(g1_graph* g);

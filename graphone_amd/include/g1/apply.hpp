@@ -117,11 +117,55 @@ __global__ void __launch_bounds__(256) pr_apply_kernel(uint32_t n, const float* 
     }
 }
 
+// PageRank / personalized PageRank apply over K columns (K = 1 for a single run) with the canonical
+// FP64 L1 reduction: block b covers vertices [256b, 256b + 256), thread t vertex 256b + t, and each
+// column's 256 |r' - r| values are folded by the same fixed tree (wave shuffles, then the four waves in
+// order) into partials[j * nblocks + b]. A batched column therefore gets bit for bit the norm its
+// single-source run gets. Column j with bit j of `frozen` set keeps its rank and x (it has converged).
+//   r'[v][j] = (v == seeds[j] ? base_seed : base_all) + y[v][j]
+//   x[v][j]  = r'[v][j] * iw[v]
+template <int K>
+__global__ void __launch_bounds__(256) rank_apply_kernel(uint32_t n, const float* __restrict__ Y, float base_all,
+                                                         const uint32_t* __restrict__ seeds, float base_seed,
+                                                         const float* __restrict__ iw, const float* __restrict__ R_old,
+                                                         float* __restrict__ R_new, float* __restrict__ X,
+                                                         double* __restrict__ partials, uint32_t nblocks,
+                                                         unsigned long long frozen) {
+    __shared__ double s_part[256 / kWave];
+    const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+    const float iwv = v < n ? iw[v] : 0.0f;
+    for (int j = 0; j < K; ++j) {
+        double d = 0.0;
+        if (v < n) {
+            const size_t at = static_cast<size_t>(v) * K + j;
+            if ((frozen >> j) & 1ull) {
+                R_new[at] = R_old[at];
+            } else {
+                const float b = (v == seeds[j]) ? base_seed : base_all;
+                const float rn = b + Y[at];
+                R_new[at] = rn;
+                X[at] = rn * iwv;
+                d = fabs(static_cast<double>(rn) - static_cast<double>(R_old[at]));
+            }
+        }
+#pragma unroll
+        for (int off = kWave / 2; off >= 1; off >>= 1) d += __shfl_down(d, off);
+        if (lane_id() == 0) s_part[threadIdx.x / kWave] = d;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            partials[static_cast<size_t>(j) * nblocks + blockIdx.x] = ((s_part[0] + s_part[1]) + s_part[2]) + s_part[3];
+        __syncthreads();
+    }
+}
+
 // Folds per-block partials in block order into out[0] (sum or max); one block.
 template <int kMode>
 __global__ void __launch_bounds__(256) reduce_partials_kernel(const double* __restrict__ partials, uint32_t count,
                                                               double* __restrict__ out) {
     __shared__ double s[256];
+    // blockIdx.y selects a column: partials[y * count ..], out[y]
+    partials += static_cast<size_t>(blockIdx.y) * count;
+    out += blockIdx.y;
     double acc = 0.0;
     // Each thread folds a contiguous run, then the runs are folded in thread order.
     const uint32_t per = (count + 255) / 256;

@@ -230,6 +230,40 @@ g1_status g1_pagerank(g1_graph* graph, float alpha, uint32_t max_iter, double to
 g1_status g1_ppr(g1_graph* graph, uint32_t seed_vertex, float restart, uint32_t max_iter, double tol,
                  float* rank_dev, g1_run_stats* stats, g1_stream_t stream);
 
+/* ---- batched multi-source engines ------------------------------------------------------------
+ * No single counterpart in the reference: they run many single-source queries of the reference's apps
+ * (apps/sssp_blas.cu:60-80 for min-plus, apps/pagerank_blas.cu:93-123 for the PageRank iteration)
+ * together, as the columns of one semiring SpMM over the in-edge CSR. Sources are cut into column tiles
+ * of k in {8, 16, 32, 64}; k = 0 picks the autotuned width of the graph (g1_batch_autotune on first use).
+ * Column c of the output (out_dev + c * n) is exactly what the single-source call returns for
+ * sources[c]: bit-identical distances, and bit-identical PPR ranks (same fold order of every row sum,
+ * same per-256-vertex FP64 L1 tree, same stop rule per column). The graph needs its transpose. */
+typedef struct g1_batch_stats {
+    uint32_t k;               /* column tile width used                               */
+    uint32_t tiles;           /* column tiles run                                     */
+    uint64_t iterations_sum;  /* SpMM iterations summed over the tiles                */
+    double elapsed_ms;        /* device time of the whole call (autotuning excluded)  */
+} g1_batch_stats;
+
+#define G1_BATCH_MINPLUS 0
+#define G1_BATCH_PPR 1
+
+/* Times one tile of every width on up to 64 of `sources` (repeated when fewer) and stores the width with
+ * the lowest time per source as the graph's default for `kind`. weighted selects the min-plus variant
+ * timed (ignored for G1_BATCH_PPR). */
+g1_status g1_batch_autotune(g1_graph* graph, int32_t kind, const uint32_t* sources, uint32_t count,
+                            int32_t weighted, uint32_t* best_k, g1_stream_t stream);
+
+/* Multi-source SSSP (weighted != 0) or BFS depths (weighted == 0): dist_dev holds count x n u32,
+ * source-major. sources are host memory. */
+g1_status g1_sssp_batch(g1_graph* graph, const uint32_t* sources, uint32_t count, uint32_t k,
+                        int32_t weighted, uint32_t* dist_dev, g1_batch_stats* stats, g1_stream_t stream);
+
+/* Multi-seed personalized PageRank (g1_ppr per column): rank_dev holds count x n floats, seed-major. */
+g1_status g1_ppr_batch(g1_graph* graph, const uint32_t* seeds, uint32_t count, float restart,
+                       uint32_t max_iter, double tol, uint32_t k, float* rank_dev, g1_batch_stats* stats,
+                       g1_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
