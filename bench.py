@@ -83,6 +83,179 @@ def cpu_baseline(host, src: int, budget_s: float) -> dict:
             "sample": sample, "seconds": round(secs, 3)}
 
 
+def _wall(fn, reps: int = 1) -> float:
+    import graphone_amd as g1
+
+    g1.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    g1.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def batched_queries(g1, g, host, count: int, seed: int) -> dict:
+    """Multi-source queries on the same graph: `count` SSSP sources and `count` personalized-PageRank seeds
+    (restart 0.15), drawn uniformly without replacement from the vertices with out-degree > 0. Each runs
+    once as a batch (columns of one SpMM, every width timed, the autotuned one reported) and once as
+    sequential single-source calls (SSSP: the push engine, the fastest single-source engine; PPR: g1_ppr);
+    every batched column is compared with its single-source result."""
+    rng = np.random.default_rng(seed)
+    deg = host.out_degrees()
+    srcs = rng.choice(np.flatnonzero(deg > 0), size=count, replace=False).astype(np.uint32)
+    n = host.n
+    out = {"sources": count, "source_seed": seed}
+
+    # ---- SSSP
+    dist = g1.DeviceArray(count * n, np.uint32)
+    one = g1.DeviceArray(n, np.uint32)
+    widths = {}
+    for k in (8, 16, 32, 64):
+        g.sssp_batch(srcs, k=k, out=dist, to_host=False)  # warm: allocation and graph capture
+        widths[k] = _wall(lambda: g.sssp_batch(srcs, k=k, out=dist, to_host=False)) * 1e3
+    k_auto = g.autotune("minplus", srcs, weighted=True)
+    res = g.sssp_batch(srcs, k=k_auto, out=dist, to_host=False)
+    t_batch = _wall(lambda: g.sssp_batch(srcs, k=k_auto, out=dist, to_host=False))
+    g.sssp(int(srcs[0]), "advance", out=one, to_host=False)
+    t_seq = _wall(lambda: [g.sssp(int(s), "advance", out=one, to_host=False) for s in srcs])
+    allcols = dist.numpy().reshape(count, n)
+    same = sum(int(np.array_equal(allcols[c], g.sssp(int(s), "advance", out=one).values))
+               for c, s in enumerate(srcs))
+    out["sssp"] = {
+        "k_autotuned": k_auto, "ms_by_k": {str(k): round(v, 3) for k, v in widths.items()},
+        "batch_ms": round(t_batch * 1e3, 3), "sequential_ms": round(t_seq * 1e3, 3),
+        "batch_qps": round(count / t_batch, 1), "sequential_qps": round(count / t_seq, 1),
+        "speedup": round(t_seq / t_batch, 3), "iterations_sum": res.stats["iterations_sum"],
+        "columns_bit_exact": f"{same}/{count}",
+    }
+    del dist, allcols
+
+    # ---- personalized PageRank
+    ranks = g1.DeviceArray(count * n, np.float32)
+    rone = g1.DeviceArray(n, np.float32)
+    widths = {}
+    for k in (8, 16, 32, 64):
+        g.ppr_batch(srcs, 0.15, 50, 1e-7, k=k, out=ranks, to_host=False)
+        widths[k] = _wall(lambda: g.ppr_batch(srcs, 0.15, 50, 1e-7, k=k, out=ranks, to_host=False)) * 1e3
+    k_auto = g.autotune("ppr", srcs)
+    res = g.ppr_batch(srcs, 0.15, 50, 1e-7, k=k_auto, out=ranks, to_host=False)
+    t_batch = _wall(lambda: g.ppr_batch(srcs, 0.15, 50, 1e-7, k=k_auto, out=ranks, to_host=False))
+    t_seq = _wall(lambda: [g.ppr(int(s), 0.15, 50, 1e-7, out=rone, to_host=False) for s in srcs])
+    allcols = ranks.numpy().reshape(count, n)
+    same = 0
+    iters = []
+    for c, s in enumerate(srcs):
+        r = g.ppr(int(s), 0.15, 50, 1e-7, out=rone)
+        iters.append(r.stats["iterations"])
+        same += int(np.array_equal(allcols[c].view(np.uint32), r.values.view(np.uint32)))
+    out["ppr"] = {
+        "restart": 0.15, "max_iter": 50, "tol_l1": 1e-7,
+        "k_autotuned": k_auto, "ms_by_k": {str(k): round(v, 3) for k, v in widths.items()},
+        "batch_ms": round(t_batch * 1e3, 3), "sequential_ms": round(t_seq * 1e3, 3),
+        "batch_qps": round(count / t_batch, 1), "sequential_qps": round(count / t_seq, 1),
+        "speedup": round(t_seq / t_batch, 3), "iterations_sum": res.stats["iterations_sum"],
+        "single_iterations_mean": round(float(np.mean(iters)), 2),
+        "columns_bit_identical": f"{same}/{count}",
+    }
+    return out
+
+
+def serving_benchmark(g1, args) -> dict:
+    """Query serving over four resident graphs: RMAT scale 18, 20 and 23 (edge factor 16, a/b/c/d as the
+    main workload, u32 weights [1, 255]) and a 2048 x 2048 grid with edge retention 0.6 (symmetric u32
+    weights [1, 255]). Poisson arrivals at each rate (seeded), mix 40% BFS / 40% SSSP / 20% personalized
+    PageRank (restart 0.15, 50 iterations or L1 < 1e-7), uniformly random graph and source. Reported per
+    rate: achieved queries/s (completed / span from first submit to last completion), p50/p99 latency
+    (submit to done, server clock), admission waits and how many ran batched. The plain baseline runs a
+    sample of the same queries one at a time through the single-source calls."""
+    t0 = time.perf_counter()
+    specs = [("rmat18", lambda: g1.HostGraph.rmat(18, 16, 0.57, 0.19, 0.19, seed=11).assign_weights(1, 255, seed=12)),
+             ("rmat20", lambda: g1.HostGraph.rmat(20, 16, 0.57, 0.19, 0.19, seed=13).assign_weights(1, 255, seed=14)),
+             ("rmat23", lambda: g1.HostGraph.rmat(23, 16, 0.57, 0.19, 0.19, seed=15).assign_weights(1, 255, seed=16)),
+             ("grid2048", lambda: g1.HostGraph.grid(2048, 2048, 0.6, seed=17).assign_weights(1, 255, seed=18,
+                                                                                             symmetric=True))]
+    graphs, sizes = [], []
+    for name, make in specs:
+        h = make()
+        graphs.append(g1.Graph(h, symmetric=name.startswith("grid")))
+        sizes.append({"graph": name, "n": h.n, "m": h.m})
+        del h
+    log(f"serving graphs ready in {time.perf_counter() - t0:.1f}s")
+    kinds = np.array(["bfs", "sssp", "ppr"])
+
+    def make_queries(count, seed):
+        rng = np.random.default_rng(seed)
+        ks = rng.choice(3, size=count, p=[0.4, 0.4, 0.2])
+        gs = rng.integers(0, len(graphs), size=count)
+        srcs = [int(rng.integers(0, graphs[gi].n)) for gi in gs]
+        return [(int(gi), str(kinds[k]), s) for gi, k, s in zip(gs, ks, srcs)]
+
+    # plain baseline: one query at a time, single-source calls
+    sample = make_queries(120, 1000)
+    outs = [g1.DeviceArray(gr.n, np.uint32) for gr in graphs]
+    routs = [g1.DeviceArray(gr.n, np.float32) for gr in graphs]
+
+    def run_one(gi, kind, src):
+        gr = graphs[gi]
+        if kind == "bfs":
+            gr.bfs(src, "diropt", out=outs[gi], to_host=False)
+        elif kind == "sssp":
+            gr.sssp(src, "advance", out=outs[gi], to_host=False)
+        else:
+            gr.ppr(src, 0.15, 50, 1e-7, out=routs[gi], to_host=False)
+
+    for q in sample[:6]:
+        run_one(*q)
+    g1.synchronize()
+    ts = time.perf_counter()
+    for q in sample:
+        run_one(*q)
+    g1.synchronize()
+    seq_s = time.perf_counter() - ts
+    seq_qps = len(sample) / seq_s
+    log(f"sequential single-query baseline: {seq_qps:.1f} q/s over {len(sample)} queries")
+    del outs, routs
+
+    results = []
+    for ri, rate in enumerate(float(r) for r in args.serve_rates.split(",")):
+        count = int(min(args.serve_max_queries, max(50, rate * args.serve_seconds)))
+        qs = make_queries(count, 2000 + ri)
+        rng = np.random.default_rng(3000 + ri)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+        srv = g1.Server(batch_k=64)
+        ids = [srv.add_graph(gr) for gr in graphs]
+        # warm: one query of each kind on each graph (sessions, lazily built arrays)
+        warm = [srv.submit(ids[gi], k, 0) for gi in range(len(graphs)) for k in ("bfs", "sssp", "ppr")]
+        for q in warm:
+            srv.wait(q, 600000)
+            srv.cancel(q)
+        qids = []
+        start = time.perf_counter()
+        for (gi, kind, src), at in zip(qs, arrivals):
+            while time.perf_counter() - start < at:
+                pass
+            qids.append(srv.submit(ids[gi], kind, src, 50, 1e-7))
+        sts = [srv.wait(q, 600000) for q in qids]
+        lat = np.array([s["t_done_ms"] - s["t_submit_ms"] for s in sts])
+        span = (max(s["t_done_ms"] for s in sts) - min(s["t_submit_ms"] for s in sts)) / 1e3
+        st = srv.stats()
+        ok = sum(1 for s in sts if s["state_name"] == "done")
+        for q in qids:
+            srv.cancel(q)  # discard the results
+        srv.close()
+        r = {"rate_qps": rate, "queries": count, "completed": ok, "achieved_qps": round(ok / span, 1),
+             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+             "admission_waits": st["admission_waits"], "batched": sum(1 for s in sts if s["path"] == 1),
+             "session_steps": st["session_steps"],
+             "hbm_peak_gb": round(st["hbm_peak"] / 1e9, 2)}
+        results.append(r)
+        log(json.dumps(r))
+    best = max(r["achieved_qps"] for r in results)
+    return {"graphs": sizes, "mix": {"bfs": 0.4, "sssp": 0.4, "ppr": 0.2}, "ppr": {"restart": 0.15, "max_iter": 50,
+            "tol_l1": 1e-7}, "sequential_qps": round(seq_qps, 1), "sequential_sample": len(sample),
+            "rates": results, "best_achieved_qps": best, "speedup_vs_sequential": round(best / seq_qps, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +267,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-sources", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", default="100,1000,3000,10000")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-max-queries", type=int, default=600)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -171,6 +350,17 @@ def main() -> int:
                                    "gteps": round(m_reach / rb.stats["elapsed_ms"] / 1e6, 3)}
         log(json.dumps(extras))
 
+    batch = None
+    if not args.no_batch:
+        batch = batched_queries(g1, g, host, args.batch_sources, args.seed + 2)
+        log(json.dumps(batch))
+
+    serving = None
+    if not args.no_serve:
+        del g
+        serving = serving_benchmark(g1, args)
+        log(json.dumps(serving))
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -207,6 +397,8 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "extras": extras,
+        "batch": batch,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -341,6 +341,96 @@ class Graph:
             pass
 
 
+# ----------------------------------------------------------------------------------------- query server
+
+QUERY_KINDS = {"bfs": _lib.G1_QUERY_BFS, "sssp": _lib.G1_QUERY_SSSP, "ppr": _lib.G1_QUERY_PPR}
+QUERY_STATES = {_lib.G1_Q_QUEUED: "queued", _lib.G1_Q_RUNNING: "running", _lib.G1_Q_DONE: "done",
+                _lib.G1_Q_FAILED: "failed", _lib.G1_Q_CANCELLED: "cancelled", _lib.G1_Q_FETCHED: "fetched"}
+
+
+class Server:
+    """The query server of the C-ABI (g1_server_*): resident graphs, per-graph request loops,
+    HBM-budget admission and continuous batching of SSSP / personalized-PageRank queries."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, batch_k: int = 0, batch_min_sssp: int = 0,
+                 batch_min_ppr: int = 0, ppr_restart: float = 0.0):
+        cfg = _lib.ServerConfig()
+        cfg.hbm_budget_bytes = int(hbm_budget_bytes)
+        cfg.batch_k = int(batch_k)
+        cfg.batch_min_sssp = int(batch_min_sssp)
+        cfg.batch_min_ppr = int(batch_min_ppr)
+        cfg.ppr_restart = float(ppr_restart)
+        h = C.c_void_p()
+        call("g1_server_create", C.byref(cfg), C.byref(h))
+        self._h = h
+        self._graphs: list = []
+
+    def add_graph(self, graph: "Graph") -> int:
+        gid = C.c_uint32()
+        call("g1_server_add_graph", self._h, graph.handle, C.byref(gid))
+        self._graphs.append(graph)  # the graph must outlive the server
+        return gid.value
+
+    def submit(self, graph_id: int, kind: str, source: int, max_iter: int = 50, tol: float = 1e-7) -> int:
+        q = _lib.Query()
+        q.graph_id = int(graph_id)
+        q.kind = QUERY_KINDS[kind] if isinstance(kind, str) else int(kind)
+        q.source = int(source)
+        q.max_iter = int(max_iter)
+        q.tol = float(tol)
+        qid = C.c_uint64()
+        call("g1_server_submit", self._h, C.byref(q), C.byref(qid))
+        return qid.value
+
+    @staticmethod
+    def _status(st) -> dict:
+        d = st.as_dict()
+        d["state_name"] = QUERY_STATES.get(d["state"], "?")
+        d["latency_ms"] = d["t_done_ms"] - d["t_submit_ms"] if d["t_done_ms"] else None
+        return d
+
+    def poll(self, qid: int) -> dict:
+        st = _lib.QueryStatus()
+        call("g1_server_poll", self._h, C.c_uint64(qid), C.byref(st))
+        return self._status(st)
+
+    def wait(self, qid: int, timeout_ms: float = 60000.0) -> dict:
+        st = _lib.QueryStatus()
+        call("g1_server_wait", self._h, C.c_uint64(qid), C.c_double(timeout_ms), C.byref(st))
+        return self._status(st)
+
+    def fetch(self, qid: int, kind: str | None = None, out: DeviceArray | None = None):
+        """Copies the result of a done query (to host numpy, or into `out` on the device) and frees it."""
+        st = self.poll(qid)
+        n = st["result_bytes"] // 4
+        if out is not None:
+            call("g1_server_fetch", self._h, C.c_uint64(qid), C.c_void_p(out.ptr), C.c_uint64(out.nbytes), 1)
+            return out
+        dtype = np.float32 if kind == "ppr" else np.uint32
+        arr = np.empty(n, dtype=dtype)
+        call("g1_server_fetch", self._h, C.c_uint64(qid), C.c_void_p(arr.ctypes.data), C.c_uint64(arr.nbytes), 0)
+        return arr
+
+    def cancel(self, qid: int) -> None:
+        call("g1_server_cancel", self._h, C.c_uint64(qid))
+
+    def stats(self) -> dict:
+        st = _lib.ServerStats()
+        call("g1_server_get_stats", self._h, C.byref(st))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().g1_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ----------------------------------------------------------------------------------------- SpMV
 
 def spmv(kind: str, semiring: str, row_offsets: DeviceArray, col_indices: DeviceArray, values, x: DeviceArray,

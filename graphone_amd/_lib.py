@@ -36,6 +36,17 @@ G1_PR_MAX_ABS = 1
 G1_BATCH_MINPLUS = 0
 G1_BATCH_PPR = 1
 
+G1_QUERY_BFS = 0
+G1_QUERY_SSSP = 1
+G1_QUERY_PPR = 2
+
+G1_Q_QUEUED = 0
+G1_Q_RUNNING = 1
+G1_Q_DONE = 2
+G1_Q_FAILED = 3
+G1_Q_CANCELLED = 4
+G1_Q_FETCHED = 5
+
 
 class G1Error(RuntimeError):
     def __init__(self, status: int, where: str, message: str):
@@ -78,6 +89,52 @@ class BatchStats(C.Structure):
         ("iterations_sum", C.c_uint64),
         ("elapsed_ms", C.c_double),
     ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("batch_k", C.c_uint32),
+        ("batch_min_sssp", C.c_uint32),
+        ("batch_min_ppr", C.c_uint32),
+        ("ppr_restart", C.c_float),
+        ("reserved", C.c_uint32 * 3),
+    ]
+
+
+class Query(C.Structure):
+    _fields_ = [
+        ("graph_id", C.c_uint32),
+        ("kind", C.c_int32),
+        ("source", C.c_uint32),
+        ("max_iter", C.c_uint32),
+        ("tol", C.c_double),
+    ]
+
+
+class QueryStatus(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32),
+        ("status", C.c_int32),
+        ("path", C.c_int32),
+        ("iterations", C.c_uint32),
+        ("t_submit_ms", C.c_double),
+        ("t_admit_ms", C.c_double),
+        ("t_done_ms", C.c_double),
+        ("result_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(name, C.c_uint64) for name in (
+        "submitted", "completed", "failed", "cancelled", "queued", "running", "admission_waits", "batched",
+        "single", "session_steps", "hbm_budget", "hbm_used", "hbm_peak")]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -145,6 +202,15 @@ _SIGNATURES = {
     "g1_batch_autotune": (_st, [_vp, _i32, _pu32, _u32, _i32, _pu32, _vp]),
     "g1_sssp_batch": (_st, [_vp, _pu32, _u32, _u32, _i32, _vp, C.POINTER(BatchStats), _vp]),
     "g1_ppr_batch": (_st, [_vp, _pu32, _u32, C.c_float, _u32, C.c_double, _u32, _vp, C.POINTER(BatchStats), _vp]),
+    "g1_server_create": (_st, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    "g1_server_destroy": (_st, [_vp]),
+    "g1_server_add_graph": (_st, [_vp, _vp, C.POINTER(_u32)]),
+    "g1_server_submit": (_st, [_vp, C.POINTER(Query), C.POINTER(_u64)]),
+    "g1_server_poll": (_st, [_vp, _u64, C.POINTER(QueryStatus)]),
+    "g1_server_wait": (_st, [_vp, _u64, C.c_double, C.POINTER(QueryStatus)]),
+    "g1_server_fetch": (_st, [_vp, _u64, _vp, _u64, _i32]),
+    "g1_server_cancel": (_st, [_vp, _u64]),
+    "g1_server_get_stats": (_st, [_vp, C.POINTER(ServerStats)]),
 }
 
 

@@ -14,6 +14,7 @@
 #include "g1/apply.hpp"
 #include "g1/spmm.hpp"
 #include "graph.hpp"
+#include "session.hpp"
 #include "spmv_launch.hpp"
 
 struct BatchWork {
@@ -21,8 +22,7 @@ struct BatchWork {
     uint32_t* D = nullptr;                                 // n x K distances
     unsigned long long* mask[2] = {nullptr, nullptr};      // changed-column masks
     void* recs = nullptr;                                  // SpMM tile records
-    uint32_t* tile_counts = nullptr;
-    unsigned long long* active = nullptr;                  // rows changed by the last iteration
+    unsigned long long* changed = nullptr;                 // [0], [1]: columns changed in iteration 1, 2 of a replay
     float* R[2] = {nullptr, nullptr};                      // PPR ranks, n x K
     float* X = nullptr;
     float* Y = nullptr;
@@ -53,15 +53,13 @@ g1_status ensure_batch(g1_graph* g, uint32_t K, bool ppr, BatchWork** out) {
     }
     BatchWork* b = g->batch[slot];
     const size_t n = g->n ? g->n : 1;
-    const uint32_t tiles = merge_num_tiles(g->n, g->m);
     if (!b->recs) {
         const size_t rb = std::max(spmm_records_bytes<MinPlusU32, 64>(g->n, g->m),
                                    spmm_records_bytes<PlusTimesKahanF32, 64>(g->n, g->m));
         void* v = nullptr;
         G1_TRY(graph_alloc(g, &v, rb + 256));
         b->recs = v;
-        G1_TRY(alloc_n(g, &b->tile_counts, tiles + 1));
-        G1_TRY(alloc_n(g, &b->active, 4));
+        G1_TRY(alloc_n(g, &b->changed, 64));
     }
     if (!ppr && !b->D) {
         G1_TRY(alloc_n(g, &b->D, n * K));
@@ -117,28 +115,25 @@ __global__ void ppr_batch_init_kernel(uint32_t n, const float* __restrict__ iw, 
     }
 }
 
-__global__ void sum_counts_kernel(const uint32_t* __restrict__ c, uint32_t count, unsigned long long* __restrict__ out) {
-    __shared__ unsigned long long s[256];
-    unsigned long long acc = 0;
-    for (uint32_t i = threadIdx.x; i < count; i += 256) acc += c[i];
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int i = 0; i < 256; ++i) t += s[i];
-        out[0] = t;
-    }
+__global__ void clear_flags_kernel(unsigned long long* __restrict__ f) {
+    if (threadIdx.x < 2) f[threadIdx.x] = 0ull;
 }
 
-// out[(col0 + j) * n + v] = D[v * K + j] for the tile's valid columns.
+// out[j * n + v] = D[v * K + j] for the tile's valid columns j < cols: 64 rows of D pass through LDS so
+// that both the row-major reads and the column-major writes are coalesced.
 template <int K, typename T>
-__global__ void batch_columns_out_kernel(uint32_t n, const T* __restrict__ D, uint32_t cols, T* __restrict__ out) {
-    const size_t total = static_cast<size_t>(n) * cols;
-    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const uint32_t j = static_cast<uint32_t>(i / n);
-        const uint32_t v = static_cast<uint32_t>(i % n);
-        out[i] = D[static_cast<size_t>(v) * K + j];
+__global__ void __launch_bounds__(256) batch_columns_out_kernel(uint32_t n, const T* __restrict__ D, uint32_t cols,
+                                                                T* __restrict__ out) {
+    __shared__ T s[K][65];
+    const uint32_t v0 = blockIdx.x * 64;
+    for (uint32_t e = threadIdx.x; e < 64u * K; e += 256) {
+        const uint32_t r = e / K, c = e % K;
+        if (v0 + r < n) s[c][r] = D[static_cast<size_t>(v0 + r) * K + c];
+    }
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < 64u * cols; f += 256) {
+        const uint32_t c = f / 64, r = f % 64;
+        if (v0 + r < n) out[static_cast<size_t>(c) * n + v0 + r] = s[c][r];
     }
 }
 
@@ -152,21 +147,20 @@ uint32_t fill_blocks(size_t items) {
 // count of changed rows into b->active.
 template <int K>
 g1_status minplus_iteration(g1_graph* g, BatchWork* b, int prev, bool weighted, hipStream_t s) {
+    unsigned long long* flag = b->changed + prev;
     const uint32_t tiles = merge_num_tiles(g->n, g->m);
     MinPlusBatchPolicy<K> pol{b->D, b->mask[prev], b->mask[prev ^ 1]};
     auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(b->recs);
     const uint32_t* vals = weighted ? g->t_w : nullptr;
     spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
-        g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, b->tile_counts);
+        g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, flag);
     G1_LAUNCHED();
     if (tiles > 1) {
         const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
         spmm_fixup_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<blocks, 256, 0, s>>>(tiles, merge_coords(g->t_ws),
-                                                                                     recs, pol, b->tile_counts);
+                                                                                     recs, pol, flag);
         G1_LAUNCHED();
     }
-    sum_counts_kernel<<<1, 256, 0, s>>>(b->tile_counts, tiles, b->active);
-    G1_LAUNCHED();
     return G1_OK;
 }
 
@@ -190,6 +184,7 @@ g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host,
         g1_status st = G1_OK;
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
+            clear_flags_kernel<<<1, 64, 0, cs>>>(b->changed);
             st = minplus_iteration<K>(g, b, 0, weighted, cs);
             if (st == G1_OK) st = minplus_iteration<K>(g, b, 1, weighted, cs);
             e = hipStreamEndCapture(cs, &graph);
@@ -201,16 +196,19 @@ g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host,
         (void)hipGraphDestroy(graph);
         G1_HIP(e);
     }
+    // Replays until an iteration changes nothing; the count includes that last, unchanged iteration.
     uint32_t it = 0;
+    unsigned long long* hf = g->h_ctr + 12;
     for (;;) {
         G1_HIP(hipGraphLaunch(exec, s));
-        it += 2;
-        G1_HIP(hipMemcpyAsync(g->h_ctr + 12, b->active, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        G1_HIP(hipMemcpyAsync(hf, b->changed, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         G1_HIP(hipStreamSynchronize(s));
-        if (g->h_ctr[12] == 0 || it >= kHistIters) break;
+        if (hf[0] == 0) { it += 1; break; }
+        it += 2;
+        if (hf[1] == 0 || it >= kHistIters) break;
     }
     *iterations = it;
-    batch_columns_out_kernel<K, uint32_t><<<fill_blocks(static_cast<size_t>(n) * cols), 256, 0, s>>>(n, b->D, cols, out);
+    batch_columns_out_kernel<K, uint32_t><<<(n + 63) / 64, 256, 0, s>>>(n, b->D, cols, out);
     G1_LAUNCHED();
     return G1_OK;
 }
@@ -226,8 +224,7 @@ g1_status ppr_tile_run(g1_graph* g, BatchWork* b, const uint32_t* seeds_host, ui
     for (uint32_t j = 0; j < cols; ++j) seeds[j] = seeds_host[j];
     G1_HIP(hipMemcpyAsync(g->seeds_dev, seeds.data(), K * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     // iw from the out-edges, then R0 = e_seed per column, X0 = R0 * iw
-    pr_init_kernel<<<fill_blocks(n), 256, 0, s>>>(n, g->row_off, g->w, 1.0f - restart, g->iw, g->r[0], 0.0f, g->xf,
-                                                  kInf);
+    pr_init_kernel<<<fill_blocks(n), 256, 0, s>>>(n, g->osum, 1.0f - restart, g->iw, g->r[0], 0.0f, g->xf, kInf);
     G1_LAUNCHED();
     ppr_batch_init_kernel<K><<<fill_blocks(static_cast<size_t>(n) * K), 256, 0, s>>>(n, g->iw, g->seeds_dev, b->R[0],
                                                                                       b->X);
@@ -251,8 +248,8 @@ g1_status ppr_tile_run(g1_graph* g, BatchWork* b, const uint32_t* seeds_host, ui
                 tiles, merge_coords(g->t_ws), recs, pol, nullptr);
             G1_LAUNCHED();
         }
-        rank_apply_kernel<K><<<nb, 256, 0, s>>>(n, b->Y, 0.0f, g->seeds_dev, restart, g->iw, b->R[cur], b->R[cur ^ 1],
-                                                b->X, g->rpart, nb, frozen);
+        rank_apply_batch_kernel<K><<<(nb + 256 / K - 1) / (256 / K), 256, 0, s>>>(
+            n, b->Y, 0.0f, g->seeds_dev, restart, g->iw, b->R[cur], b->R[cur ^ 1], b->X, g->rpart, nb, frozen);
         G1_LAUNCHED();
         reduce_partials_kernel<0><<<dim3(1, K), 256, 0, s>>>(g->rpart, nb, g->rpart + static_cast<size_t>(nb) * kBatchCols);
         G1_LAUNCHED();
@@ -269,7 +266,7 @@ g1_status ppr_tile_run(g1_graph* g, BatchWork* b, const uint32_t* seeds_host, ui
     }
     // A column frozen at iteration t kept its rank in both buffers since, so R[cur] holds every column.
     *iterations = it;
-    batch_columns_out_kernel<K, float><<<fill_blocks(static_cast<size_t>(n) * cols), 256, 0, s>>>(n, b->R[cur], cols, out);
+    batch_columns_out_kernel<K, float><<<(n + 63) / 64, 256, 0, s>>>(n, b->R[cur], cols, out);
     G1_LAUNCHED();
     return G1_OK;
 }
@@ -385,7 +382,351 @@ g1_status autotune(g1_graph* g, int kind, const uint32_t* srcs, uint32_t count, 
     return G1_OK;
 }
 
+// ---------------------------------------------------------------- continuous-batching sessions
+
+// Column col of a min-plus session starts over from `src` (kInf: the column is emptied).
+template <int K>
+__global__ void minplus_column_reset_kernel(uint32_t n, uint32_t* __restrict__ D, unsigned long long* __restrict__ mask,
+                                            uint32_t col, uint32_t src) {
+    const unsigned long long bit = 1ull << col;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        D[static_cast<size_t>(v) * K + col] = (v == src) ? 0u : kInf;
+        mask[v] = (mask[v] & ~bit) | ((v == src) ? bit : 0ull);
+    }
+}
+
+// Column col of a PPR session starts from e_seed: R = e_seed, X = R * iw (pr_init_kernel's seed variant).
+template <int K>
+__global__ void ppr_column_reset_kernel(uint32_t n, const float* __restrict__ iw, float* __restrict__ R,
+                                        float* __restrict__ X, uint32_t col, uint32_t seed) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const float rv = (v == seed) ? 1.0f : 0.0f;
+        R[static_cast<size_t>(v) * K + col] = rv;
+        X[static_cast<size_t>(v) * K + col] = rv * iw[v];
+    }
+}
+
+template <int K, typename T>
+__global__ void column_extract_kernel(uint32_t n, const T* __restrict__ A, uint32_t col, T* __restrict__ out) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) out[v] = A[static_cast<size_t>(v) * K + col];
+}
+
+// iw[u] = alpha / osum[u] (0 without out-weight): the expression of pr_init_kernel, so the same bits.
+__global__ void inv_weight_kernel(uint32_t n, const float* __restrict__ osum, float alpha, float* __restrict__ iw) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += stride) {
+        const float sw = osum[u];
+        iw[u] = (sw != 0.0f) ? alpha / sw : 0.0f;
+    }
+}
+
+__global__ void set_u32_kernel(uint32_t* __restrict__ p, uint32_t v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
 }  // namespace
+
+struct Session {
+    g1_graph* g = nullptr;
+    int kind = 0;
+    uint32_t K = 0;
+    float restart = 0.0f;
+    hipStream_t s = nullptr;
+    uint32_t* D = nullptr;
+    unsigned long long* mask[2] = {nullptr, nullptr};
+    int mcur = 0;  // mask[mcur]: the sources of the next iteration
+    float* R[2] = {nullptr, nullptr};
+    float* X = nullptr;
+    float* Y = nullptr;
+    float* iw = nullptr;
+    int rcur = 0;  // R[rcur]: the latest iterate
+    double* part = nullptr;
+    double* deltas = nullptr;
+    uint32_t* seeds_dev = nullptr;
+    void* recs = nullptr;
+    unsigned long long* changed = nullptr;
+    unsigned long long* h_changed = nullptr;  // pinned
+    double* h_deltas = nullptr;               // pinned
+    uint64_t active = 0;
+    uint64_t converged = 0;
+    uint32_t iters[kBatchCols] = {};
+    uint32_t max_iter[kBatchCols] = {};
+    double tol[kBatchCols] = {};
+    std::vector<void*> owned;
+};
+
+namespace {
+
+size_t records_bytes(const g1_graph* g, int kind, uint32_t k) {
+    auto pick = [&](auto tag) -> size_t {
+        constexpr int KK = decltype(tag)::value;
+        return kind == kSessPpr ? spmm_records_bytes<PlusTimesKahanF32, KK>(g->n, g->m)
+                                : spmm_records_bytes<MinPlusU32, KK>(g->n, g->m);
+    };
+    switch (k) {
+        case 8: return pick(std::integral_constant<int, 8>());
+        case 16: return pick(std::integral_constant<int, 16>());
+        case 32: return pick(std::integral_constant<int, 32>());
+        default: return pick(std::integral_constant<int, 64>());
+    }
+}
+
+template <typename T>
+g1_status sess_alloc(Session* ss, T** p, size_t count) {
+    void* v = nullptr;
+    G1_HIP(hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T)));
+    ss->owned.push_back(v);
+    *p = static_cast<T*>(v);
+    return G1_OK;
+}
+
+template <int K>
+g1_status session_step_minplus(Session* ss, uint64_t* done) {
+    g1_graph* g = ss->g;
+    const uint32_t tiles = merge_num_tiles(g->n, g->m);
+    const hipStream_t s = ss->s;
+    clear_flags_kernel<<<1, 64, 0, s>>>(ss->changed);
+    G1_LAUNCHED();
+    MinPlusBatchPolicy<K> pol{ss->D, ss->mask[ss->mcur], ss->mask[ss->mcur ^ 1]};
+    auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(ss->recs);
+    const uint32_t* vals = ss->kind == kSessMinPlusWeighted ? g->t_w : nullptr;
+    spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+        g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, ss->changed);
+    G1_LAUNCHED();
+    if (tiles > 1) {
+        const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
+        spmm_fixup_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<blocks, 256, 0, s>>>(tiles, merge_coords(g->t_ws), recs,
+                                                                                     pol, ss->changed);
+        G1_LAUNCHED();
+    }
+    G1_HIP(hipMemcpyAsync(ss->h_changed, ss->changed, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    G1_HIP(hipStreamSynchronize(s));
+    ss->mcur ^= 1;
+    const uint64_t ch = ss->h_changed[0];
+    uint64_t d = 0;
+    for (uint32_t j = 0; j < K; ++j) {
+        if (!((ss->active >> j) & 1ull) || ((ss->converged >> j) & 1ull)) continue;
+        ++ss->iters[j];
+        // Bellman-Ford settles within n rounds; the bound only guards the loop.
+        if (!((ch >> j) & 1ull) || ss->iters[j] > g->n + 1) d |= 1ull << j;
+    }
+    ss->converged |= d;
+    *done = d;
+    return G1_OK;
+}
+
+template <int K>
+g1_status session_step_ppr(Session* ss, uint64_t* done) {
+    g1_graph* g = ss->g;
+    const uint32_t n = g->n;
+    const uint32_t tiles = merge_num_tiles(g->n, g->m);
+    const uint32_t nb = rank_blocks(n);
+    const hipStream_t s = ss->s;
+    PlusTimesBatchPolicy<K> pol{ss->X, ss->Y};
+    auto* recs = static_cast<SpmmTileRec<PlusTimesKahanF32, K>*>(ss->recs);
+    spmm_tile_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+        g->n, g->m, g->t_row_off, g->t_col, g->t_wf, merge_coords(g->t_ws), recs, pol, nullptr);
+    G1_LAUNCHED();
+    if (tiles > 1) {
+        const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
+        spmm_fixup_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<blocks, 256, 0, s>>>(
+            tiles, merge_coords(g->t_ws), recs, pol, nullptr);
+        G1_LAUNCHED();
+    }
+    const unsigned long long live = ss->active & ~ss->converged;
+    rank_apply_batch_kernel<K><<<(nb + 256 / K - 1) / (256 / K), 256, 0, s>>>(
+        n, ss->Y, 0.0f, ss->seeds_dev, ss->restart, ss->iw, ss->R[ss->rcur], ss->R[ss->rcur ^ 1], ss->X, ss->part, nb,
+        ~live);
+    G1_LAUNCHED();
+    reduce_partials_kernel<0><<<dim3(1, K), 256, 0, s>>>(ss->part, nb, ss->deltas);
+    G1_LAUNCHED();
+    G1_HIP(hipMemcpyAsync(ss->h_deltas, ss->deltas, K * sizeof(double), hipMemcpyDeviceToHost, s));
+    G1_HIP(hipStreamSynchronize(s));
+    ss->rcur ^= 1;
+    uint64_t d = 0;
+    for (uint32_t j = 0; j < K; ++j) {
+        if (!((live >> j) & 1ull)) continue;
+        ++ss->iters[j];
+        // g1_ppr's rule: stop after the iteration whose L1 delta is below tol, or at max_iter.
+        if (ss->h_deltas[j] < ss->tol[j] || ss->iters[j] >= ss->max_iter[j]) d |= 1ull << j;
+    }
+    ss->converged |= d;
+    *done = d;
+    return G1_OK;
+}
+
+template <int K>
+g1_status session_insert_k(Session* ss, uint32_t col, uint32_t src) {
+    const uint32_t n = ss->g->n;
+    const uint32_t blocks = fill_blocks(n);
+    if (ss->kind == kSessPpr) {
+        set_u32_kernel<<<1, 64, 0, ss->s>>>(ss->seeds_dev + col, src);
+        G1_LAUNCHED();
+        ppr_column_reset_kernel<K><<<blocks, 256, 0, ss->s>>>(n, ss->iw, ss->R[ss->rcur], ss->X, col, src);
+    } else {
+        minplus_column_reset_kernel<K><<<blocks, 256, 0, ss->s>>>(n, ss->D, ss->mask[ss->mcur], col, src);
+    }
+    G1_LAUNCHED();
+    return G1_OK;
+}
+
+template <int K>
+g1_status session_extract_k(Session* ss, uint32_t col, void* out) {
+    const uint32_t n = ss->g->n;
+    if (ss->kind == kSessPpr)
+        column_extract_kernel<K, float><<<fill_blocks(n), 256, 0, ss->s>>>(n, ss->R[ss->rcur], col, static_cast<float*>(out));
+    else
+        column_extract_kernel<K, uint32_t><<<fill_blocks(n), 256, 0, ss->s>>>(n, ss->D, col, static_cast<uint32_t*>(out));
+    G1_LAUNCHED();
+    return G1_OK;
+}
+
+#define G1_SESSION_DISPATCH(fn, ss, ...)                                                              \
+    switch ((ss)->K) {                                                                               \
+        case 8: return fn<8>(ss, __VA_ARGS__);                                                       \
+        case 16: return fn<16>(ss, __VA_ARGS__);                                                     \
+        case 32: return fn<32>(ss, __VA_ARGS__);                                                     \
+        default: return fn<64>(ss, __VA_ARGS__);                                                     \
+    }
+
+}  // namespace
+
+size_t session_bytes(const g1_graph* g, int kind, uint32_t k) {
+    const size_t n = g->n ? g->n : 1;
+    size_t b = records_bytes(g, kind, k) + 64 * sizeof(unsigned long long);
+    if (kind == kSessPpr)
+        b += 4 * n * k * sizeof(float) + n * sizeof(float) +
+             (static_cast<size_t>(rank_blocks(g->n)) * k + k) * sizeof(double) + k * sizeof(uint32_t);
+    else
+        b += n * k * sizeof(uint32_t) + 2 * n * sizeof(unsigned long long);
+    return b;
+}
+
+g1_status session_create(g1_graph* g, int kind, uint32_t k, float restart, hipStream_t stream, Session** out) {
+    G1_CHECK(g != nullptr && out != nullptr, G1_ERR_INVALID_ARG, "session_create: null argument");
+    G1_CHECK(k_slot(k) >= 0, G1_ERR_INVALID_ARG, "column width must be 8, 16, 32 or 64 (got %u)", k);
+    G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "batched engines need the graph's transpose");
+    G1_CHECK(kind != kSessMinPlusWeighted || g->t_w != nullptr, G1_ERR_INVALID_ARG, "graph has no weights");
+    if (kind == kSessPpr) {
+        std::lock_guard<std::mutex> lock(g->mu);  // lazily built graph arrays (float weights, out-weight sums)
+        G1_TRY(ensure_pull_work(g, true));
+    }
+    Session* ss = new Session();
+    ss->g = g;
+    ss->kind = kind;
+    ss->K = k;
+    ss->restart = restart;
+    ss->s = stream;
+    auto fail = [&](g1_status st) {
+        session_destroy(ss);
+        return st;
+    };
+    const size_t n = g->n ? g->n : 1;
+    g1_status st = G1_OK;
+    void* recs = nullptr;
+    if ((st = sess_alloc(ss, reinterpret_cast<uint8_t**>(&recs), records_bytes(g, kind, k))) != G1_OK) return fail(st);
+    ss->recs = recs;
+    if ((st = sess_alloc(ss, &ss->changed, 64)) != G1_OK) return fail(st);
+    if (hipHostMalloc(reinterpret_cast<void**>(&ss->h_changed), 64 * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&ss->h_deltas), kBatchCols * sizeof(double)) != hipSuccess)
+        return fail(set_error(G1_ERR_OUT_OF_MEMORY, "session_create: pinned host memory"));
+    if (kind == kSessPpr) {
+        for (int i = 0; i < 2 && st == G1_OK; ++i) st = sess_alloc(ss, &ss->R[i], n * k);
+        if (st == G1_OK) st = sess_alloc(ss, &ss->X, n * k);
+        if (st == G1_OK) st = sess_alloc(ss, &ss->Y, n * k);
+        if (st == G1_OK) st = sess_alloc(ss, &ss->iw, n);
+        if (st == G1_OK) st = sess_alloc(ss, &ss->part, static_cast<size_t>(rank_blocks(g->n)) * k);
+        if (st == G1_OK) st = sess_alloc(ss, &ss->deltas, k);
+        if (st == G1_OK) st = sess_alloc(ss, &ss->seeds_dev, k);
+        if (st != G1_OK) return fail(st);
+        inv_weight_kernel<<<fill_blocks(g->n), 256, 0, stream>>>(g->n, g->osum, 1.0f - restart, ss->iw);
+        if (hipGetLastError() != hipSuccess) return fail(set_error(G1_ERR_HIP, "session_create: launch"));
+    } else {
+        if ((st = sess_alloc(ss, &ss->D, n * k)) != G1_OK) return fail(st);
+        for (int i = 0; i < 2 && st == G1_OK; ++i) st = sess_alloc(ss, &ss->mask[i], n);
+        if (st != G1_OK) return fail(st);
+        // every column empty: D = INF, no source bits in either mask
+        if ((st = g1_fill_u32(ss->D, kInf, n * k, stream)) != G1_OK) return fail(st);
+        for (int i = 0; i < 2 && st == G1_OK; ++i) st = g1_fill_u32(reinterpret_cast<uint32_t*>(ss->mask[i]), 0u, 2 * n, stream);
+        if (st != G1_OK) return fail(st);
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess) return fail(set_error(G1_ERR_HIP, "session_create: sync"));
+    *out = ss;
+    return G1_OK;
+}
+
+void session_destroy(Session* ss) {
+    if (!ss) return;
+    if (ss->s) (void)hipStreamSynchronize(ss->s);
+    for (void* p : ss->owned) (void)hipFree(p);
+    if (ss->h_changed) (void)hipHostFree(ss->h_changed);
+    if (ss->h_deltas) (void)hipHostFree(ss->h_deltas);
+    delete ss;
+}
+
+uint32_t session_width(const Session* ss) { return ss->K; }
+uint64_t session_active(const Session* ss) { return ss->active; }
+
+g1_status session_insert(Session* ss, uint32_t col, uint32_t source, uint32_t max_iter, double tol, bool* done_now) {
+    G1_CHECK(col < ss->K && !((ss->active >> col) & 1ull), G1_ERR_INVALID_ARG, "session_insert: column %u busy", col);
+    G1_CHECK(source < ss->g->n, G1_ERR_INVALID_ARG, "session_insert: source %u out of range", source);
+    ss->active |= 1ull << col;
+    ss->converged &= ~(1ull << col);
+    ss->iters[col] = 0;
+    ss->max_iter[col] = max_iter;
+    ss->tol[col] = tol;
+    const bool now = ss->kind == kSessPpr && max_iter == 0;
+    if (now) ss->converged |= 1ull << col;
+    if (done_now) *done_now = now;
+    G1_SESSION_DISPATCH(session_insert_k, ss, col, source)
+}
+
+g1_status session_step(Session* ss, uint64_t* done, uint32_t* iters) {
+    *done = 0;
+    if (!(ss->active & ~ss->converged)) return G1_OK;
+    g1_status st;
+    if (ss->kind == kSessPpr) {
+        switch (ss->K) {
+            case 8: st = session_step_ppr<8>(ss, done); break;
+            case 16: st = session_step_ppr<16>(ss, done); break;
+            case 32: st = session_step_ppr<32>(ss, done); break;
+            default: st = session_step_ppr<64>(ss, done); break;
+        }
+    } else {
+        switch (ss->K) {
+            case 8: st = session_step_minplus<8>(ss, done); break;
+            case 16: st = session_step_minplus<16>(ss, done); break;
+            case 32: st = session_step_minplus<32>(ss, done); break;
+            default: st = session_step_minplus<64>(ss, done); break;
+        }
+    }
+    if (iters)
+        for (uint32_t j = 0; j < ss->K; ++j) iters[j] = ss->iters[j];
+    return st;
+}
+
+g1_status session_extract(Session* ss, uint32_t col, void* out_dev) {
+    G1_CHECK(col < ss->K && out_dev != nullptr, G1_ERR_INVALID_ARG, "session_extract: bad column or output");
+    G1_SESSION_DISPATCH(session_extract_k, ss, col, out_dev)
+}
+
+void session_release(Session* ss, uint32_t col) {
+    if (col >= ss->K || !((ss->active >> col) & 1ull)) return;
+    // A min-plus column left before converging still has source bits: empty it so it stops propagating.
+    if (ss->kind != kSessPpr && !((ss->converged >> col) & 1ull)) {
+        switch (ss->K) {
+            case 8: minplus_column_reset_kernel<8><<<fill_blocks(ss->g->n), 256, 0, ss->s>>>(ss->g->n, ss->D, ss->mask[ss->mcur], col, kInf); break;
+            case 16: minplus_column_reset_kernel<16><<<fill_blocks(ss->g->n), 256, 0, ss->s>>>(ss->g->n, ss->D, ss->mask[ss->mcur], col, kInf); break;
+            case 32: minplus_column_reset_kernel<32><<<fill_blocks(ss->g->n), 256, 0, ss->s>>>(ss->g->n, ss->D, ss->mask[ss->mcur], col, kInf); break;
+            default: minplus_column_reset_kernel<64><<<fill_blocks(ss->g->n), 256, 0, ss->s>>>(ss->g->n, ss->D, ss->mask[ss->mcur], col, kInf); break;
+        }
+    }
+    ss->active &= ~(1ull << col);
+    ss->converged &= ~(1ull << col);
+}
 
 void batch_release(g1_graph* g) {
     for (BatchWork*& b : g->batch) {

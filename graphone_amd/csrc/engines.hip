@@ -360,7 +360,7 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
     // alpha of the propagation: PageRank damping, or 1 - restart for the personalized variant.
     const float prop = personalized ? 1.0f - restart : alpha;
     const float r0 = n ? 1.0f / static_cast<float>(n) : 0.0f;
-    pr_init_kernel<<<grid_for(n, 256, persistent_cap()), 256, 0, s>>>(n, g->row_off, g->w, prop, g->iw, g->r[0],
+    pr_init_kernel<<<grid_for(n, 256, persistent_cap()), 256, 0, s>>>(n, g->osum, prop, g->iw, g->r[0],
                                                                       r0, g->xf, seed);
     G1_LAUNCHED();
     const float base_all = personalized ? 0.0f : static_cast<float>((1.0 - alpha) / static_cast<double>(n));
@@ -415,8 +415,8 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
             }
             prof_end(g, tok, s, spmv_bytes);
             tok = prof_begin(g, "pr_apply", s);
-            rank_apply_kernel<1><<<nb, 256, 0, s>>>(n, g->yf, base_all, g->seeds_dev, base_seed, g->iw, g->r[cur],
-                                                    g->r[cur ^ 1], g->xf, g->rpart, nb, 0ull);
+            rank_apply_kernel<<<nb, 256, 0, s>>>(n, g->yf, base_all, g->seeds_dev, base_seed, g->iw, g->r[cur],
+                                                 g->r[cur ^ 1], g->xf, g->rpart);
             G1_LAUNCHED();
             prof_end(g, tok, s, 20.0 * n);
             reduce_partials_kernel<0><<<dim3(1, 1), 256, 0, s>>>(g->rpart, nb, g->rpart + static_cast<size_t>(nb) * kBatchCols);

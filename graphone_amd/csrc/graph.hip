@@ -7,6 +7,7 @@
 
 #include "g1/push.hpp"
 #include "g1/spmv_merge.hpp"
+#include "g1/apply.hpp"
 #include "graph.hpp"
 #include "spmv_launch.hpp"
 
@@ -66,6 +67,13 @@ g1_status ensure_pull_work(g1_graph* g, bool pagerank) {
         G1_TRY(alloc_n(g, &g->iw, n));
         G1_TRY(alloc_n(g, &g->r[0], n));
         G1_TRY(alloc_n(g, &g->r[1], n));
+        G1_TRY(alloc_n(g, &g->osum, n));
+        if (g->n) {
+            const uint32_t blocks = std::min<uint32_t>((g->n + 3) / 4, 16384);
+            out_weight_sum_kernel<<<blocks, 256>>>(g->n, g->row_off, g->w, g->osum);
+            G1_LAUNCHED();
+            G1_HIP(hipDeviceSynchronize());
+        }
     }
     if (pagerank && g->t_w && !g->t_wf) {
         G1_TRY(alloc_n(g, &g->t_wf, g->m ? g->m : 1));

@@ -47,6 +47,7 @@ struct g1_graph {
     float* xf = nullptr;
     float* yf = nullptr;
     float* iw = nullptr;
+    float* osum = nullptr;  // exact out-weight sum (or out-degree) of every vertex, rounded to float
     float* r[2] = {nullptr, nullptr};
     double* partials = nullptr;  // per-block partial sums, then the folded scalar at [kPartials]
     double* rpart = nullptr;     // canonical L1 partials: kBatchCols columns x ceil(n/256) blocks (+ sums)

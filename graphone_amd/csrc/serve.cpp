@@ -1,0 +1,629 @@
+// Query server: graphs resident in HBM, a request loop per graph, HBM-budget admission and continuous
+// batching of SSSP / personalized-PageRank queries into column sessions (session.hpp).
+//
+// The reference has no serving layer (each app loads one graph and answers one query per process,
+// apps/sssp_adv.cu:74-124, apps/pagerank_blas.cu:127-175); the engines under it are the same ones the
+// single-source C-ABI calls use (g1_bfs, g1_sssp, g1_ppr) and the batched SpMM of batch.hip.
+//
+// Threads: callers submit, poll, wait, fetch and cancel from any thread (one server mutex guards the
+// query table and the budget); one worker thread per registered graph owns that graph's HIP stream and
+// sessions, and never holds the server mutex while it calls into HIP.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "graph.hpp"
+#include "internal.hpp"
+#include "session.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Query {
+    uint64_t id = 0;
+    g1_query q{};
+    int32_t state = G1_Q_QUEUED;
+    g1_status status = G1_OK;
+    int32_t path = G1_PATH_SINGLE;
+    uint32_t iterations = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    uint64_t bytes = 0;        // result bytes (n * 4)
+    void* result = nullptr;    // device buffer, owned by the server until fetched
+    bool cancel = false;       // cancel requested while running
+    bool reserved = false;     // its bytes are counted in hbm_used
+};
+
+struct GraphSlot {
+    uint32_t id = 0;
+    g1_graph* g = nullptr;
+    uint32_t n = 0;
+    std::deque<Query*> queue;                  // submitted, not admitted (FIFO)
+    std::thread worker;
+    // session bookkeeping shared with admission (guarded by the server mutex)
+    bool session_open[3] = {false, false, false};  // [G1_QUERY_*]
+    uint64_t session_reserved[3] = {0, 0, 0};
+    uint32_t waiting_kind[3] = {0, 0, 0};        // queued queries per kind
+    std::deque<Query*> admitted;                 // admitted, not yet picked up by the worker
+};
+
+}  // namespace
+
+struct g1_server {
+    g1_server_config cfg{};
+    Clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_work;  // new submissions / cancellations / budget freed / stop
+    std::condition_variable cv_done;  // a query reached a final state
+    bool stop = false;
+    std::vector<std::unique_ptr<GraphSlot>> graphs;
+    std::unordered_map<uint64_t, std::unique_ptr<Query>> queries;
+    uint64_t next_id = 1;
+    uint64_t hbm_used = 0, hbm_peak = 0;  // reserved: graphs, sessions, live results
+    // Result buffers returned by fetch, kept for reuse by size (hipFree may wait for the whole device).
+    std::unordered_map<uint64_t, std::vector<void*>> pool;
+    uint64_t pooled = 0;
+    g1_server_stats st{};
+
+    double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+    void reserve(uint64_t b) {
+        hbm_used += b;
+        hbm_peak = std::max(hbm_peak, hbm_used);
+    }
+    void unreserve(uint64_t b) { hbm_used -= std::min(hbm_used, b); }
+};
+
+namespace {
+
+// A device buffer of `bytes` for a result (worker thread): reused from the pool when one is free.
+// Pooled buffers beyond the budget are freed first, so reserved + pooled memory stays within it.
+void* result_buffer(g1_server* sv, uint64_t bytes) {
+    std::vector<void*> drop;
+    void* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        auto it = sv->pool.find(bytes);
+        if (it != sv->pool.end() && !it->second.empty()) {
+            p = it->second.back();
+            it->second.pop_back();
+            sv->pooled -= bytes;
+            return p;
+        }
+        for (auto& kv : sv->pool) {
+            while (!kv.second.empty() && sv->hbm_used + sv->pooled > sv->cfg.hbm_budget_bytes) {
+                drop.push_back(kv.second.back());
+                kv.second.pop_back();
+                sv->pooled -= kv.first;
+            }
+        }
+    }
+    for (void* d : drop) (void)hipFree(d);
+    if (hipMalloc(&p, std::max<uint64_t>(bytes, 4)) != hipSuccess) return nullptr;
+    return p;
+}
+
+// Lock held: returns a result buffer to the pool.
+void recycle(g1_server* sv, void* p, uint64_t bytes) {
+    if (!p) return;
+    sv->pool[bytes].push_back(p);
+    sv->pooled += bytes;
+}
+
+bool final_state(int32_t s) { return s == G1_Q_DONE || s == G1_Q_FAILED || s == G1_Q_CANCELLED || s == G1_Q_FETCHED; }
+
+int session_kind_of(int32_t qkind) { return qkind == G1_QUERY_PPR ? g1::kSessPpr : g1::kSessMinPlusWeighted; }
+
+// ---- worker side ---------------------------------------------------------------------------------
+
+struct Worker {
+    g1_server* sv;
+    GraphSlot* slot;
+    hipStream_t stream = nullptr;
+    g1::Session* sess[3] = {nullptr, nullptr, nullptr};  // [G1_QUERY_SSSP], [G1_QUERY_PPR] used
+    Query* col_query[3][g1::kBatchCols] = {};
+    std::deque<Query*> single;      // admitted for a single-source engine
+    std::deque<Query*> pending[3];  // admitted for a session, waiting for a free column
+
+    // Marks q final under the server lock and releases what it holds when it did not complete.
+    void finish(Query* q, g1_status status, uint32_t iterations, bool cancel = false) {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        if (cancel) q->cancel = true;
+        q->iterations = iterations;
+        q->t_done = sv->now_ms();
+        if (q->cancel) {
+            q->state = G1_Q_CANCELLED;
+            ++sv->st.cancelled;
+        } else if (status != G1_OK) {
+            q->state = G1_Q_FAILED;
+            q->status = status;
+            ++sv->st.failed;
+        } else {
+            q->state = G1_Q_DONE;
+            ++sv->st.completed;
+            if (q->path == G1_PATH_BATCHED) ++sv->st.batched; else ++sv->st.single;
+        }
+        if (sv->st.running) --sv->st.running;
+        if (q->state != G1_Q_DONE) {
+            recycle(sv, q->result, q->bytes);
+            q->result = nullptr;
+            if (q->reserved) sv->unreserve(q->bytes);
+            q->reserved = false;
+            sv->cv_work.notify_all();
+        }
+        sv->cv_done.notify_all();
+    }
+
+    g1_status run_single(Query* q) {
+        g1_graph* g = slot->g;
+        const g1_stream_t s = reinterpret_cast<g1_stream_t>(stream);
+        g1_run_stats rs;
+        memset(&rs, 0, sizeof(rs));
+        g1_status st;
+        switch (q->q.kind) {
+            case G1_QUERY_BFS:
+                st = g1_bfs(g, q->q.source, static_cast<uint32_t*>(q->result), G1_ENGINE_DIROPT, &rs, s);
+                break;
+            case G1_QUERY_SSSP:
+                st = g1_sssp(g, q->q.source, static_cast<uint32_t*>(q->result), G1_ENGINE_ADVANCE, &rs, s);
+                break;
+            default:
+                st = g1_ppr(g, q->q.source, sv->cfg.ppr_restart, q->q.max_iter, q->q.tol, static_cast<float*>(q->result),
+                            &rs, s);
+                break;
+        }
+        if (st == G1_OK && hipStreamSynchronize(stream) != hipSuccess) st = G1_ERR_HIP;
+        finish(q, st, rs.iterations);
+        return st;
+    }
+
+    // One session iteration of kind k: fill free columns, step, hand out the finished columns.
+    void session_round(int k) {
+        g1::Session* ss = sess[k];
+        const uint32_t width = g1::session_width(ss);
+        uint64_t active = g1::session_active(ss);
+        uint64_t done_now = 0;
+        // cancelled columns leave at once
+        for (uint32_t j = 0; j < width; ++j) {
+            Query* q = col_query[k][j];
+            if (!q) continue;
+            bool c;
+            {
+                std::lock_guard<std::mutex> lock(sv->mu);
+                c = q->cancel;
+            }
+            if (c) {
+                g1::session_release(ss, j);
+                col_query[k][j] = nullptr;
+                finish(q, G1_OK, 0);
+            }
+        }
+        active = g1::session_active(ss);
+        for (uint32_t j = 0; j < width && !pending[k].empty(); ++j) {
+            if ((active >> j) & 1ull) continue;
+            Query* q = pending[k].front();
+            pending[k].pop_front();
+            bool now = false;
+            const g1_status st = g1::session_insert(ss, j, q->q.source, q->q.max_iter, q->q.tol, &now);
+            if (st != G1_OK) {
+                finish(q, st, 0);
+                continue;
+            }
+            col_query[k][j] = q;
+            active |= 1ull << j;
+            if (now) done_now |= 1ull << j;
+        }
+        uint64_t done = 0;
+        uint32_t iters[g1::kBatchCols] = {};
+        g1_status st = g1::session_step(ss, &done, iters);
+        {
+            std::lock_guard<std::mutex> lock(sv->mu);
+            ++sv->st.session_steps;
+        }
+        done |= done_now;
+        if (st != G1_OK) done = active;  // a failed step fails every column in flight
+        for (uint32_t j = 0; j < width; ++j) {
+            if (!((done >> j) & 1ull) || !col_query[k][j]) continue;
+            Query* q = col_query[k][j];
+            g1_status qs = st;
+            if (qs == G1_OK) qs = g1::session_extract(ss, j, q->result);
+            if (qs == G1_OK && hipStreamSynchronize(stream) != hipSuccess) qs = G1_ERR_HIP;
+            g1::session_release(ss, j);
+            col_query[k][j] = nullptr;
+            finish(q, qs, iters[j]);
+        }
+    }
+
+    bool busy() const {
+        if (!single.empty()) return true;
+        for (int k = 0; k < 3; ++k) {
+            if (!pending[k].empty()) return true;
+            if (sess[k] && g1::session_active(sess[k])) return true;
+        }
+        return false;
+    }
+
+    void close_idle_sessions() {
+        for (int k = 1; k < 3; ++k) {
+            if (!sess[k] || g1::session_active(sess[k]) || !pending[k].empty()) continue;
+            g1::session_destroy(sess[k]);
+            sess[k] = nullptr;
+            std::lock_guard<std::mutex> lock(sv->mu);
+            slot->session_open[k] = false;
+            sv->unreserve(slot->session_reserved[k]);
+            slot->session_reserved[k] = 0;
+            sv->cv_work.notify_all();
+        }
+    }
+
+    void loop() {
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) stream = nullptr;
+        for (;;) {
+            std::vector<Query*> fresh;
+            {
+                std::unique_lock<std::mutex> lock(sv->mu);
+                if (!busy())
+                    sv->cv_work.wait_for(lock, std::chrono::milliseconds(20),
+                                         [&] { return sv->stop || !slot->admitted.empty(); });
+                if (sv->stop) break;
+                while (!slot->admitted.empty()) {
+                    fresh.push_back(slot->admitted.front());
+                    slot->admitted.pop_front();
+                }
+            }
+            // result buffers (their bytes were reserved at admission)
+            for (Query* q : fresh) {
+                void* p = result_buffer(sv, q->bytes);
+                if (!p) {
+                    finish(q, G1_ERR_OUT_OF_MEMORY, 0);
+                    continue;
+                }
+                q->result = p;
+                if (q->path == G1_PATH_BATCHED) pending[q->q.kind].push_back(q);
+                else single.push_back(q);
+            }
+            // sessions: open on demand (admission reserved their bytes), then one iteration each
+            for (int k = 1; k < 3; ++k) {
+                if (!sess[k] && !pending[k].empty()) {
+                    g1::Session* ss = nullptr;
+                    const g1_status st = g1::session_create(slot->g, session_kind_of(k), sv->cfg.batch_k,
+                                                            sv->cfg.ppr_restart, stream, &ss);
+                    if (st != G1_OK) {
+                        while (!pending[k].empty()) {
+                            finish(pending[k].front(), st, 0);
+                            pending[k].pop_front();
+                        }
+                        std::lock_guard<std::mutex> lock(sv->mu);
+                        slot->session_open[k] = false;
+                        sv->unreserve(slot->session_reserved[k]);
+                        slot->session_reserved[k] = 0;
+                        continue;
+                    }
+                    sess[k] = ss;
+                }
+                if (sess[k]) session_round(k);
+            }
+            // one single-source query between session iterations
+            if (!single.empty()) {
+                Query* q = single.front();
+                single.pop_front();
+                bool c;
+                {
+                    std::lock_guard<std::mutex> lock(sv->mu);
+                    c = q->cancel;
+                }
+                if (c) finish(q, G1_OK, 0);
+                else run_single(q);
+            }
+            close_idle_sessions();
+        }
+        // shutdown: everything still held is cancelled
+        for (int k = 1; k < 3; ++k) {
+            for (uint32_t j = 0; j < g1::kBatchCols; ++j)
+                if (col_query[k][j]) finish(col_query[k][j], G1_OK, 0, true);
+            for (Query* q : pending[k]) finish(q, G1_OK, 0, true);
+            if (sess[k]) g1::session_destroy(sess[k]);
+        }
+        for (Query* q : single) finish(q, G1_OK, 0, true);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// Admission (server lock held): moves queries from the head of the graph's queue to its worker while
+// their memory fits. A query that does not fit stops its graph's queue (FIFO) until memory is freed.
+void admit(g1_server* sv, GraphSlot* gs) {
+    while (!gs->queue.empty()) {
+        Query* q = gs->queue.front();
+        if (q->cancel) {
+            gs->queue.pop_front();
+            if (gs->waiting_kind[q->q.kind]) --gs->waiting_kind[q->q.kind];
+            if (sv->st.queued) --sv->st.queued;
+            q->state = G1_Q_CANCELLED;
+            q->t_done = sv->now_ms();
+            ++sv->st.cancelled;
+            sv->cv_done.notify_all();
+            continue;
+        }
+        const int k = q->q.kind;
+        bool batched = false;
+        uint64_t need = q->bytes;
+        if (k != G1_QUERY_BFS) {
+            const uint32_t threshold = k == G1_QUERY_PPR ? sv->cfg.batch_min_ppr : sv->cfg.batch_min_sssp;
+            if (gs->session_open[k]) {
+                batched = true;
+            } else if (gs->waiting_kind[k] >= threshold) {
+                batched = true;
+                need += g1::session_bytes(gs->g, session_kind_of(k), sv->cfg.batch_k);
+            }
+        }
+        if (sv->hbm_used + need > sv->cfg.hbm_budget_bytes) {
+            if (!q->reserved && q->t_admit == 0) {
+                q->t_admit = -1;  // counted once
+                ++sv->st.admission_waits;
+            }
+            return;
+        }
+        gs->queue.pop_front();
+        if (gs->waiting_kind[k]) --gs->waiting_kind[k];
+        if (batched && !gs->session_open[k]) {
+            gs->session_open[k] = true;
+            gs->session_reserved[k] = need - q->bytes;
+        }
+        sv->reserve(need);
+        q->reserved = true;
+        q->path = batched ? G1_PATH_BATCHED : G1_PATH_SINGLE;
+        q->state = G1_Q_RUNNING;
+        q->t_admit = sv->now_ms();
+        if (sv->st.queued) --sv->st.queued;
+        ++sv->st.running;
+        gs->admitted.push_back(q);
+    }
+}
+
+void admit_all(g1_server* sv) {
+    for (auto& gs : sv->graphs) admit(sv, gs.get());
+    sv->cv_work.notify_all();
+}
+
+Query* find(g1_server* sv, uint64_t id) {
+    auto it = sv->queries.find(id);
+    return it == sv->queries.end() ? nullptr : it->second.get();
+}
+
+void fill_status(const Query* q, g1_query_status* st) {
+    memset(st, 0, sizeof(*st));
+    st->state = q->state;
+    st->status = q->status;
+    st->path = q->path;
+    st->iterations = q->iterations;
+    st->t_submit_ms = q->t_submit;
+    st->t_admit_ms = q->t_admit > 0 ? q->t_admit : 0.0;
+    st->t_done_ms = q->t_done;
+    st->result_bytes = q->bytes;
+}
+
+}  // namespace
+
+extern "C" {
+
+g1_status g1_server_create(const g1_server_config* config, g1_server** out) {
+    G1_CHECK(out != nullptr, G1_ERR_INVALID_ARG, "g1_server_create: null output");
+    g1_server_config c;
+    memset(&c, 0, sizeof(c));
+    if (config) c = *config;
+    if (c.batch_k == 0) c.batch_k = 64;
+    G1_CHECK(c.batch_k == 8 || c.batch_k == 16 || c.batch_k == 32 || c.batch_k == 64, G1_ERR_INVALID_ARG,
+             "g1_server_create: batch_k must be 8, 16, 32 or 64 (got %u)", c.batch_k);
+    if (c.batch_min_sssp == 0) c.batch_min_sssp = 8;
+    if (c.batch_min_ppr == 0) c.batch_min_ppr = 2;
+    if (c.ppr_restart == 0.0f) c.ppr_restart = 0.15f;
+    G1_CHECK(c.ppr_restart > 0.0f && c.ppr_restart <= 1.0f, G1_ERR_INVALID_ARG,
+             "g1_server_create: ppr_restart must be in (0, 1]");
+    if (c.hbm_budget_bytes == 0) {
+        size_t free_b = 0, total_b = 0;
+        G1_HIP(hipMemGetInfo(&free_b, &total_b));
+        c.hbm_budget_bytes = static_cast<uint64_t>(static_cast<double>(free_b) * 0.9);
+    }
+    g1_server* sv = new g1_server();
+    sv->cfg = c;
+    sv->t0 = Clock::now();
+    sv->st.hbm_budget = c.hbm_budget_bytes;
+    *out = sv;
+    return G1_OK;
+}
+
+g1_status g1_server_destroy(g1_server* sv) {
+    if (!sv) return G1_OK;
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        sv->stop = true;
+        for (auto& gs : sv->graphs) {
+            for (Query* q : gs->admitted) q->cancel = true;
+        }
+        sv->cv_work.notify_all();
+    }
+    for (auto& gs : sv->graphs)
+        if (gs->worker.joinable()) gs->worker.join();
+    for (auto& kv : sv->queries)
+        if (kv.second->result) (void)hipFree(kv.second->result);
+    for (auto& kv : sv->pool)
+        for (void* p : kv.second) (void)hipFree(p);
+    delete sv;
+    return G1_OK;
+}
+
+g1_status g1_server_add_graph(g1_server* sv, g1_graph* graph, uint32_t* graph_id) {
+    G1_CHECK(sv != nullptr && graph != nullptr, G1_ERR_INVALID_ARG, "g1_server_add_graph: null argument");
+    uint32_t n = 0, m = 0;
+    uint64_t bytes = 0;
+    G1_TRY(g1_graph_info(graph, &n, &m, &bytes));
+    G1_CHECK(graph->has_t, G1_ERR_INVALID_ARG, "g1_server_add_graph: the graph needs its transpose");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    G1_CHECK(sv->hbm_used + bytes <= sv->cfg.hbm_budget_bytes, G1_ERR_BUDGET,
+             "g1_server_add_graph: graph (%llu bytes) exceeds the HBM budget", static_cast<unsigned long long>(bytes));
+    auto gs = std::make_unique<GraphSlot>();
+    gs->id = static_cast<uint32_t>(sv->graphs.size());
+    gs->g = graph;
+    gs->n = n;
+    sv->reserve(bytes);
+    GraphSlot* raw = gs.get();
+    sv->graphs.push_back(std::move(gs));
+    raw->worker = std::thread([sv, raw] {
+        Worker w{sv, raw};
+        w.loop();
+    });
+    if (graph_id) *graph_id = raw->id;
+    return G1_OK;
+}
+
+g1_status g1_server_submit(g1_server* sv, const g1_query* query, uint64_t* query_id) {
+    G1_CHECK(sv != nullptr && query != nullptr && query_id != nullptr, G1_ERR_INVALID_ARG,
+             "g1_server_submit: null argument");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    G1_CHECK(!sv->stop, G1_ERR_CANCELLED, "g1_server_submit: server is shutting down");
+    G1_CHECK(query->graph_id < sv->graphs.size(), G1_ERR_NOT_FOUND, "g1_server_submit: unknown graph %u",
+             query->graph_id);
+    G1_CHECK(query->kind == G1_QUERY_BFS || query->kind == G1_QUERY_SSSP || query->kind == G1_QUERY_PPR,
+             G1_ERR_INVALID_ARG, "g1_server_submit: unknown query kind %d", query->kind);
+    GraphSlot* gs = sv->graphs[query->graph_id].get();
+    G1_CHECK(query->source < gs->n, G1_ERR_INVALID_ARG, "g1_server_submit: source %u out of range (n = %u)",
+             query->source, gs->n);
+    G1_CHECK(query->kind != G1_QUERY_SSSP || gs->g->w != nullptr, G1_ERR_INVALID_ARG,
+             "g1_server_submit: SSSP on a graph without weights");
+    const uint64_t bytes = static_cast<uint64_t>(gs->n) * 4;
+    G1_CHECK(sv->graphs.size() && bytes <= sv->cfg.hbm_budget_bytes, G1_ERR_BUDGET,
+             "g1_server_submit: the result alone exceeds the HBM budget");
+    auto q = std::make_unique<Query>();
+    q->id = sv->next_id++;
+    q->q = *query;
+    q->bytes = bytes;
+    q->t_submit = sv->now_ms();
+    Query* raw = q.get();
+    sv->queries.emplace(raw->id, std::move(q));
+    gs->queue.push_back(raw);
+    ++gs->waiting_kind[query->kind];
+    ++sv->st.submitted;
+    ++sv->st.queued;
+    *query_id = raw->id;
+    admit(sv, gs);
+    sv->cv_work.notify_all();
+    return G1_OK;
+}
+
+g1_status g1_server_poll(g1_server* sv, uint64_t query_id, g1_query_status* status) {
+    G1_CHECK(sv != nullptr && status != nullptr, G1_ERR_INVALID_ARG, "g1_server_poll: null argument");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    Query* q = find(sv, query_id);
+    G1_CHECK(q != nullptr, G1_ERR_NOT_FOUND, "g1_server_poll: unknown query %llu",
+             static_cast<unsigned long long>(query_id));
+    fill_status(q, status);
+    return G1_OK;
+}
+
+g1_status g1_server_wait(g1_server* sv, uint64_t query_id, double timeout_ms, g1_query_status* status) {
+    G1_CHECK(sv != nullptr, G1_ERR_INVALID_ARG, "g1_server_wait: null server");
+    std::unique_lock<std::mutex> lock(sv->mu);
+    Query* q = find(sv, query_id);
+    G1_CHECK(q != nullptr, G1_ERR_NOT_FOUND, "g1_server_wait: unknown query %llu",
+             static_cast<unsigned long long>(query_id));
+    const auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(
+                                             std::chrono::duration<double, std::milli>(std::max(timeout_ms, 0.0)));
+    while (!final_state(q->state)) {
+        if (sv->cv_done.wait_until(lock, deadline) == std::cv_status::timeout && !final_state(q->state)) {
+            if (status) fill_status(q, status);
+            return g1::set_error(G1_ERR_BUSY, "g1_server_wait: query %llu not finished after %.1f ms",
+                                 static_cast<unsigned long long>(query_id), timeout_ms);
+        }
+    }
+    if (status) fill_status(q, status);
+    return G1_OK;
+}
+
+g1_status g1_server_fetch(g1_server* sv, uint64_t query_id, void* dst, uint64_t dst_bytes, int32_t dst_on_device) {
+    G1_CHECK(sv != nullptr && dst != nullptr, G1_ERR_INVALID_ARG, "g1_server_fetch: null argument");
+    void* src = nullptr;
+    uint64_t bytes = 0;
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        Query* q = find(sv, query_id);
+        G1_CHECK(q != nullptr, G1_ERR_NOT_FOUND, "g1_server_fetch: unknown query %llu",
+                 static_cast<unsigned long long>(query_id));
+        G1_CHECK(q->state == G1_Q_DONE, q->state == G1_Q_FETCHED ? G1_ERR_INVALID_ARG : G1_ERR_BUSY,
+                 "g1_server_fetch: query %llu is not done (state %d)", static_cast<unsigned long long>(query_id),
+                 q->state);
+        G1_CHECK(dst_bytes >= q->bytes, G1_ERR_INVALID_ARG, "g1_server_fetch: destination holds %llu of %llu bytes",
+                 static_cast<unsigned long long>(dst_bytes), static_cast<unsigned long long>(q->bytes));
+        src = q->result;
+        bytes = q->bytes;
+        q->result = nullptr;
+        q->state = G1_Q_FETCHED;
+    }
+    const hipError_t e = hipMemcpy(dst, src, bytes, dst_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost);
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        recycle(sv, src, bytes);
+        Query* q = find(sv, query_id);
+        if (q->reserved) sv->unreserve(q->bytes);
+        q->reserved = false;
+        admit_all(sv);
+    }
+    G1_CHECK(e == hipSuccess, G1_ERR_HIP, "g1_server_fetch: copy failed: %s", hipGetErrorString(e));
+    return G1_OK;
+}
+
+g1_status g1_server_cancel(g1_server* sv, uint64_t query_id) {
+    G1_CHECK(sv != nullptr, G1_ERR_INVALID_ARG, "g1_server_cancel: null server");
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        Query* q = find(sv, query_id);
+        G1_CHECK(q != nullptr, G1_ERR_NOT_FOUND, "g1_server_cancel: unknown query %llu",
+                 static_cast<unsigned long long>(query_id));
+        if (q->state == G1_Q_QUEUED) {
+            q->cancel = true;
+            GraphSlot* gs = sv->graphs[q->q.graph_id].get();
+            auto it = std::find(gs->queue.begin(), gs->queue.end(), q);
+            if (it != gs->queue.end()) {
+                gs->queue.erase(it);
+                if (gs->waiting_kind[q->q.kind]) --gs->waiting_kind[q->q.kind];
+                if (sv->st.queued) --sv->st.queued;
+            }
+            q->state = G1_Q_CANCELLED;
+            q->t_done = sv->now_ms();
+            ++sv->st.cancelled;
+            sv->cv_done.notify_all();
+            admit_all(sv);
+        } else if (q->state == G1_Q_RUNNING) {
+            q->cancel = true;  // the worker drops it at its next step
+        } else if (q->state == G1_Q_DONE) {
+            recycle(sv, q->result, q->bytes);  // a finished, unfetched result is discarded
+            q->result = nullptr;
+            q->state = G1_Q_CANCELLED;
+            if (q->reserved) sv->unreserve(q->bytes);
+            q->reserved = false;
+            ++sv->st.cancelled;
+            admit_all(sv);
+        }
+    }
+    return G1_OK;
+}
+
+g1_status g1_server_get_stats(g1_server* sv, g1_server_stats* stats) {
+    G1_CHECK(sv != nullptr && stats != nullptr, G1_ERR_INVALID_ARG, "g1_server_get_stats: null argument");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    *stats = sv->st;
+    stats->hbm_budget = sv->cfg.hbm_budget_bytes;
+    stats->hbm_used = sv->hbm_used;
+    stats->hbm_peak = sv->hbm_peak;
+    return G1_OK;
+}
+
+}  // extern "C"

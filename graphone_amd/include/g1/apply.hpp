@@ -56,20 +56,35 @@ static __global__ void __launch_bounds__(256) minplus_seed_kernel(uint32_t n, ui
 
 // iw[u] = alpha / sum_out w(u) (0 when u has no out-edge), summed in float left to right as
 // apps/pagerank_blas.cu:41-53 does; r[u] = r0; x[u] = r[u] * iw[u].
-static __global__ void __launch_bounds__(256) pr_init_kernel(uint32_t n, const uint32_t* __restrict__ row_off,
-                                                      const uint32_t* __restrict__ wts, float alpha,
+// Exact out-weight sum of every vertex (apps/pagerank_blas.cu:41-53 sums the out-edge weights; here the
+// u32 weights are summed exactly in 64 bits and rounded once to float). One wave per row.
+static __global__ void __launch_bounds__(256) out_weight_sum_kernel(uint32_t n, const uint32_t* __restrict__ row_off,
+                                                             const uint32_t* __restrict__ wts,
+                                                             float* __restrict__ osum) {
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += waves) {
+        const uint32_t b = row_off[u], e = row_off[u + 1];
+        unsigned long long s = 0;
+        if (wts) {
+            for (uint32_t k = b + lane_id(); k < e; k += kWave) s += wts[k];
+#pragma unroll
+            for (int off = kWave / 2; off >= 1; off >>= 1) s += __shfl_down(s, off);
+        } else {
+            s = e - b;
+        }
+        if (lane_id() == 0) osum[u] = static_cast<float>(s);
+    }
+}
+
+// iw[u] = alpha / osum[u] (0 for a vertex without out-weight), r0 and x0 = r0 * iw; with a seed vertex,
+// r0 = e_seed (personalized PageRank).
+static __global__ void __launch_bounds__(256) pr_init_kernel(uint32_t n, const float* __restrict__ osum, float alpha,
                                                       float* __restrict__ iw, float* __restrict__ r, float r0,
                                                       float* __restrict__ x, uint32_t seed_vertex) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += stride) {
-        float s = 0.0f;
-        const uint32_t b = row_off[u], e = row_off[u + 1];
-        if (wts) {
-            for (uint32_t k = b; k < e; ++k) s += static_cast<float>(wts[k]);
-        } else {
-            s = static_cast<float>(e - b);
-        }
-        const float inv = (s != 0.0f) ? alpha / s : 0.0f;
+        const float sw = osum[u];
+        const float inv = (sw != 0.0f) ? alpha / sw : 0.0f;
         iw[u] = inv;
         const float rv = (seed_vertex == kInf) ? r0 : (u == seed_vertex ? 1.0f : 0.0f);
         r[u] = rv;
@@ -117,45 +132,82 @@ __global__ void __launch_bounds__(256) pr_apply_kernel(uint32_t n, const float* 
     }
 }
 
-// PageRank / personalized PageRank apply over K columns (K = 1 for a single run) with the canonical
-// FP64 L1 reduction: block b covers vertices [256b, 256b + 256), thread t vertex 256b + t, and each
-// column's 256 |r' - r| values are folded by the same fixed tree (wave shuffles, then the four waves in
-// order) into partials[j * nblocks + b]. A batched column therefore gets bit for bit the norm its
-// single-source run gets. Column j with bit j of `frozen` set keeps its rank and x (it has converged).
-//   r'[v][j] = (v == seeds[j] ? base_seed : base_all) + y[v][j]
-//   x[v][j]  = r'[v][j] * iw[v]
-template <int K>
-__global__ void __launch_bounds__(256) rank_apply_kernel(uint32_t n, const float* __restrict__ Y, float base_all,
+// Canonical FP64 L1 norm of r' - r, shared by the single-source and the batched PageRank so that a
+// batched column stops exactly when its single-source run stops: for every block b of 256 vertices
+// [256b, 256b + 256) and every column, the four quarters of 64 vertices are each summed sequentially
+// in vertex order (starting from 0.0), and partials[j * nblocks + b] = ((q0 + q1) + q2) + q3.
+//   r'[v] = (v == seed ? base_seed : base_all) + y[v]
+//   x[v]  = r'[v] * iw[v]
+// Single column: thread t owns vertex 256b + t; lane 0 of each wave folds its quarter from LDS.
+static __global__ void __launch_bounds__(256) rank_apply_kernel(uint32_t n, const float* __restrict__ y, float base_all,
                                                          const uint32_t* __restrict__ seeds, float base_seed,
-                                                         const float* __restrict__ iw, const float* __restrict__ R_old,
-                                                         float* __restrict__ R_new, float* __restrict__ X,
-                                                         double* __restrict__ partials, uint32_t nblocks,
-                                                         unsigned long long frozen) {
-    __shared__ double s_part[256 / kWave];
+                                                         const float* __restrict__ iw,
+                                                         const float* __restrict__ r_old, float* __restrict__ r_new,
+                                                         float* __restrict__ x, double* __restrict__ partials) {
+    __shared__ double s_d[256];
+    __shared__ double s_q[256 / kWave];
     const uint32_t v = blockIdx.x * 256 + threadIdx.x;
-    const float iwv = v < n ? iw[v] : 0.0f;
-    for (int j = 0; j < K; ++j) {
-        double d = 0.0;
-        if (v < n) {
+    double d = 0.0;
+    if (v < n) {
+        const float b = (v == seeds[0]) ? base_seed : base_all;
+        const float rn = b + y[v];
+        r_new[v] = rn;
+        x[v] = rn * iw[v];
+        d = fabs(static_cast<double>(rn) - static_cast<double>(r_old[v]));
+    }
+    s_d[threadIdx.x] = d;
+    __syncthreads();
+    if (lane_id() == 0) {
+        const uint32_t w0 = (threadIdx.x / kWave) * kWave;
+        double q = 0.0;
+        for (int k = 0; k < kWave; ++k) q += s_d[w0 + k];
+        s_q[threadIdx.x / kWave] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = ((s_q[0] + s_q[1]) + s_q[2]) + s_q[3];
+}
+
+// K columns, row-major n x K: a group of K lanes (lane j = column j) walks one block of 256 vertices in
+// vertex order with coalesced K-wide accesses; a workgroup covers 256 / K blocks. Column j with bit j of
+// `frozen` set keeps its rank (r' = r) and contributes nothing (it has converged).
+template <int K>
+__global__ void __launch_bounds__(256) rank_apply_batch_kernel(uint32_t n, const float* __restrict__ Y,
+                                                               float base_all, const uint32_t* __restrict__ seeds,
+                                                               float base_seed, const float* __restrict__ iw,
+                                                               const float* __restrict__ R_old,
+                                                               float* __restrict__ R_new, float* __restrict__ X,
+                                                               double* __restrict__ partials, uint32_t nblocks,
+                                                               unsigned long long frozen) {
+    constexpr uint32_t G = 256 / K;
+    const uint32_t j = threadIdx.x % K;
+    const uint32_t blk = blockIdx.x * G + threadIdx.x / K;
+    if (blk >= nblocks) return;
+    const bool fz = (frozen >> j) & 1ull;
+    const uint32_t seed = seeds[j];
+    double q[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        double acc = 0.0;
+        const uint32_t v0 = blk * 256 + w * kWave;
+#pragma unroll 8
+        for (uint32_t k = 0; k < static_cast<uint32_t>(kWave); ++k) {
+            const uint32_t v = v0 + k;
+            if (v >= n) break;
             const size_t at = static_cast<size_t>(v) * K + j;
-            if ((frozen >> j) & 1ull) {
-                R_new[at] = R_old[at];
+            const float ro = R_old[at];
+            if (fz) {
+                R_new[at] = ro;
             } else {
-                const float b = (v == seeds[j]) ? base_seed : base_all;
+                const float b = (v == seed) ? base_seed : base_all;
                 const float rn = b + Y[at];
                 R_new[at] = rn;
-                X[at] = rn * iwv;
-                d = fabs(static_cast<double>(rn) - static_cast<double>(R_old[at]));
+                X[at] = rn * iw[v];
+                acc += fabs(static_cast<double>(rn) - static_cast<double>(ro));
             }
         }
-#pragma unroll
-        for (int off = kWave / 2; off >= 1; off >>= 1) d += __shfl_down(d, off);
-        if (lane_id() == 0) s_part[threadIdx.x / kWave] = d;
-        __syncthreads();
-        if (threadIdx.x == 0)
-            partials[static_cast<size_t>(j) * nblocks + blockIdx.x] = ((s_part[0] + s_part[1]) + s_part[2]) + s_part[3];
-        __syncthreads();
+        q[w] = acc;
     }
+    partials[static_cast<size_t>(j) * nblocks + blk] = ((q[0] + q[1]) + q[2]) + q[3];
 }
 
 // Folds per-block partials in block order into out[0] (sum or max); one block.

@@ -3,11 +3,16 @@
 // Same merge-path tiles as the SpMV of g1/spmv_merge.hpp (same coordinates, kMergeThreads "virtual
 // threads" of kMergeItems merge steps per tile); here each virtual thread is executed by a group of K
 // lanes, lane j owning column j, so the K lanes walk the same steps in lockstep and load the K values of
-// a source row with one coalesced access. A workgroup runs its tile's virtual threads in rounds of
-// 256/K; after each round the lanes of column j (threads 0..K-1) fold, left to right, the round's heads
-// and tails into the open row exactly as the SpMV's closing threads do. The fold order of every column
-// is therefore the SpMV's fold order, bit for bit: a batched plus-times column equals the single-source
-// SpMV of that column.
+// a source row with one coalesced access. A workgroup runs its tile's virtual threads in K rounds of
+// G = 256/K. Every value a round needs from global memory (the x values of its nonzeros and, for the
+// min-plus apply, the old values of the rows it closes) is loaded one round ahead into registers, with
+// a fixed count of loads per round, so that the loads of round r+1 are in flight while round r walks.
+//
+// Fold order: a row's value is the fold, left to right, of the virtual threads' partial folds, exactly
+// as in spmv_merge_tile_kernel: a closing virtual thread merges the tails since the last closing
+// virtual thread (across rounds through the open carry), then its head; the tile's open row leaves as
+// the tile carry. A batched plus-times column therefore equals the single-source SpMV of that column,
+// bit for bit.
 //
 // Rows are finished by a policy (row_done) called together by the K lanes of a group, with the row's
 // K column values; the policy fuses the engine's apply step. Rows that cross tiles are finished by the
@@ -51,18 +56,24 @@ __global__ void __launch_bounds__(256)
 spmm_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row_offsets,
                  const uint32_t* __restrict__ col, const typename S::value_t* __restrict__ vals,
                  const MergeCoord* __restrict__ coords, SpmmTileRec<S, K>* __restrict__ recs, P pol,
-                 uint32_t* __restrict__ tile_counts) {
+                 unsigned long long* __restrict__ changed) {
     using acc_t = typename S::acc_t;
     using value_t = typename S::value_t;
-    constexpr int G = 256 / K;  // virtual threads per round
+    using x_t = typename S::x_t;
+    using aux_t = typename P::aux_t;
+    constexpr int G = 256 / K;                  // virtual threads per round
+    constexpr int kRounds = kMergeThreads / G;  // = K
+    constexpr int kWavesG = kWave / K;          // groups per wave
     __shared__ uint32_t s_rowend[kMergeTile + 1];
     __shared__ uint32_t s_col[kMergeTile];
     __shared__ value_t s_w[kMergeTile];
-    __shared__ acc_t s_head[G][K];
-    __shared__ acc_t s_tail[G][K];
-    __shared__ uint32_t s_hasend[G];
-    __shared__ uint32_t s_first[G];
-    __shared__ uint32_t s_red[4];
+    __shared__ aux_t s_aux[P::kHasAux ? kMergeTile : 1];
+    __shared__ uint32_t s_vi[kMergeThreads + 1];  // first row (tile-relative) of each virtual thread
+    __shared__ uint32_t s_vj[kMergeThreads + 1];  // first nonzero (tile-relative) of each virtual thread
+    __shared__ acc_t s_tail[2][G][K];
+    __shared__ acc_t s_carry[2][K];
+    __shared__ uint32_t s_hm[2][256 / kWave];  // closing groups of the round, one word per wave
+    __shared__ unsigned long long s_or;
 
     const uint32_t tile = blockIdx.x;
     const uint32_t tid = threadIdx.x;
@@ -83,6 +94,19 @@ spmm_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row
                 wv[r] = vals ? vals[c0.y + k] : value_t(1);
             }
         }
+        if constexpr (P::kHasAux) {
+            aux_t ax[kMergeItems];
+#pragma unroll
+            for (int r = 0; r < kMergeItems; ++r) {
+                const uint32_t k = r * 256 + tid;
+                if (k < nzs) ax[r] = pol.aux(ci[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < kMergeItems; ++r) {
+                const uint32_t k = r * 256 + tid;
+                if (k < nzs) s_aux[k] = ax[r];
+            }
+        }
 #pragma unroll
         for (int r = 0; r < kMergeItems; ++r) {
             const uint32_t k = r * 256 + tid;
@@ -95,21 +119,10 @@ spmm_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row
         if (tid == 0) s_rowend[rows] = kInf;
     }
     __syncthreads();
-
-    const uint32_t j = tid % K;  // column of this lane
-    const uint32_t g = tid / K;  // virtual thread slot within a round
-    acc_t carry = S::identity();  // threads 0..K-1: open-row fold of column j
-    bool seen_end = false;
-    uint32_t counted = 0;  // rows this thread's policy calls reported
-    // The first closed row is deferred to the fixup when it began in an earlier tile.
-    const bool first_started_before = (rows > 0) && (row_offsets[c0.x] < c0.y);
-
-    for (int round = 0; round < kMergeThreads / G; ++round) {
-        const uint32_t vt = round * G + g;
-        uint32_t d = vt * kMergeItems;
+    {
+        // start coordinate of virtual thread tid (the SpMV thread's search)
+        uint32_t d = tid * kMergeItems;
         if (d > total) d = total;
-        uint32_t d_end = d + kMergeItems;
-        if (d_end > total) d_end = total;
         uint32_t lo = d > nzs ? d - nzs : 0u;
         uint32_t hi = d < rows ? d : rows;
         while (lo < hi) {
@@ -117,66 +130,174 @@ spmm_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row
             if (static_cast<uint64_t>(s_rowend[mid]) + 1 <= static_cast<uint64_t>(c0.y) + (d - mid)) lo = mid + 1;
             else hi = mid;
         }
-        uint32_t i = lo;
-        uint32_t jn = d - lo;
+        s_vi[tid] = lo;
+        s_vj[tid] = d - lo;
+        if (tid == 0) {
+            s_vi[kMergeThreads] = rows;
+            s_vj[kMergeThreads] = nzs;
+        }
+        if (tid < static_cast<uint32_t>(K)) s_carry[0][tid] = S::identity();
+    }
+    __syncthreads();
+
+    const uint32_t j = tid % K;  // column of this lane
+    const uint32_t g = tid / K;  // virtual thread slot within a round
+    const uint32_t wave = tid / kWave;
+    // The tile's first closed row is deferred to the fixup when it began in an earlier tile.
+    const bool first_started_before = (rows > 0) && (row_offsets[c0.x] < c0.y);
+    bool seen = false;     // some row closed in an earlier round
+    uint64_t counted = 0;  // OR of the changed-column masks this lane's policy calls reported
+
+    // Loads of round r: x values of its nonzeros, old values of the rows it closes (fixed count).
+    auto fetch = [&](int r, x_t (&xs)[kMergeItems], x_t (&ds)[kMergeItems]) {
+        const uint32_t vt = r * G + g;
+        const uint32_t j0 = s_vj[vt], j1 = s_vj[vt + 1];
+        const uint32_t i0 = s_vi[vt], i1 = s_vi[vt + 1];
+#pragma unroll
+        for (int t = 0; t < kMergeItems; ++t) {
+            const bool has = j0 + t < j1;
+            const uint32_t q = has ? j0 + t : 0u;
+            const uint32_t u = has ? s_col[q] : 0u;
+            xs[t] = pol.x(u, has ? s_aux[P::kHasAux ? q : 0u] : aux_t(0), j, has);
+            if constexpr (P::kRowOld) {
+                const bool hr = i0 + t < i1;
+                ds[t] = pol.row_old(hr ? c0.x + i0 + t : 0u, j);
+            }
+        }
+    };
+
+    // One round: walk, then the closing virtual threads. xc/dc hold the round's loads.
+    auto round = [&](int r, x_t (&xc)[kMergeItems], x_t (&dc)[kMergeItems]) {
+
+        // ---- walk of virtual thread vt (the SpMV thread's loop, over this lane's column)
+        const uint32_t vt = r * G + g;
+        uint32_t d = vt * kMergeItems;
+        if (d > total) d = total;
+        uint32_t d_end = d + kMergeItems;
+        if (d_end > total) d_end = total;
+        uint32_t i = s_vi[vt];
+        uint32_t jn = s_vj[vt];
+        x_t p[kMergeItems];
+#pragma unroll
+        for (int t = 0; t < kMergeItems; ++t) {
+            uint32_t q = jn + t;
+            if (q >= static_cast<uint32_t>(kMergeTile)) q = kMergeTile - 1;
+            p[t] = S::combine(s_w[q], xc[t]);
+        }
         acc_t acc = S::identity();
         acc_t head = S::identity();
         bool hasend = false;
         uint32_t first_row = 0;
-        for (uint32_t k = d; k < d_end; ++k) {
-            if (c0.y + jn < s_rowend[i]) {
-                S::add(acc, S::combine(s_w[jn], pol.x(s_col[jn], j)));
-                ++jn;
-            } else {
-                if (!hasend) {
-                    head = acc;
-                    hasend = true;
-                    first_row = c0.x + i;
+        x_t head_old = x_t(0);
+        uint32_t re = s_rowend[i];
+#pragma unroll
+        for (int k = 0; k < kMergeItems; ++k) {
+            if (d + k < d_end) {
+                if (c0.y + jn < re) {
+                    S::add(acc, p[0]);
+#pragma unroll
+                    for (int t = 0; t + 1 < kMergeItems; ++t) p[t] = p[t + 1];
+                    ++jn;
                 } else {
-                    counted += pol.row_done(c0.x + i, j, S::finalize(acc));
-                }
-                acc = S::identity();
-                ++i;
-            }
-        }
-        s_head[g][j] = head;
-        s_tail[g][j] = acc;
-        if (j == 0) {
-            s_hasend[g] = hasend ? 1u : 0u;
-            s_first[g] = first_row;
-        }
-        __syncthreads();
-        if (tid < static_cast<uint32_t>(K)) {
-            for (int gg = 0; gg < G; ++gg) {
-                if (s_hasend[gg]) {
-                    const acc_t whole = S::merge(carry, s_head[gg][tid]);
-                    if (!seen_end && first_started_before) {
-                        recs[tile].head[tid] = whole;
+                    if (!hasend) {
+                        head = acc;
+                        hasend = true;
+                        first_row = c0.x + i;
+                        head_old = dc[0];
                     } else {
-                        counted += pol.row_done(s_first[gg], tid, S::finalize(whole));
+                        counted |= pol.row_done(c0.x + i, j, S::finalize(acc), dc[0]);
                     }
-                    seen_end = true;
-                    carry = S::merge(S::identity(), s_tail[gg][tid]);
-                } else {
-                    carry = S::merge(carry, s_tail[gg][tid]);
+#pragma unroll
+                    for (int t = 0; t + 1 < kMergeItems; ++t) dc[t] = dc[t + 1];
+                    acc = S::identity();
+                    ++i;
+                    re = s_rowend[i];
                 }
             }
         }
-        __syncthreads();
-    }
-    if (tid < static_cast<uint32_t>(K)) {
-        recs[tile].carry[tid] = carry;
-        if (tid == 0) {
-            uint32_t flags = seen_end ? kTileHasEnd : 0u;
-            if (seen_end && first_started_before) flags |= kTileDeferred;
-            recs[tile].flags = flags;
+        const int par = r & 1;
+        s_tail[par][g][j] = acc;
+        {
+            const uint64_t b = ballot(j == 0 && hasend);
+            if (lane_id() == 0) {
+                uint32_t hw = 0;
+#pragma unroll
+                for (int q = 0; q < kWavesG; ++q) hw |= static_cast<uint32_t>((b >> (q * K)) & 1ull) << q;
+                s_hm[par][wave] = hw << (wave * kWavesG);
+            }
         }
-    }
-    if (tile_counts) {
-        const uint32_t w = wave_sum(counted);
-        if (lane_id() == 0) s_red[tid / kWave] = w;
         __syncthreads();
-        if (tid == 0) tile_counts[tile] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+
+        // ---- closing virtual threads: fold the tails since the last closer, then the head
+        uint32_t hm = 0;
+#pragma unroll
+        for (int q = 0; q < 256 / kWave; ++q) hm |= s_hm[par][q];
+        const acc_t open = s_carry[par][j];
+        if (hasend) {
+            const uint32_t below = hm & ((g == 31) ? 0x7fffffffu : ((1u << g) - 1u));
+            acc_t chain;
+            uint32_t v0;
+            if (below) {
+                v0 = 31u - __clz(below);
+                chain = S::identity();
+            } else {
+                v0 = 0;
+                chain = open;
+            }
+            for (uint32_t v = v0; v < g; ++v) chain = S::merge(chain, s_tail[par][v][j]);
+            const acc_t whole = S::merge(chain, head);
+            if (!below && !seen && first_started_before) {
+                recs[tile].head[j] = whole;
+            } else {
+                counted |= pol.row_done(first_row, j, S::finalize(whole), head_old);
+            }
+        }
+        if (g == static_cast<uint32_t>(G - 1)) {
+            acc_t c;
+            uint32_t v0;
+            if (hm) {
+                v0 = 31u - __clz(hm);
+                c = S::identity();
+            } else {
+                v0 = 0;
+                c = open;
+            }
+            for (uint32_t v = v0; v < static_cast<uint32_t>(G); ++v) c = S::merge(c, s_tail[par][v][j]);
+            s_carry[par ^ 1][j] = c;
+            if (r + 1 == kRounds) {
+                recs[tile].carry[j] = c;
+                if (j == 0) {
+                    const bool any = seen || hm != 0;
+                    uint32_t flags = any ? kTileHasEnd : 0u;
+                    if (any && first_started_before) flags |= kTileDeferred;
+                    recs[tile].flags = flags;
+                }
+            }
+        }
+        seen = seen || hm != 0;
+    };
+
+    // Two register buffers in turn (no copies, so the loads of the next round stay in flight while a
+    // round walks); every fetch issues the same number of loads (the last ones repeat the final round).
+    x_t xa[kMergeItems], da[kMergeItems], xb[kMergeItems], db[kMergeItems];
+    fetch(0, xa, da);
+    for (int r = 0; r < kRounds; r += 2) {
+        fetch(r + 1, xb, db);
+        round(r, xa, da);
+        fetch(r + 2 < kRounds ? r + 2 : kRounds - 1, xa, da);
+        round(r + 1, xb, db);
+    }
+    // Columns with a changed row: OR them into *changed, with an atomic only when this tile adds bits
+    // that are not set yet (a device-wide word hit by every tile would serialize them).
+    if (changed) {
+        if (tid == 0) s_or = 0ull;
+        __syncthreads();
+        if (counted) atomicOr(&s_or, static_cast<unsigned long long>(counted));
+        __syncthreads();
+        if (tid == 0 && s_or) {
+            const unsigned long long cur = *reinterpret_cast<volatile unsigned long long*>(changed);
+            if (s_or & ~cur) atomicOr(changed, s_or);
+        }
     }
 }
 
@@ -185,7 +306,7 @@ spmm_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row
 template <typename S, int K, typename P>
 __global__ void __launch_bounds__(256) spmm_fixup_kernel(uint32_t num_tiles, const MergeCoord* __restrict__ coords,
                                                         const SpmmTileRec<S, K>* __restrict__ recs, P pol,
-                                                        uint32_t* __restrict__ tile_counts) {
+                                                        unsigned long long* __restrict__ changed) {
     using acc_t = typename S::acc_t;
     const uint32_t t = (blockIdx.x * 256 + threadIdx.x) / K;
     const uint32_t j = threadIdx.x % K;
@@ -197,8 +318,11 @@ __global__ void __launch_bounds__(256) spmm_fixup_kernel(uint32_t num_tiles, con
     acc_t acc = S::identity();
     for (uint32_t v = u; v < t; ++v) acc = S::merge(acc, recs[v].carry[j]);
     acc = S::merge(acc, recs[t].head[j]);
-    const uint32_t c = pol.row_done(coords[t].x, j, S::finalize(acc));
-    if (c && tile_counts) tile_counts[t] += c;  // one group per deferred tile: no other writer
+    const uint32_t row = coords[t].x;
+    typename S::x_t old = typename S::x_t(0);
+    if constexpr (P::kRowOld) old = pol.row_old(row, j);
+    const uint64_t c = pol.row_done(row, j, S::finalize(acc), old);
+    if (c && changed) atomicOr(changed, static_cast<unsigned long long>(c));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -206,43 +330,53 @@ __global__ void __launch_bounds__(256) spmm_fixup_kernel(uint32_t num_tiles, con
 // columns whose distance changed in the last iteration (mask_prev[v] bit j). A source contributes to
 // column j only when that bit is set, so inactive sources cost one 8-byte mask read per nonzero. The
 // row's owner applies the result: d = min(D, y); changed bits go to mask_next[v] (written once per row
-// per iteration, so the masks need no atomics and no clearing). Rows that changed are counted per tile.
+// per iteration, so the masks need no atomics and no clearing). The columns that changed are ORed into a
+// per-iteration word, so each column's convergence is known.
 template <int K>
 struct MinPlusBatchPolicy {
+    using aux_t = unsigned long long;
+    static constexpr bool kHasAux = true;   // the source's change mask, staged per nonzero
+    static constexpr bool kRowOld = true;   // the apply needs the row's old distance
     uint32_t* D;
     const unsigned long long* mask_prev;
     unsigned long long* mask_next;
-    __device__ __forceinline__ uint32_t x(uint32_t u, uint32_t j) const {
-        const unsigned long long m = mask_prev[u];
-        return ((m >> j) & 1ull) ? D[static_cast<size_t>(u) * K + j] : kInf;
+    __device__ __forceinline__ aux_t aux(uint32_t u) const { return mask_prev[u]; }
+    // Always one load (a resident dummy row for inactive entries) so every round issues the same count.
+    __device__ __forceinline__ uint32_t x(uint32_t u, aux_t m, uint32_t j, bool has) const {
+        const bool on = has && ((m >> j) & 1ull);
+        const uint32_t v = D[on ? static_cast<size_t>(u) * K + j : j];
+        return on ? v : kInf;
+    }
+    __device__ __forceinline__ uint32_t row_old(uint32_t row, uint32_t j) const {
+        return D[static_cast<size_t>(row) * K + j];
     }
     // Called by the K lanes of a group together (lane j = column j); returns 1 on lane 0 of the group
-    // when the row changed in some column (the count of active rows of the next iteration).
-    __device__ __forceinline__ uint32_t row_done(uint32_t row, uint32_t j, uint32_t y) const {
-        const size_t at = static_cast<size_t>(row) * K + j;
-        bool improved = false;
-        if (y != kInf) {
-            const uint32_t cur = D[at];
-            if (y < cur) {
-                D[at] = y;
-                improved = true;
-            }
-        }
+    // with the columns of the row that changed (0 on the other lanes).
+    __device__ __forceinline__ uint64_t row_done(uint32_t row, uint32_t j, uint32_t y, uint32_t old) const {
+        const bool improved = y < old;
+        if (improved) D[static_cast<size_t>(row) * K + j] = y;
         const uint64_t bits = group_bits<K>(ballot(improved));
         if (j == 0) mask_next[row] = bits;
-        return (j == 0 && bits) ? 1u : 0u;
+        return (j == 0) ? bits : 0ull;
     }
 };
 
 // Batched plus-times: writes y row-major; the PageRank-style apply runs as its own kernel.
 template <int K>
 struct PlusTimesBatchPolicy {
+    using aux_t = uint32_t;
+    static constexpr bool kHasAux = false;
+    static constexpr bool kRowOld = false;
     const float* X;
     float* Y;
-    __device__ __forceinline__ float x(uint32_t u, uint32_t j) const { return X[static_cast<size_t>(u) * K + j]; }
-    __device__ __forceinline__ uint32_t row_done(uint32_t row, uint32_t j, float y) const {
+    __device__ __forceinline__ aux_t aux(uint32_t) const { return 0u; }
+    __device__ __forceinline__ float x(uint32_t u, aux_t, uint32_t j, bool) const {
+        return X[static_cast<size_t>(u) * K + j];
+    }
+    __device__ __forceinline__ float row_old(uint32_t, uint32_t) const { return 0.0f; }
+    __device__ __forceinline__ uint64_t row_done(uint32_t row, uint32_t j, float y, float) const {
         Y[static_cast<size_t>(row) * K + j] = y;
-        return 0u;
+        return 0ull;
     }
 };
 

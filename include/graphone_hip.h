@@ -264,6 +264,85 @@ g1_status g1_ppr_batch(g1_graph* graph, const uint32_t* seeds, uint32_t count, f
                        uint32_t max_iter, double tol, uint32_t k, float* rank_dev, g1_batch_stats* stats,
                        g1_stream_t stream);
 
+/* ---- query server ------------------------------------------------------------------------------
+ * No counterpart in the reference (its apps run one query per process, apps/sssp_adv.cu:74-124): a
+ * persistent request loop over graphs resident in HBM. One host worker thread and one HIP stream per
+ * graph, so queries on different graphs overlap on the device. Per graph, BFS runs on the single-source
+ * direction-optimizing engine; SSSP and personalized PageRank join a continuous-batching column session
+ * (a multi-source SpMM whose columns are freed and refilled one query at a time while the others keep
+ * iterating) when enough of them are waiting, and run on the single-source engines otherwise. Every
+ * result is the one the single-source call returns (bit-identical), whichever path and whichever
+ * neighbours it ran with.
+ * Admission: a query is admitted when the HBM it needs (its result vector, plus a session's work
+ * arrays when it opens one) fits in the budget, counting the registered graphs, open sessions and
+ * results not yet fetched; otherwise it waits, in order, for memory to be freed (fetch, cancel). */
+typedef struct g1_server g1_server;
+
+typedef struct g1_server_config {
+    uint64_t hbm_budget_bytes; /* 0: 90% of the device's free memory at creation                   */
+    uint32_t batch_k;          /* session column width (8, 16, 32, 64); 0: 64                       */
+    uint32_t batch_min_sssp;   /* open an SSSP session when this many SSSP queries wait; 0: 8       */
+    uint32_t batch_min_ppr;    /* open a PPR session when this many PPR queries wait; 0: 2           */
+    float ppr_restart;         /* restart probability of every PPR query; 0: 0.15                   */
+    uint32_t reserved[3];
+} g1_server_config;
+
+#define G1_QUERY_BFS 0
+#define G1_QUERY_SSSP 1
+#define G1_QUERY_PPR 2
+
+typedef struct g1_query {
+    uint32_t graph_id;
+    int32_t kind;       /* G1_QUERY_*                                   */
+    uint32_t source;    /* source (BFS/SSSP) or seed vertex (PPR)       */
+    uint32_t max_iter;  /* PPR only                                     */
+    double tol;         /* PPR only: FP64 L1 stop threshold             */
+} g1_query;
+
+#define G1_Q_QUEUED 0
+#define G1_Q_RUNNING 1
+#define G1_Q_DONE 2
+#define G1_Q_FAILED 3
+#define G1_Q_CANCELLED 4
+#define G1_Q_FETCHED 5
+
+#define G1_PATH_SINGLE 0  /* single-source engine           */
+#define G1_PATH_BATCHED 1 /* a column of a batching session */
+
+typedef struct g1_query_status {
+    int32_t state;          /* G1_Q_*                                                          */
+    int32_t status;         /* g1_status of a failed query                                     */
+    int32_t path;           /* G1_PATH_*                                                       */
+    uint32_t iterations;    /* engine iterations the query ran                                 */
+    double t_submit_ms;     /* server clock (ms since the server started)                      */
+    double t_admit_ms;
+    double t_done_ms;
+    uint64_t result_bytes;  /* n * 4: u32 distances / depths, or f32 ranks                     */
+} g1_query_status;
+
+typedef struct g1_server_stats {
+    uint64_t submitted, completed, failed, cancelled;
+    uint64_t queued, running;
+    uint64_t admission_waits;  /* queries that waited for HBM at the head of their graph's queue */
+    uint64_t batched, single;  /* completed queries per path                                    */
+    uint64_t session_steps;    /* batched iterations run                                        */
+    uint64_t hbm_budget, hbm_used, hbm_peak;
+} g1_server_stats;
+
+g1_status g1_server_create(const g1_server_config* config, g1_server** out);
+/* Cancels what is still queued or running, joins the workers, frees every result and session. */
+g1_status g1_server_destroy(g1_server* server);
+/* The graph stays owned by the caller and must outlive the server; it needs its transpose. */
+g1_status g1_server_add_graph(g1_server* server, g1_graph* graph, uint32_t* graph_id);
+g1_status g1_server_submit(g1_server* server, const g1_query* query, uint64_t* query_id);
+g1_status g1_server_poll(g1_server* server, uint64_t query_id, g1_query_status* status);
+/* Waits until the query is done, failed or cancelled, or timeout_ms passes (G1_ERR_BUSY then). */
+g1_status g1_server_wait(g1_server* server, uint64_t query_id, double timeout_ms, g1_query_status* status);
+/* Copies a done query's result (n values) to dst and frees it on the device. */
+g1_status g1_server_fetch(g1_server* server, uint64_t query_id, void* dst, uint64_t dst_bytes, int32_t dst_on_device);
+g1_status g1_server_cancel(g1_server* server, uint64_t query_id);
+g1_status g1_server_get_stats(g1_server* server, g1_server_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

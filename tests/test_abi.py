@@ -57,3 +57,29 @@ def test_merge_workspace_size_formula():
     assert L.g1_spmv_csrvec_lanes(4194304, 65000000) == 16
     assert L.g1_spmv_csrvec_lanes(45369, 108374) == 2  # integer average 2
     assert L.g1_spmv_csrvec_lanes(10, 5000) == 64
+
+
+def test_server_argument_errors_without_a_gpu():
+    """Failure paths of the query server that need no device: bad configuration, unknown graph or query,
+    waiting on or fetching what does not exist."""
+    with pytest.raises(g1.G1Error) as e:
+        g1.Server(hbm_budget_bytes=1 << 20, batch_k=12)
+    assert e.value.status == _lib.G1_ERR_INVALID_ARG
+    with pytest.raises(g1.G1Error):
+        g1.Server(hbm_budget_bytes=1 << 20, ppr_restart=1.5)
+    srv = g1.Server(hbm_budget_bytes=1 << 20)
+    with pytest.raises(g1.G1Error) as e:
+        srv.submit(0, "bfs", 0)
+    assert e.value.status == _lib.G1_ERR_NOT_FOUND
+    for fn in (srv.poll, srv.cancel):
+        with pytest.raises(g1.G1Error) as e:
+            fn(12345)
+        assert e.value.status == _lib.G1_ERR_NOT_FOUND
+    with pytest.raises(g1.G1Error):
+        srv.wait(12345, 1.0)
+    s = srv.stats()
+    assert s["hbm_budget"] == 1 << 20 and s["submitted"] == 0
+    srv.close()
+    L = _lib.lib()
+    assert L.g1_server_destroy(None) == _lib.G1_OK
+    assert L.g1_server_get_stats(None, None) == _lib.G1_ERR_INVALID_ARG
