@@ -160,6 +160,49 @@ def batched_queries(g1, g, host, count: int, seed: int) -> dict:
     return out
 
 
+def pagerank_benchmark(g1, scale: int, seed: int, with_oracle: bool) -> dict:
+    """PageRank (apps/pagerank_blas.cu formula) on an unweighted RMAT graph (edge factor 16, the main a/b/c/d):
+    damping 0.85, FP32 ranks with Kahan row sums, FP64 L1 stop at 1e-7 or 50 iterations, both SpMV engines.
+    Per iteration the algorithmic bytes are 4(n+1) + 4m + 4m (x gather) + 4n (y) for the SpMV and 20n for
+    the apply. Error against the FP64 oracle (CPU) when with_oracle."""
+    import oracle
+
+    t0 = time.perf_counter()
+    h = g1.HostGraph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=seed + 10)
+    gr = g1.Graph(h)
+    log(f"pagerank graph RMAT{scale}: n={h.n} m={h.m} ready in {time.perf_counter() - t0:.1f}s")
+    n, m = h.n, h.m
+    r = g1.DeviceArray(n, np.float32)
+    out = {"workload": f"rmat{scale}_pagerank", "n": n, "m": m, "alpha": 0.85, "tol_l1": 1e-7, "max_iter": 50}
+    for eng in ("merge", "csrvec"):
+        gr.pagerank(0.85, 50, 1e-7, "l1", eng, out=r, to_host=False)
+        gr.profile(True)
+        res = gr.pagerank(0.85, 50, 1e-7, "l1", eng, out=r, to_host=False)
+        prof = gr.read_profile()
+        gr.profile(False)
+        it = res.stats["iterations"]
+        spmv = [e for e in prof if e["name"].startswith("spmv")]
+        sp_ms = sum(e["total_ms"] for e in spmv)
+        sp_b = sum(e["algorithmic_bytes"] for e in spmv)
+        out[eng] = {"ms": round(res.stats["elapsed_ms"], 3), "iterations": it,
+                    "ms_per_iteration": round(res.stats["elapsed_ms"] / max(it, 1), 4),
+                    "spmv_gbs": round(sp_b / max(sp_ms, 1e-9) / 1e6, 1),
+                    "spmv_frac": round(sp_b / max(sp_ms, 1e-9) / 1e6 / HBM_PEAK_GBS, 4),
+                    "last_l1_delta": res.stats["last_delta"]}
+        if eng == "merge":
+            ranks = r.numpy()
+    if with_oracle:
+        t1 = time.perf_counter()
+        t = h.transpose()
+        want, ost = oracle.pagerank_f64(h.row_offsets, None, t.row_offsets, t.col_indices, None, 0.85,
+                                        max_iter=50, tol=1e-7)
+        diff = np.abs(ranks.astype(np.float64) - want)
+        out["vs_fp64_oracle"] = {"l1": float(diff.sum()), "max_abs": float(diff.max()),
+                                 "oracle_iterations": ost["iterations"],
+                                 "oracle_seconds": round(time.perf_counter() - t1, 1)}
+    return out
+
+
 def serving_benchmark(g1, args) -> dict:
     """Query serving over four resident graphs: RMAT scale 18, 20 and 23 (edge factor 16, a/b/c/d as the
     main workload, u32 weights [1, 255]) and a 2048 x 2048 grid with edge retention 0.6 (symmetric u32
@@ -269,6 +312,12 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-sources", type=int, default=64)
+    ap.add_argument("--pagerank-scale", type=int, default=0,
+                    help="also run PageRank on an unweighted RMAT graph of this scale (24: the RMAT24 workload) "
+                         "with its FP64-oracle error")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with a 1 GiB device-to-device copy: the known-size kernel that calibrates "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rates", default="100,1000,3000,10000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -350,14 +399,30 @@ def main() -> int:
                                    "gteps": round(m_reach / rb.stats["elapsed_ms"] / 1e6, 3)}
         log(json.dumps(extras))
 
+    if args.copy_probe:
+        a = g1.DeviceArray(1 << 28, np.uint32)
+        b = g1.DeviceArray(1 << 28, np.uint32)
+        g1._lib.call("g1_fill_u32", g1.C.c_void_p(a.ptr), 1, g1.C.c_size_t(1 << 28), None)
+        g1._lib.call("g1_memcpy_d2d", g1.C.c_void_p(b.ptr), g1.C.c_void_p(a.ptr), g1.C.c_size_t(1 << 30), None)
+        g1.synchronize()
+        del a, b
+        log("copy probe: 1 GiB device-to-device copy done")
+
     batch = None
     if not args.no_batch:
         batch = batched_queries(g1, g, host, args.batch_sources, args.seed + 2)
         log(json.dumps(batch))
 
+    pagerank = None
+    if args.pagerank_scale:
+        del g
+        g = None
+        pagerank = pagerank_benchmark(g1, args.pagerank_scale, args.seed, not args.no_cpu)
+        log(json.dumps(pagerank))
+
     serving = None
     if not args.no_serve:
-        del g
+        g = None
         serving = serving_benchmark(g1, args)
         log(json.dumps(serving))
 
@@ -399,6 +464,7 @@ def main() -> int:
         "extras": extras,
         "batch": batch,
         "serving": serving,
+        "pagerank": pagerank,
     }
     print(json.dumps(line), flush=True)
     return 0

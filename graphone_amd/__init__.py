@@ -353,8 +353,9 @@ class Server:
     HBM-budget admission and continuous batching of SSSP / personalized-PageRank queries."""
 
     def __init__(self, hbm_budget_bytes: int = 0, batch_k: int = 0, batch_min_sssp: int = 0,
-                 batch_min_ppr: int = 0, ppr_restart: float = 0.0):
+                 batch_min_ppr: int = 0, ppr_restart: float = 0.0, single_workers: int = 0):
         cfg = _lib.ServerConfig()
+        cfg.single_workers = int(single_workers)
         cfg.hbm_budget_bytes = int(hbm_budget_bytes)
         cfg.batch_k = int(batch_k)
         cfg.batch_min_sssp = int(batch_min_sssp)

@@ -101,7 +101,8 @@ class ServerConfig(C.Structure):
         ("batch_min_sssp", C.c_uint32),
         ("batch_min_ppr", C.c_uint32),
         ("ppr_restart", C.c_float),
-        ("reserved", C.c_uint32 * 3),
+        ("single_workers", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -191,6 +192,7 @@ _SIGNATURES = {
     "g1_host_csr_free": (None, [C.POINTER(HostCSR)]),
     "g1_graph_create": (_st, [C.POINTER(_vp), C.POINTER(HostCSR), _u32]),
     "g1_graph_destroy": (_st, [_vp]),
+    "g1_graph_create_view": (_st, [_vp, C.POINTER(_vp)]),
     "g1_graph_info": (_st, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u64)]),
     "g1_graph_csr_dev": (_st, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "g1_profile_enable": (_st, [_vp, _i32]),

@@ -6,6 +6,7 @@
 // keeps the fastest per source. Min-plus iterations are captured once per (graph, k) as a HIP graph of
 // two iterations (the change masks swap back) and replayed until no column changes.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -31,6 +32,16 @@ struct BatchWork {
 
 namespace g1 {
 namespace {
+
+// Min-plus SpMM kernel: the chunked, barrier-free walk (default) or the round-based one of the
+// plus-times path (G1_MINPLUS_ROUNDS=1); both give the same bits.
+bool minplus_chunked() {
+    static const bool chunked = [] {
+        const char* e = getenv("G1_MINPLUS_ROUNDS");
+        return !(e && e[0] == '1');
+    }();
+    return chunked;
+}
 
 int k_slot(uint32_t k) { return k == 8 ? 0 : k == 16 ? 1 : k == 32 ? 2 : k == 64 ? 3 : -1; }
 
@@ -152,8 +163,12 @@ g1_status minplus_iteration(g1_graph* g, BatchWork* b, int prev, bool weighted, 
     MinPlusBatchPolicy<K> pol{b->D, b->mask[prev], b->mask[prev ^ 1]};
     auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(b->recs);
     const uint32_t* vals = weighted ? g->t_w : nullptr;
-    spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
-        g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, flag);
+    if (minplus_chunked())
+        spmm_minplus_kernel<K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, flag);
+    else
+        spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, flag);
     G1_LAUNCHED();
     if (tiles > 1) {
         const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
@@ -493,8 +508,12 @@ g1_status session_step_minplus(Session* ss, uint64_t* done) {
     MinPlusBatchPolicy<K> pol{ss->D, ss->mask[ss->mcur], ss->mask[ss->mcur ^ 1]};
     auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(ss->recs);
     const uint32_t* vals = ss->kind == kSessMinPlusWeighted ? g->t_w : nullptr;
-    spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
-        g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, ss->changed);
+    if (minplus_chunked())
+        spmm_minplus_kernel<K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, ss->changed);
+    else
+        spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, ss->changed);
     G1_LAUNCHED();
     if (tiles > 1) {
         const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);

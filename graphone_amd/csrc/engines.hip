@@ -5,6 +5,7 @@
 // Host loop: one launch per engine phase, then an 8-byte readback of the device frontier counter
 // (the convergence test the reference does with frontier.IsConvergent()/a D2H copy per iteration).
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -26,6 +27,16 @@ uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
     if (b < 1) b = 1;
     if (b > cap) b = cap;
     return static_cast<uint32_t>(b);
+}
+
+// SSSP relaxation order (measurement switch): G1_SSSP_PHASED=1 reads all of a thread's target
+// distances before relaxing any.
+uint32_t sssp_phased() {
+    static const uint32_t v = [] {
+        const char* e = getenv("G1_SSSP_PHASED");
+        return (e && e[0] == '1') ? 1u : 0u;
+    }();
+    return v;
 }
 
 uint32_t persistent_cap() { return static_cast<uint32_t>(device_cu_count()) * 8; }
@@ -259,7 +270,7 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     int tok = -1;
     auto launch = [&](uint32_t it) -> g1_status {
         PushQueues pq = make_queues(g, it);
-        SsspPush pol{g->dstamp, it + 1};
+        SsspPush pol{g->dstamp, it + 1, sssp_phased()};
         tok = prof_begin(g, "sssp_advance", s);
         push_advance_kernel<SsspPush><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq, pol, it,
                                                                              g->hist);

@@ -67,6 +67,8 @@ g1_status ensure_pull_work(g1_graph* g, bool pagerank) {
         G1_TRY(alloc_n(g, &g->iw, n));
         G1_TRY(alloc_n(g, &g->r[0], n));
         G1_TRY(alloc_n(g, &g->r[1], n));
+    }
+    if (pagerank && !g->osum) {
         G1_TRY(alloc_n(g, &g->osum, n));
         if (g->n) {
             const uint32_t blocks = std::min<uint32_t>((g->n + 3) / 4, 16384);
@@ -234,6 +236,50 @@ g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags
     if (hipEventCreate(&g->ev[0]) != hipSuccess || hipEventCreate(&g->ev[1]) != hipSuccess)
         return fail(set_error(G1_ERR_HIP, "g1_graph_create: event creation failed"));
     if (hipDeviceSynchronize() != hipSuccess) return fail(set_error(G1_ERR_HIP, "g1_graph_create: sync failed"));
+    *out = g;
+    return G1_OK;
+}
+
+g1_status g1_graph_create_view(g1_graph* base, g1_graph** out) {
+    using namespace g1;
+    G1_CHECK(base != nullptr && out != nullptr, G1_ERR_INVALID_ARG, "g1_graph_create_view: null argument");
+    *out = nullptr;
+    {
+        // the arrays a view shares are built in the base first (float weights, out-weight sums)
+        std::lock_guard<std::mutex> lock(base->mu);
+        if (base->has_t) G1_TRY(ensure_pull_work(base, true));
+    }
+    g1_graph* g = new g1_graph();
+    g->n = base->n;
+    g->m = base->m;
+    g->flags = base->flags;
+    g->row_off = base->row_off;
+    g->col = base->col;
+    g->w = base->w;
+    g->t_row_off = base->t_row_off;
+    g->t_col = base->t_col;
+    g->t_w = base->t_w;
+    g->t_wf = base->t_wf;
+    g->has_t = base->has_t;
+    g->t_alias = base->t_alias;
+    g->t_ws = base->t_ws;
+    g->t_ws_bytes = base->t_ws_bytes;
+    g->osum = base->osum;
+    g->best_k[0] = base->best_k[0];
+    g->best_k[1] = base->best_k[1];
+    auto fail = [&](g1_status s) {
+        graph_free_all(g);
+        delete g;
+        return s;
+    };
+    g1_status s;
+    if ((s = alloc_n(g, &g->ctr, 16)) != G1_OK) return fail(s);
+    if ((s = alloc_n(g, &g->partials, kPartials + 8)) != G1_OK) return fail(s);
+    if (hipHostMalloc(reinterpret_cast<void**>(&g->h_ctr), 256 * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&g->h_scalar), 8 * sizeof(double)) != hipSuccess)
+        return fail(set_error(G1_ERR_HIP, "g1_graph_create_view: pinned host allocation failed"));
+    if (hipEventCreate(&g->ev[0]) != hipSuccess || hipEventCreate(&g->ev[1]) != hipSuccess)
+        return fail(set_error(G1_ERR_HIP, "g1_graph_create_view: event creation failed"));
     *out = g;
     return G1_OK;
 }

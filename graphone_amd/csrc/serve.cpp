@@ -48,13 +48,15 @@ struct GraphSlot {
     uint32_t id = 0;
     g1_graph* g = nullptr;
     uint32_t n = 0;
-    std::deque<Query*> queue;                  // submitted, not admitted (FIFO)
+    // guarded by the server mutex
+    std::deque<Query*> queue;     // submitted, not admitted (FIFO)
+    std::deque<Query*> admitted;  // admitted, not yet routed by the session thread
+    std::deque<Query*> singles;   // routed to a single-source engine, not started
+    uint64_t sssp_runs = 0, sssp_iters = 0;  // single-source SSSP iterations seen on this graph
+    // threads: one session thread, `single_workers` single-source threads with a view each
     std::thread worker;
-    // session bookkeeping shared with admission (guarded by the server mutex)
-    bool session_open[3] = {false, false, false};  // [G1_QUERY_*]
-    uint64_t session_reserved[3] = {0, 0, 0};
-    uint32_t waiting_kind[3] = {0, 0, 0};        // queued queries per kind
-    std::deque<Query*> admitted;                 // admitted, not yet picked up by the worker
+    std::vector<std::thread> helpers;
+    std::vector<g1_graph*> views;
 };
 
 }  // namespace
@@ -125,65 +127,94 @@ int session_kind_of(int32_t qkind) { return qkind == G1_QUERY_PPR ? g1::kSessPpr
 
 // ---- worker side ---------------------------------------------------------------------------------
 
+// Marks q final under the server lock and releases what it holds when it did not complete.
+void finish(g1_server* sv, Query* q, g1_status status, uint32_t iterations, bool cancel = false) {
+    std::lock_guard<std::mutex> lock(sv->mu);
+    if (cancel) q->cancel = true;
+    q->iterations = iterations;
+    q->t_done = sv->now_ms();
+    if (q->cancel) {
+        q->state = G1_Q_CANCELLED;
+        ++sv->st.cancelled;
+    } else if (status != G1_OK) {
+        q->state = G1_Q_FAILED;
+        q->status = status;
+        ++sv->st.failed;
+    } else {
+        q->state = G1_Q_DONE;
+        ++sv->st.completed;
+        if (q->path == G1_PATH_BATCHED) ++sv->st.batched; else ++sv->st.single;
+    }
+    if (sv->st.running) --sv->st.running;
+    if (q->state != G1_Q_DONE) {
+        recycle(sv, q->result, q->bytes);
+        q->result = nullptr;
+        if (q->reserved) sv->unreserve(q->bytes);
+        q->reserved = false;
+        sv->cv_work.notify_all();
+    }
+    sv->cv_done.notify_all();
+}
+
+// A single-source thread: takes routed queries of its graph and runs them on its own view and stream.
+void single_loop(g1_server* sv, GraphSlot* slot, g1_graph* view) {
+    hipStream_t stream = nullptr;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) stream = nullptr;
+    const g1_stream_t s = reinterpret_cast<g1_stream_t>(stream);
+    for (;;) {
+        Query* q = nullptr;
+        {
+            std::unique_lock<std::mutex> lock(sv->mu);
+            sv->cv_work.wait(lock, [&] { return sv->stop || !slot->singles.empty(); });
+            if (sv->stop) break;
+            q = slot->singles.front();
+            slot->singles.pop_front();
+            if (q->cancel) {
+                lock.unlock();
+                finish(sv, q, G1_OK, 0);
+                continue;
+            }
+        }
+        g1_run_stats rs;
+        memset(&rs, 0, sizeof(rs));
+        g1_status st;
+        switch (q->q.kind) {
+            case G1_QUERY_BFS:
+                st = g1_bfs(view, q->q.source, static_cast<uint32_t*>(q->result), G1_ENGINE_DIROPT, &rs, s);
+                break;
+            case G1_QUERY_SSSP:
+                st = g1_sssp(view, q->q.source, static_cast<uint32_t*>(q->result), G1_ENGINE_ADVANCE, &rs, s);
+                break;
+            default:
+                st = g1_ppr(view, q->q.source, sv->cfg.ppr_restart, q->q.max_iter, q->q.tol,
+                            static_cast<float*>(q->result), &rs, s);
+                break;
+        }
+        if (st == G1_OK && hipStreamSynchronize(stream) != hipSuccess) st = G1_ERR_HIP;
+        if (st == G1_OK && q->q.kind == G1_QUERY_SSSP) {
+            std::lock_guard<std::mutex> lock(sv->mu);
+            ++slot->sssp_runs;
+            slot->sssp_iters += rs.iterations;
+        }
+        finish(sv, q, st, rs.iterations);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// The session thread of a graph: routes admitted queries and drives the column sessions.
 struct Worker {
     g1_server* sv;
     GraphSlot* slot;
     hipStream_t stream = nullptr;
     g1::Session* sess[3] = {nullptr, nullptr, nullptr};  // [G1_QUERY_SSSP], [G1_QUERY_PPR] used
     Query* col_query[3][g1::kBatchCols] = {};
-    std::deque<Query*> single;      // admitted for a single-source engine
-    std::deque<Query*> pending[3];  // admitted for a session, waiting for a free column
+    std::deque<Query*> backlog[3];  // admitted, path not chosen yet, per kind
+    std::deque<Query*> pending[3];  // routed to a session, waiting for a free column
+    uint64_t sess_bytes[3] = {0, 0, 0};
+    Clock::time_point idle_since[3];
 
-    // Marks q final under the server lock and releases what it holds when it did not complete.
     void finish(Query* q, g1_status status, uint32_t iterations, bool cancel = false) {
-        std::lock_guard<std::mutex> lock(sv->mu);
-        if (cancel) q->cancel = true;
-        q->iterations = iterations;
-        q->t_done = sv->now_ms();
-        if (q->cancel) {
-            q->state = G1_Q_CANCELLED;
-            ++sv->st.cancelled;
-        } else if (status != G1_OK) {
-            q->state = G1_Q_FAILED;
-            q->status = status;
-            ++sv->st.failed;
-        } else {
-            q->state = G1_Q_DONE;
-            ++sv->st.completed;
-            if (q->path == G1_PATH_BATCHED) ++sv->st.batched; else ++sv->st.single;
-        }
-        if (sv->st.running) --sv->st.running;
-        if (q->state != G1_Q_DONE) {
-            recycle(sv, q->result, q->bytes);
-            q->result = nullptr;
-            if (q->reserved) sv->unreserve(q->bytes);
-            q->reserved = false;
-            sv->cv_work.notify_all();
-        }
-        sv->cv_done.notify_all();
-    }
-
-    g1_status run_single(Query* q) {
-        g1_graph* g = slot->g;
-        const g1_stream_t s = reinterpret_cast<g1_stream_t>(stream);
-        g1_run_stats rs;
-        memset(&rs, 0, sizeof(rs));
-        g1_status st;
-        switch (q->q.kind) {
-            case G1_QUERY_BFS:
-                st = g1_bfs(g, q->q.source, static_cast<uint32_t*>(q->result), G1_ENGINE_DIROPT, &rs, s);
-                break;
-            case G1_QUERY_SSSP:
-                st = g1_sssp(g, q->q.source, static_cast<uint32_t*>(q->result), G1_ENGINE_ADVANCE, &rs, s);
-                break;
-            default:
-                st = g1_ppr(g, q->q.source, sv->cfg.ppr_restart, q->q.max_iter, q->q.tol, static_cast<float*>(q->result),
-                            &rs, s);
-                break;
-        }
-        if (st == G1_OK && hipStreamSynchronize(stream) != hipSuccess) st = G1_ERR_HIP;
-        finish(q, st, rs.iterations);
-        return st;
+        ::finish(sv, q, status, iterations, cancel);
     }
 
     // One session iteration of kind k: fill free columns, step, hand out the finished columns.
@@ -244,7 +275,6 @@ struct Worker {
     }
 
     bool busy() const {
-        if (!single.empty()) return true;
         for (int k = 0; k < 3; ++k) {
             if (!pending[k].empty()) return true;
             if (sess[k] && g1::session_active(sess[k])) return true;
@@ -252,17 +282,94 @@ struct Worker {
         return false;
     }
 
+    // A session idle for half a second is closed and its memory returned to the budget.
     void close_idle_sessions() {
+        const auto now = Clock::now();
         for (int k = 1; k < 3; ++k) {
-            if (!sess[k] || g1::session_active(sess[k]) || !pending[k].empty()) continue;
+            if (!sess[k]) continue;
+            if (g1::session_active(sess[k]) || !pending[k].empty() || !backlog[k].empty()) {
+                idle_since[k] = now;
+                continue;
+            }
+            if (now - idle_since[k] < std::chrono::milliseconds(500)) continue;
             g1::session_destroy(sess[k]);
             sess[k] = nullptr;
             std::lock_guard<std::mutex> lock(sv->mu);
-            slot->session_open[k] = false;
-            sv->unreserve(slot->session_reserved[k]);
-            slot->session_reserved[k] = 0;
+            sv->unreserve(sess_bytes[k]);
+            sess_bytes[k] = 0;
             sv->cv_work.notify_all();
         }
+    }
+
+    // Path choice: BFS always single-source. SSSP / PPR: an open session takes every query of its kind;
+    // with no session, when the queries of the kind waiting (fresh plus routed but not started) reach the
+    // threshold, a session opens if its memory fits the budget (SSSP only on low-diameter graphs: at most
+    // 64 iterations per single-source run seen on the graph) and takes them all; otherwise they go to the
+    // single-source threads.
+    void route(int k) {
+        if (k == G1_QUERY_BFS || (!sess[k] && backlog[k].empty())) {
+            to_singles(k);
+            return;
+        }
+        if (!sess[k]) {
+            const uint32_t threshold = k == G1_QUERY_PPR ? sv->cfg.batch_min_ppr : sv->cfg.batch_min_sssp;
+            const uint64_t bytes = g1::session_bytes(slot->g, session_kind_of(k), sv->cfg.batch_k);
+            bool open = false;
+            {
+                std::lock_guard<std::mutex> lock(sv->mu);
+                size_t waiting = backlog[k].size();
+                for (Query* q : slot->singles) waiting += (q->q.kind == k);
+                const bool shallow = k == G1_QUERY_PPR ||
+                                     (slot->sssp_runs > 0 && slot->sssp_iters <= 64 * slot->sssp_runs);
+                if (waiting >= threshold && shallow && sv->hbm_used + bytes <= sv->cfg.hbm_budget_bytes) {
+                    sv->reserve(bytes);
+                    open = true;
+                    // take back the routed-but-not-started queries of this kind
+                    for (auto it = slot->singles.begin(); it != slot->singles.end();) {
+                        if ((*it)->q.kind == k) {
+                            backlog[k].push_back(*it);
+                            it = slot->singles.erase(it);
+                        } else {
+                            ++it;
+                        }
+                    }
+                }
+            }
+            if (open) {
+                g1::Session* ss = nullptr;
+                const g1_status st = g1::session_create(slot->g, session_kind_of(k), sv->cfg.batch_k,
+                                                        sv->cfg.ppr_restart, stream, &ss);
+                if (st == G1_OK) {
+                    sess[k] = ss;
+                    sess_bytes[k] = bytes;
+                    idle_since[k] = Clock::now();
+                } else {
+                    std::lock_guard<std::mutex> lock(sv->mu);
+                    sv->unreserve(bytes);
+                }
+            }
+        }
+        if (!sess[k]) {
+            to_singles(k);
+            return;
+        }
+        std::lock_guard<std::mutex> lock(sv->mu);
+        while (!backlog[k].empty()) {
+            Query* q = backlog[k].front();
+            backlog[k].pop_front();
+            q->path = G1_PATH_BATCHED;
+            pending[k].push_back(q);
+        }
+    }
+
+    void to_singles(int k) {
+        if (backlog[k].empty()) return;
+        std::lock_guard<std::mutex> lock(sv->mu);
+        while (!backlog[k].empty()) {
+            slot->singles.push_back(backlog[k].front());
+            backlog[k].pop_front();
+        }
+        sv->cv_work.notify_all();
     }
 
     void loop() {
@@ -288,42 +395,12 @@ struct Worker {
                     continue;
                 }
                 q->result = p;
-                if (q->path == G1_PATH_BATCHED) pending[q->q.kind].push_back(q);
-                else single.push_back(q);
+                backlog[q->q.kind].push_back(q);
             }
-            // sessions: open on demand (admission reserved their bytes), then one iteration each
-            for (int k = 1; k < 3; ++k) {
-                if (!sess[k] && !pending[k].empty()) {
-                    g1::Session* ss = nullptr;
-                    const g1_status st = g1::session_create(slot->g, session_kind_of(k), sv->cfg.batch_k,
-                                                            sv->cfg.ppr_restart, stream, &ss);
-                    if (st != G1_OK) {
-                        while (!pending[k].empty()) {
-                            finish(pending[k].front(), st, 0);
-                            pending[k].pop_front();
-                        }
-                        std::lock_guard<std::mutex> lock(sv->mu);
-                        slot->session_open[k] = false;
-                        sv->unreserve(slot->session_reserved[k]);
-                        slot->session_reserved[k] = 0;
-                        continue;
-                    }
-                    sess[k] = ss;
-                }
+            // route, then one iteration of each open session
+            for (int k = 0; k < 3; ++k) route(k);
+            for (int k = 1; k < 3; ++k)
                 if (sess[k]) session_round(k);
-            }
-            // one single-source query between session iterations
-            if (!single.empty()) {
-                Query* q = single.front();
-                single.pop_front();
-                bool c;
-                {
-                    std::lock_guard<std::mutex> lock(sv->mu);
-                    c = q->cancel;
-                }
-                if (c) finish(q, G1_OK, 0);
-                else run_single(q);
-            }
             close_idle_sessions();
         }
         // shutdown: everything still held is cancelled
@@ -333,8 +410,16 @@ struct Worker {
             for (Query* q : pending[k]) finish(q, G1_OK, 0, true);
             if (sess[k]) g1::session_destroy(sess[k]);
         }
-        for (Query* q : single) finish(q, G1_OK, 0, true);
+        for (int k = 0; k < 3; ++k)
+            for (Query* q : backlog[k]) finish(q, G1_OK, 0, true);
         if (stream) (void)hipStreamDestroy(stream);
+        std::vector<Query*> left;
+        {
+            std::lock_guard<std::mutex> lock(sv->mu);
+            left.assign(slot->singles.begin(), slot->singles.end());
+            slot->singles.clear();
+        }
+        for (Query* q : left) finish(q, G1_OK, 0, true);
     }
 };
 
@@ -345,7 +430,6 @@ void admit(g1_server* sv, GraphSlot* gs) {
         Query* q = gs->queue.front();
         if (q->cancel) {
             gs->queue.pop_front();
-            if (gs->waiting_kind[q->q.kind]) --gs->waiting_kind[q->q.kind];
             if (sv->st.queued) --sv->st.queued;
             q->state = G1_Q_CANCELLED;
             q->t_done = sv->now_ms();
@@ -353,19 +437,7 @@ void admit(g1_server* sv, GraphSlot* gs) {
             sv->cv_done.notify_all();
             continue;
         }
-        const int k = q->q.kind;
-        bool batched = false;
-        uint64_t need = q->bytes;
-        if (k != G1_QUERY_BFS) {
-            const uint32_t threshold = k == G1_QUERY_PPR ? sv->cfg.batch_min_ppr : sv->cfg.batch_min_sssp;
-            if (gs->session_open[k]) {
-                batched = true;
-            } else if (gs->waiting_kind[k] >= threshold) {
-                batched = true;
-                need += g1::session_bytes(gs->g, session_kind_of(k), sv->cfg.batch_k);
-            }
-        }
-        if (sv->hbm_used + need > sv->cfg.hbm_budget_bytes) {
+        if (sv->hbm_used + q->bytes > sv->cfg.hbm_budget_bytes) {
             if (!q->reserved && q->t_admit == 0) {
                 q->t_admit = -1;  // counted once
                 ++sv->st.admission_waits;
@@ -373,14 +445,9 @@ void admit(g1_server* sv, GraphSlot* gs) {
             return;
         }
         gs->queue.pop_front();
-        if (gs->waiting_kind[k]) --gs->waiting_kind[k];
-        if (batched && !gs->session_open[k]) {
-            gs->session_open[k] = true;
-            gs->session_reserved[k] = need - q->bytes;
-        }
-        sv->reserve(need);
+        sv->reserve(q->bytes);
         q->reserved = true;
-        q->path = batched ? G1_PATH_BATCHED : G1_PATH_SINGLE;
+        q->path = G1_PATH_SINGLE;
         q->state = G1_Q_RUNNING;
         q->t_admit = sv->now_ms();
         if (sv->st.queued) --sv->st.queued;
@@ -426,6 +493,8 @@ g1_status g1_server_create(const g1_server_config* config, g1_server** out) {
     if (c.batch_min_sssp == 0) c.batch_min_sssp = 8;
     if (c.batch_min_ppr == 0) c.batch_min_ppr = 2;
     if (c.ppr_restart == 0.0f) c.ppr_restart = 0.15f;
+    if (c.single_workers == 0) c.single_workers = 4;
+    G1_CHECK(c.single_workers <= 32, G1_ERR_INVALID_ARG, "g1_server_create: at most 32 single-source workers");
     G1_CHECK(c.ppr_restart > 0.0f && c.ppr_restart <= 1.0f, G1_ERR_INVALID_ARG,
              "g1_server_create: ppr_restart must be in (0, 1]");
     if (c.hbm_budget_bytes == 0) {
@@ -451,8 +520,12 @@ g1_status g1_server_destroy(g1_server* sv) {
         }
         sv->cv_work.notify_all();
     }
-    for (auto& gs : sv->graphs)
+    for (auto& gs : sv->graphs) {
         if (gs->worker.joinable()) gs->worker.join();
+        for (auto& h : gs->helpers)
+            if (h.joinable()) h.join();
+        for (g1_graph* v : gs->views) g1_graph_destroy(v);
+    }
     for (auto& kv : sv->queries)
         if (kv.second->result) (void)hipFree(kv.second->result);
     for (auto& kv : sv->pool)
@@ -474,6 +547,15 @@ g1_status g1_server_add_graph(g1_server* sv, g1_graph* graph, uint32_t* graph_id
     gs->id = static_cast<uint32_t>(sv->graphs.size());
     gs->g = graph;
     gs->n = n;
+    for (uint32_t i = 0; i < sv->cfg.single_workers; ++i) {
+        g1_graph* v = nullptr;
+        const g1_status st = g1_graph_create_view(graph, &v);
+        if (st != G1_OK) {
+            for (g1_graph* x : gs->views) g1_graph_destroy(x);
+            return st;
+        }
+        gs->views.push_back(v);
+    }
     sv->reserve(bytes);
     GraphSlot* raw = gs.get();
     sv->graphs.push_back(std::move(gs));
@@ -481,6 +563,7 @@ g1_status g1_server_add_graph(g1_server* sv, g1_graph* graph, uint32_t* graph_id
         Worker w{sv, raw};
         w.loop();
     });
+    for (g1_graph* v : raw->views) raw->helpers.emplace_back([sv, raw, v] { single_loop(sv, raw, v); });
     if (graph_id) *graph_id = raw->id;
     return G1_OK;
 }
@@ -510,7 +593,6 @@ g1_status g1_server_submit(g1_server* sv, const g1_query* query, uint64_t* query
     Query* raw = q.get();
     sv->queries.emplace(raw->id, std::move(q));
     gs->queue.push_back(raw);
-    ++gs->waiting_kind[query->kind];
     ++sv->st.submitted;
     ++sv->st.queued;
     *query_id = raw->id;
@@ -593,7 +675,6 @@ g1_status g1_server_cancel(g1_server* sv, uint64_t query_id) {
             auto it = std::find(gs->queue.begin(), gs->queue.end(), q);
             if (it != gs->queue.end()) {
                 gs->queue.erase(it);
-                if (gs->waiting_kind[q->q.kind]) --gs->waiting_kind[q->q.kind];
                 if (sv->st.queued) --sv->st.queued;
             }
             q->state = G1_Q_CANCELLED;

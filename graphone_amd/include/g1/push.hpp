@@ -280,6 +280,7 @@ struct SsspPush {
     static constexpr bool kQueueLeaves = false;
     unsigned long long* dstamp;
     uint32_t iter;
+    uint32_t phased;  // 1: read every slot's current distance first, then relax slot by slot
     __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
         return static_cast<uint32_t>(dstamp[u] >> 32);
     }
@@ -289,6 +290,22 @@ struct SsspPush {
     template <int N>
     __device__ __forceinline__ void relax_many(const uint32_t (&)[N], const uint32_t (&du)[N], const uint32_t (&v)[N],
                                                const uint32_t (&w)[N], const uint32_t (&)[N], bool (&won)[N]) const {
+        if (phased) {
+            uint32_t cur[N];
+#pragma unroll
+            for (int r = 0; r < N; ++r) cur[r] = (v[r] != kInf) ? static_cast<uint32_t>(dstamp[v[r]] >> 32) : 0u;
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                won[r] = false;
+                if (v[r] == kInf) continue;
+                const uint32_t nd = du[r] + w[r];
+                if (nd >= cur[r]) continue;
+                const unsigned long long old =
+                    atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | iter);
+                won[r] = nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
+            }
+            return;
+        }
 #pragma unroll
         for (int r = 0; r < N; ++r) {
             won[r] = false;

@@ -301,6 +301,213 @@ spmm_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row
     }
 }
 
+// Min-plus variant of the tile kernel for order-free semirings (min is exact, so any fold order gives
+// the same bits): group g of K lanes walks the contiguous run of K virtual threads g*K .. g*K+K-1 (8K
+// merge items) alone, with its running minimum in registers and no workgroup barrier inside the walk;
+// the loads of the next virtual thread are in flight while one walks (two register buffers). The
+// group's first closed row and its open tail are combined across groups once, after one barrier,
+// in group order. Tile records and the fixup are those of spmm_tile_kernel.
+template <int K, typename P>
+__global__ void __launch_bounds__(256)
+spmm_minplus_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row_offsets,
+                    const uint32_t* __restrict__ col, const uint32_t* __restrict__ vals,
+                    const MergeCoord* __restrict__ coords, SpmmTileRec<MinPlusU32, K>* __restrict__ recs, P pol,
+                    unsigned long long* __restrict__ changed) {
+    using S = MinPlusU32;
+    using acc_t = S::acc_t;
+    using aux_t = typename P::aux_t;
+    constexpr int G = 256 / K;  // groups
+    __shared__ uint32_t s_rowend[kMergeTile + 1];
+    __shared__ uint32_t s_col[kMergeTile];
+    __shared__ uint32_t s_w[kMergeTile];
+    __shared__ aux_t s_aux[kMergeTile];
+    __shared__ uint32_t s_vi[kMergeThreads + 1];
+    __shared__ uint32_t s_vj[kMergeThreads + 1];
+    __shared__ uint32_t s_ghead[G][K];
+    __shared__ uint32_t s_gtail[G][K];
+    __shared__ uint32_t s_gold[G][K];
+    __shared__ uint32_t s_grow[G];
+    __shared__ uint32_t s_gend[G];
+    __shared__ unsigned long long s_or;
+
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const MergeCoord c0 = coords[tile];
+    const MergeCoord c1 = coords[tile + 1];
+    const uint32_t rows = c1.x - c0.x;
+    const uint32_t nzs = c1.y - c0.y;
+    const uint32_t total = rows + nzs;
+    {
+        uint32_t re[kMergeItems], ci[kMergeItems], wv[kMergeItems];
+        aux_t ax[kMergeItems];
+#pragma unroll
+        for (int r = 0; r < kMergeItems; ++r) {
+            const uint32_t k = r * 256 + tid;
+            if (k < rows) re[r] = row_offsets[c0.x + 1 + k];
+            if (k < nzs) {
+                ci[r] = col[c0.y + k];
+                wv[r] = vals ? vals[c0.y + k] : 1u;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kMergeItems; ++r) {
+            const uint32_t k = r * 256 + tid;
+            if (k < nzs) ax[r] = pol.aux(ci[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < kMergeItems; ++r) {
+            const uint32_t k = r * 256 + tid;
+            if (k < rows) s_rowend[k] = re[r];
+            if (k < nzs) {
+                s_col[k] = ci[r];
+                s_w[k] = wv[r];
+                s_aux[k] = ax[r];
+            }
+        }
+        if (tid == 0) {
+            s_rowend[rows] = kInf;
+            s_or = 0ull;
+        }
+    }
+    __syncthreads();
+    {
+        uint32_t d = tid * kMergeItems;
+        if (d > total) d = total;
+        uint32_t lo = d > nzs ? d - nzs : 0u;
+        uint32_t hi = d < rows ? d : rows;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (static_cast<uint64_t>(s_rowend[mid]) + 1 <= static_cast<uint64_t>(c0.y) + (d - mid)) lo = mid + 1;
+            else hi = mid;
+        }
+        s_vi[tid] = lo;
+        s_vj[tid] = d - lo;
+        if (tid == 0) {
+            s_vi[kMergeThreads] = rows;
+            s_vj[kMergeThreads] = nzs;
+        }
+    }
+    __syncthreads();
+
+    const uint32_t j = tid % K;
+    const uint32_t g = tid / K;
+    uint64_t counted = 0;
+
+    auto fetch = [&](uint32_t vt, uint32_t (&xs)[kMergeItems], uint32_t (&ds)[kMergeItems]) {
+        const uint32_t j0 = s_vj[vt], j1 = s_vj[vt + 1];
+        const uint32_t i0 = s_vi[vt], i1 = s_vi[vt + 1];
+#pragma unroll
+        for (int t = 0; t < kMergeItems; ++t) {
+            const bool has = j0 + t < j1;
+            const uint32_t q = has ? j0 + t : 0u;
+            const uint32_t u = has ? s_col[q] : 0u;
+            xs[t] = pol.x(u, has ? s_aux[q] : aux_t(0), j, has);
+            const bool hr = i0 + t < i1;
+            ds[t] = pol.row_old(hr ? c0.x + i0 + t : 0u, j);
+        }
+    };
+
+    uint32_t acc = kInf;       // running minimum of the open row
+    bool gend = false;         // the group closed a row
+    uint32_t grow = 0, ghead = kInf, gold = 0;
+    uint32_t i = s_vi[g * K];  // current row (tile-relative)
+    uint32_t jn = s_vj[g * K];
+    uint32_t re = s_rowend[i];
+    auto walk = [&](uint32_t vt, uint32_t (&xc)[kMergeItems], uint32_t (&dc)[kMergeItems]) {
+        uint32_t d = vt * kMergeItems;
+        if (d > total) d = total;
+        uint32_t d_end = d + kMergeItems;
+        if (d_end > total) d_end = total;
+        uint32_t p[kMergeItems];
+#pragma unroll
+        for (int t = 0; t < kMergeItems; ++t) {
+            uint32_t q = jn + t;
+            if (q >= static_cast<uint32_t>(kMergeTile)) q = kMergeTile - 1;
+            p[t] = S::combine(s_w[q], xc[t]);
+        }
+#pragma unroll
+        for (int k = 0; k < kMergeItems; ++k) {
+            if (d + k < d_end) {
+                if (c0.y + jn < re) {
+                    acc = p[0] < acc ? p[0] : acc;
+#pragma unroll
+                    for (int t = 0; t + 1 < kMergeItems; ++t) p[t] = p[t + 1];
+                    ++jn;
+                } else {
+                    if (!gend) {
+                        gend = true;
+                        grow = c0.x + i;
+                        ghead = acc;
+                        gold = dc[0];
+                    } else {
+                        counted |= pol.row_done(c0.x + i, j, acc, dc[0]);
+                    }
+#pragma unroll
+                    for (int t = 0; t + 1 < kMergeItems; ++t) dc[t] = dc[t + 1];
+                    acc = kInf;
+                    ++i;
+                    re = s_rowend[i];
+                }
+            }
+        }
+    };
+
+    uint32_t xa[kMergeItems], da[kMergeItems], xb[kMergeItems], db[kMergeItems];
+    const uint32_t v0 = g * K;
+    fetch(v0, xa, da);
+    for (uint32_t r = 0; r < static_cast<uint32_t>(K); r += 2) {
+        fetch(v0 + r + 1, xb, db);
+        walk(v0 + r, xa, da);
+        fetch(v0 + (r + 2 < static_cast<uint32_t>(K) ? r + 2 : K - 1), xa, da);
+        walk(v0 + r + 1, xb, db);
+    }
+    s_ghead[g][j] = ghead;
+    s_gtail[g][j] = acc;
+    s_gold[g][j] = gold;
+    if (j == 0) {
+        s_grow[g] = grow;
+        s_gend[g] = gend ? 1u : 0u;
+    }
+    __syncthreads();
+    // Groups in order (lanes of group 0): a group's first closed row gets the open minimum carried
+    // from the groups before it; the tile's first closed row is deferred when it began before the tile.
+    const bool first_started_before = (rows > 0) && (row_offsets[c0.x] < c0.y);
+    if (g == 0) {
+        uint32_t carry = kInf;
+        bool seen = false;
+        for (int q = 0; q < G; ++q) {
+            if (s_gend[q]) {
+                const uint32_t h = s_ghead[q][j];
+                const uint32_t whole = h < carry ? h : carry;
+                if (!seen && first_started_before) {
+                    recs[tile].head[j] = acc_t{whole};
+                } else {
+                    counted |= pol.row_done(s_grow[q], j, whole, s_gold[q][j]);
+                }
+                seen = true;
+                carry = s_gtail[q][j];
+            } else {
+                const uint32_t t = s_gtail[q][j];
+                carry = t < carry ? t : carry;
+            }
+        }
+        recs[tile].carry[j] = acc_t{carry};
+        if (j == 0) {
+            uint32_t flags = seen ? kTileHasEnd : 0u;
+            if (seen && first_started_before) flags |= kTileDeferred;
+            recs[tile].flags = flags;
+        }
+    }
+    if (changed) {
+        if (counted) atomicOr(&s_or, static_cast<unsigned long long>(counted));
+        __syncthreads();
+        if (tid == 0 && s_or) {
+            const unsigned long long cur = *reinterpret_cast<volatile unsigned long long*>(changed);
+            if (s_or & ~cur) atomicOr(changed, s_or);
+        }
+    }
+}
+
 // One group of K lanes per tile: finish the deferred first row of each tile (carries of the tiles it
 // crosses, left to right, then the tile's head), as spmv_merge_fixup_kernel does per column.
 template <typename S, int K, typename P>

@@ -160,6 +160,11 @@ typedef struct g1_graph g1_graph;
 
 g1_status g1_graph_create(g1_graph** graph, const g1_host_csr* csr, uint32_t flags);
 g1_status g1_graph_destroy(g1_graph* graph);
+/* A second handle on the same resident graph with its own work areas: runs on a view and runs on the base
+ * (or on other views) may proceed at the same time on different streams, where runs on one handle are
+ * serialized. The view shares the base's device arrays (no copy) and must be destroyed before the base.
+ * No counterpart in the reference (one graph, one engine per process). */
+g1_status g1_graph_create_view(g1_graph* base, g1_graph** view);
 g1_status g1_graph_info(const g1_graph* graph, uint32_t* n, uint32_t* m, uint64_t* device_bytes);
 g1_status g1_graph_csr_dev(const g1_graph* graph, int32_t transpose, const uint32_t** row_offsets_dev,
                            const uint32_t** col_indices_dev, const uint32_t** weights_dev);
@@ -266,16 +271,18 @@ g1_status g1_ppr_batch(g1_graph* graph, const uint32_t* seeds, uint32_t count, f
 
 /* ---- query server ------------------------------------------------------------------------------
  * No counterpart in the reference (its apps run one query per process, apps/sssp_adv.cu:74-124): a
- * persistent request loop over graphs resident in HBM. One host worker thread and one HIP stream per
- * graph, so queries on different graphs overlap on the device. Per graph, BFS runs on the single-source
+ * persistent request loop over graphs resident in HBM. Per graph, a session thread and `single_workers`
+ * single-source threads, each with its own HIP stream and its own view of the graph (g1_graph_create_view),
+ * so queries on one graph and on different graphs overlap on the device. BFS runs on the single-source
  * direction-optimizing engine; SSSP and personalized PageRank join a continuous-batching column session
  * (a multi-source SpMM whose columns are freed and refilled one query at a time while the others keep
- * iterating) when enough of them are waiting, and run on the single-source engines otherwise. Every
- * result is the one the single-source call returns (bit-identical), whichever path and whichever
- * neighbours it ran with.
- * Admission: a query is admitted when the HBM it needs (its result vector, plus a session's work
- * arrays when it opens one) fits in the budget, counting the registered graphs, open sessions and
- * results not yet fetched; otherwise it waits, in order, for memory to be freed (fetch, cancel). */
+ * iterating) when one is open or when enough of them wait (the session opens if its arrays fit the budget;
+ * SSSP only on graphs whose single-source runs took at most 64 iterations, where a full SpMM per iteration
+ * pays), and run on the single-source engines otherwise. Every result is the one the
+ * single-source call returns (bit-identical), whichever path and whichever neighbours it ran with.
+ * Admission: a query is admitted when its result vector fits in the HBM budget, counting the registered
+ * graphs, open sessions and results not yet fetched; otherwise it waits, in order, for memory to be freed
+ * (fetch, cancel, a session closing after half a second idle). */
 typedef struct g1_server g1_server;
 
 typedef struct g1_server_config {
@@ -284,7 +291,8 @@ typedef struct g1_server_config {
     uint32_t batch_min_sssp;   /* open an SSSP session when this many SSSP queries wait; 0: 8       */
     uint32_t batch_min_ppr;    /* open a PPR session when this many PPR queries wait; 0: 2           */
     float ppr_restart;         /* restart probability of every PPR query; 0: 0.15                   */
-    uint32_t reserved[3];
+    uint32_t single_workers;   /* single-source runs in flight per graph (graph views); 0: 4         */
+    uint32_t reserved[2];
 } g1_server_config;
 
 #define G1_QUERY_BFS 0
