@@ -20,6 +20,10 @@ import os
 import sys
 import time
 
+# One hardware queue per stream for the query server's concurrent streams (HIP's default is 4 per process;
+# streams sharing a queue run one after another). Read when the HIP runtime starts, so set before loading.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "24")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -286,11 +290,16 @@ def serving_benchmark(g1, args) -> dict:
         for q in qids:
             srv.cancel(q)  # discard the results
         srv.close()
+        by = {}
+        for (gi, kind, _), st_q in zip(qs, sts):
+            by.setdefault(f"{specs[gi][0]}/{kind}", []).append(st_q["t_done_ms"] - st_q["t_submit_ms"])
         r = {"rate_qps": rate, "queries": count, "completed": ok, "achieved_qps": round(ok / span, 1),
              "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
              "admission_waits": st["admission_waits"], "batched": sum(1 for s in sts if s["path"] == 1),
              "session_steps": st["session_steps"],
-             "hbm_peak_gb": round(st["hbm_peak"] / 1e9, 2)}
+             "hbm_peak_gb": round(st["hbm_peak"] / 1e9, 2),
+             "by_graph_kind": {key: {"n": len(v), "p50_ms": round(float(np.percentile(v, 50)), 2),
+                                     "p99_ms": round(float(np.percentile(v, 99)), 2)} for key, v in sorted(by.items())}}
         results.append(r)
         log(json.dumps(r))
     best = max(r["achieved_qps"] for r in results)

@@ -29,16 +29,6 @@ uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
     return static_cast<uint32_t>(b);
 }
 
-// SSSP relaxation order (measurement switch): G1_SSSP_PHASED=1 reads all of a thread's target
-// distances before relaxing any.
-uint32_t sssp_phased() {
-    static const uint32_t v = [] {
-        const char* e = getenv("G1_SSSP_PHASED");
-        return (e && e[0] == '1') ? 1u : 0u;
-    }();
-    return v;
-}
-
 uint32_t persistent_cap() { return static_cast<uint32_t>(device_cu_count()) * 8; }
 
 g1_status read_ctr(g1_graph* g, hipStream_t s, uint32_t first, uint32_t count) {
@@ -136,10 +126,21 @@ g1_status check_overflow(g1_graph* g, hipStream_t s) {
 // on the device and is a no-op once the frontier is empty, so the host enqueues iterations in batches
 // (2, 4, 8, ... up to 32) and reads back only the output counters of the last launch of a batch.
 // With profiling on, the batch is one launch so each launch's counts can be attributed to it.
-template <typename Launch, typename AfterLaunch>
-g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& after, uint32_t* launched) {
+template <typename Launch, typename AfterLaunch, typename Small>
+g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& after, Small&& small,
+                     bool small_first, uint32_t* launched) {
     uint32_t it = 0, batch = 2;
+    bool try_small = small_first;
     for (;;) {
+        // A frontier whose out-edges fit the single-workgroup kernel runs there (see push_small_kernel)
+        // until it grows past it or ends; profiling keeps one launch per iteration instead.
+        if (try_small && !g->profile) {
+            uint32_t res[2];
+            G1_TRY(small(it, res));
+            it = res[0];
+            if (res[1] == 0) break;
+            batch = 2;
+        }
         const uint32_t b = g->profile ? 1u : batch;
         G1_CHECK(it + b <= kHistIters, G1_ERR_INTERNAL, "push engine exceeded %u iterations", kHistIters);
         for (uint32_t k = 0; k < b; ++k) {
@@ -150,9 +151,28 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
         unsigned long long next = 0;
         G1_TRY(read_frontier(g, it % 3, s, &next));
         if (static_cast<uint32_t>(next) == 0) break;
+        try_small = (next >> 32) <= kSmallCap;
         if (batch < 32) batch *= 2;
     }
     *launched = it;
+    return G1_OK;
+}
+
+// Runs push_small_kernel from iteration it0 and reads its result: res[0] the next iteration, res[1] 1
+// if the frontier was handed back to the segmented queue (0: the run is over).
+template <typename P>
+g1_status run_small(g1_graph* g, hipStream_t s, const P& pol, const uint32_t* wts, uint32_t* fb, uint32_t words,
+                    uint32_t it0, uint32_t* res) {
+    uint32_t* dres = reinterpret_cast<uint32_t*>(g->ctr + 10);
+    push_small_kernel<P><<<1, kSmallThreads, 0, s>>>(g->row_off, g->col, wts, pol, g->q[0], g->q[1], g->qoff[0],
+                                                      g->qoff[1], g->qcnt, g->seg_cap, it0, kHistIters - 1, fb,
+                                                      words, g->hist, g->slots, dres);
+    G1_LAUNCHED();
+    G1_HIP(hipMemcpyAsync(g->h_ctr + 10, dres, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    G1_HIP(hipStreamSynchronize(s));
+    const uint32_t* h = reinterpret_cast<const uint32_t*>(g->h_ctr + 10);
+    res[0] = h[0];
+    res[1] = h[1];
     return G1_OK;
 }
 
@@ -232,8 +252,12 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
             prof_set_bytes(g, tok, 16.0 * nf + 8.0 * E + 4.0 * act);
         return G1_OK;
     };
+    auto small = [&](uint32_t it0, uint32_t* res) -> g1_status {
+        BfsPush pol{depth, nullptr, 0};
+        return run_small(g, s, pol, nullptr, diropt ? g->fb3 : nullptr, words, it0, res);
+    };
     uint32_t launched = 0;
-    G1_TRY(drive_push(g, s, launch, after, &launched));
+    G1_TRY(drive_push(g, s, launch, after, small, g->max_out_deg <= kSmallCap, &launched));
     G1_HIP(hipEventRecord(g->ev[1], s));
     G1_TRY(elapsed(g, st));
     G1_TRY(prof_flush(g));
@@ -270,7 +294,7 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     int tok = -1;
     auto launch = [&](uint32_t it) -> g1_status {
         PushQueues pq = make_queues(g, it);
-        SsspPush pol{g->dstamp, it + 1, sssp_phased()};
+        SsspPush pol{g->dstamp, it + 1};
         tok = prof_begin(g, "sssp_advance", s);
         push_advance_kernel<SsspPush><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq, pol, it,
                                                                              g->hist);
@@ -289,8 +313,12 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
         prof_set_bytes(g, tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E + 4.0 * static_cast<double>(g->h_ctr[3] - act0));
         return G1_OK;
     };
+    auto small = [&](uint32_t it0, uint32_t* res) -> g1_status {
+        SsspPush pol{g->dstamp, 0};
+        return run_small(g, s, pol, g->w, nullptr, 0, it0, res);
+    };
     uint32_t launched = 0;
-    G1_TRY(drive_push(g, s, launch, after, &launched));
+    G1_TRY(drive_push(g, s, launch, after, small, g->max_out_deg <= kSmallCap, &launched));
     dstamp_extract_kernel<<<fill_blocks, 256, 0, s>>>(n, g->dstamp, dist);
     G1_LAUNCHED();
     G1_HIP(hipEventRecord(g->ev[1], s));

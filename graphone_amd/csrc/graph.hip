@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "g1/push.hpp"
 #include "g1/spmv_merge.hpp"
 #include "g1/apply.hpp"
@@ -183,6 +185,8 @@ g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags
     g->n = csr->n;
     g->m = csr->m;
     g->flags = flags;
+    for (uint32_t v = 0; v < csr->n; ++v)
+        g->max_out_deg = std::max(g->max_out_deg, csr->row_offsets[v + 1] - csr->row_offsets[v]);
     auto fail = [&](g1_status s) {
         graph_free_all(g);
         delete g;
@@ -253,6 +257,7 @@ g1_status g1_graph_create_view(g1_graph* base, g1_graph** out) {
     g->n = base->n;
     g->m = base->m;
     g->flags = base->flags;
+    g->max_out_deg = base->max_out_deg;
     g->row_off = base->row_off;
     g->col = base->col;
     g->w = base->w;

@@ -14,6 +14,7 @@ struct g1_graph {
     uint32_t n = 0;
     uint32_t m = 0;
     uint32_t flags = 0;
+    uint32_t max_out_deg = 0;  // largest out-degree (set at creation)
     // out-edge CSR ("normal" view)
     uint32_t* row_off = nullptr;
     uint32_t* col = nullptr;

@@ -313,12 +313,17 @@ struct Worker {
         }
         if (!sess[k]) {
             const uint32_t threshold = k == G1_QUERY_PPR ? sv->cfg.batch_min_ppr : sv->cfg.batch_min_sssp;
-            const uint64_t bytes = g1::session_bytes(slot->g, session_kind_of(k), sv->cfg.batch_k);
             bool open = false;
+            uint32_t width = 8;
+            uint64_t bytes = 0;
             {
                 std::lock_guard<std::mutex> lock(sv->mu);
                 size_t waiting = backlog[k].size();
                 for (Query* q : slot->singles) waiting += (q->q.kind == k);
+                // a column costs a full row of every gathered vector: the width grows with the backlog (room
+                // for as many arrivals again), up to batch_k
+                while (width < sv->cfg.batch_k && width < 2 * waiting) width *= 2;
+                bytes = g1::session_bytes(slot->g, session_kind_of(k), width);
                 const bool shallow = k == G1_QUERY_PPR ||
                                      (slot->sssp_runs > 0 && slot->sssp_iters <= 64 * slot->sssp_runs);
                 if (waiting >= threshold && shallow && sv->hbm_used + bytes <= sv->cfg.hbm_budget_bytes) {
@@ -337,8 +342,8 @@ struct Worker {
             }
             if (open) {
                 g1::Session* ss = nullptr;
-                const g1_status st = g1::session_create(slot->g, session_kind_of(k), sv->cfg.batch_k,
-                                                        sv->cfg.ppr_restart, stream, &ss);
+                const g1_status st = g1::session_create(slot->g, session_kind_of(k), width, sv->cfg.ppr_restart,
+                                                        stream, &ss);
                 if (st == G1_OK) {
                     sess[k] = ss;
                     sess_bytes[k] = bytes;
@@ -490,7 +495,7 @@ g1_status g1_server_create(const g1_server_config* config, g1_server** out) {
     if (c.batch_k == 0) c.batch_k = 64;
     G1_CHECK(c.batch_k == 8 || c.batch_k == 16 || c.batch_k == 32 || c.batch_k == 64, G1_ERR_INVALID_ARG,
              "g1_server_create: batch_k must be 8, 16, 32 or 64 (got %u)", c.batch_k);
-    if (c.batch_min_sssp == 0) c.batch_min_sssp = 8;
+    if (c.batch_min_sssp == 0) c.batch_min_sssp = 32;
     if (c.batch_min_ppr == 0) c.batch_min_ppr = 2;
     if (c.ppr_restart == 0.0f) c.ppr_restart = 0.15f;
     if (c.single_workers == 0) c.single_workers = 4;

@@ -280,7 +280,6 @@ struct SsspPush {
     static constexpr bool kQueueLeaves = false;
     unsigned long long* dstamp;
     uint32_t iter;
-    uint32_t phased;  // 1: read every slot's current distance first, then relax slot by slot
     __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
         return static_cast<uint32_t>(dstamp[u] >> 32);
     }
@@ -290,22 +289,6 @@ struct SsspPush {
     template <int N>
     __device__ __forceinline__ void relax_many(const uint32_t (&)[N], const uint32_t (&du)[N], const uint32_t (&v)[N],
                                                const uint32_t (&w)[N], const uint32_t (&)[N], bool (&won)[N]) const {
-        if (phased) {
-            uint32_t cur[N];
-#pragma unroll
-            for (int r = 0; r < N; ++r) cur[r] = (v[r] != kInf) ? static_cast<uint32_t>(dstamp[v[r]] >> 32) : 0u;
-#pragma unroll
-            for (int r = 0; r < N; ++r) {
-                won[r] = false;
-                if (v[r] == kInf) continue;
-                const uint32_t nd = du[r] + w[r];
-                if (nd >= cur[r]) continue;
-                const unsigned long long old =
-                    atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | iter);
-                won[r] = nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
-            }
-            return;
-        }
 #pragma unroll
         for (int r = 0; r < N; ++r) {
             won[r] = false;
@@ -316,6 +299,20 @@ struct SsspPush {
             won[r] = nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
         }
     }
+    // One slot (the small-frontier kernel): the same relaxation.
+    __device__ __forceinline__ bool relax1(uint32_t du, uint32_t v, uint32_t w) const {
+        const uint32_t nd = du + w;
+        if (nd >= static_cast<uint32_t>(dstamp[v] >> 32)) return false;
+        const unsigned long long old = atomicMin(&dstamp[v], (static_cast<unsigned long long>(nd) << 32) | iter);
+        return nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
+    }
+    __device__ __forceinline__ SsspPush at_iteration(uint32_t it) const { return SsspPush{dstamp, it + 1}; }
+    // The source's distance read past the CU's cache: a workgroup that stays resident across iterations
+    // must see improvements made by its own atomics in earlier iterations.
+    __device__ __forceinline__ uint32_t fresh_source_value(uint32_t u) const {
+        return static_cast<uint32_t>(__hip_atomic_load(&dstamp[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
+    }
+    __device__ __forceinline__ void set_next_bitmap(uint32_t*) {}
 };
 
 // BFS top-down step: the first visitor of v sets its depth; optionally marks v in a bitmap that a
@@ -346,6 +343,14 @@ struct BfsPush {
             if (won[r] && fb_next) atomicOr(&fb_next[v[r] >> 5], 1u << (v[r] & 31u));
         }
     }
+    __device__ __forceinline__ bool relax1(uint32_t, uint32_t v, uint32_t) const {
+        if (depth[v] != kInf || atomicCAS(&depth[v], kInf, next_level) != kInf) return false;
+        if (fb_next) atomicOr(&fb_next[v >> 5], 1u << (v & 31u));
+        return true;
+    }
+    __device__ __forceinline__ BfsPush at_iteration(uint32_t it) const { return BfsPush{depth, nullptr, it + 1}; }
+    __device__ __forceinline__ void set_next_bitmap(uint32_t* b) { fb_next = b; }
+    __device__ __forceinline__ uint32_t fresh_source_value(uint32_t) const { return 0u; }
 };
 
 // BFS bottom-up step over the in-edge CSR: every unvisited vertex looks for an in-neighbour in the
@@ -488,6 +493,162 @@ static __global__ void __launch_bounds__(256) sum_slots_kernel(unsigned long lon
         unsigned long long t = 0;
         for (int i = 0; i < 256; ++i) t += s[i];
         out[0] += t;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Small frontiers: one persistent workgroup of 1024 threads runs consecutive iterations while the
+// frontier's out-edges fit in LDS (kSmallCap), with workgroup barriers between iterations instead of
+// kernel launches. A high-diameter graph (a road network or grid: thousands of levels of a few thousand
+// vertices) otherwise pays a full launch of the 2048-workgroup kernel and its chain of dependent loads
+// per level. The iterations are the advance kernel's, with the same relaxation policies (iteration
+// stamps, depths, bitmaps and history records), so the engines can switch between the two kernels at
+// any iteration.
+//
+// Input: the frontier of iteration it0 in the segmented queue (counter set it0 % 3, entries in q[it0&1];
+// the caller guarantees nf <= kSmallCap). Per iteration: degrees and their exclusive prefix (LDS), then
+// one thread per edge slot: relax, and winners append to the other LDS buffer (winners <= edges <=
+// kSmallCap). Stops when the frontier is empty (result[1] = 0) or when the next input has more than
+// kSmallCap edges or reaches it_limit: that frontier is written back to the segmented queue as the
+// input of iteration result[0] (result[1] = 1), entries without out-edges dropped, all in segment 0.
+// BFS direction-optimizing state is kept as the advance kernel keeps it in push mode: the frontier
+// bitmap of the next level (policy's fb_next), the bitmap after next cleared, hist[2*it + 1] = edges
+// explored through the level.
+constexpr int kSmallThreads = 1024;
+constexpr uint32_t kSmallCap = 8192;
+constexpr uint32_t kSmallPer = kSmallCap / kSmallThreads;
+
+template <typename P>
+static __global__ void __launch_bounds__(kSmallThreads)
+push_small_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+                  const uint32_t* __restrict__ wts, P pol, uint32_t* q0, uint32_t* q1, uint32_t* qoff0,
+                  uint32_t* qoff1, unsigned long long* qcnt, uint32_t cap, uint32_t it0, uint32_t it_limit,
+                  uint32_t* fb, uint32_t words, unsigned long long* __restrict__ hist,
+                  unsigned long long* __restrict__ stats, uint32_t* __restrict__ result) {
+    __shared__ uint32_t s_q[2][kSmallCap];
+    __shared__ uint32_t s_pre[kSmallCap + 1];
+    __shared__ uint32_t s_rb[kSmallCap];
+    __shared__ uint32_t s_vb[kSegments + 1];
+    __shared__ uint32_t s_eb[kSegments + 1];
+    __shared__ uint32_t s_n[2];
+    __shared__ uint32_t s_scan[kSmallThreads / kWave];
+    const uint32_t tid = threadIdx.x;
+
+    // input frontier of iteration it0 from the segmented queue
+    {
+        const unsigned long long* cnt = qcnt + (it0 % 3) * kSegments;
+        const uint32_t* qin = (it0 & 1) ? q1 : q0;
+        if (tid < kWave) load_segment_bases(cnt, s_vb, s_eb);
+        if (tid == 0) s_n[1] = 0;
+        __syncthreads();
+        const uint32_t nf = s_vb[kSegments] < kSmallCap ? s_vb[kSegments] : kSmallCap;
+        for (uint32_t i = tid; i < nf; i += kSmallThreads) {
+            const uint32_t sg = search_last_le(s_vb, kSegments, i);
+            s_q[0][i] = qin[static_cast<size_t>(sg) * cap + (i - s_vb[sg])];
+        }
+        if (tid == 0) s_n[0] = nf;
+        __syncthreads();
+    }
+    unsigned long long explored = 0;
+    if (it0 > 0) explored = hist[2 * (it0 - 1) + 1] & ~(1ull << 63);
+    uint32_t it = it0, cur = 0, activated = 0;
+    uint32_t status = 0;
+    for (;;) {
+        const uint32_t n = s_n[cur];
+        if (n == 0) break;
+        // degrees and their exclusive prefix; kSmallPer consecutive entries per thread
+        uint32_t rb[kSmallPer], dg[kSmallPer], local = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < kSmallPer; ++r) {
+            const uint32_t i = tid * kSmallPer + r;
+            rb[r] = 0;
+            dg[r] = 0;
+            if (i < n) {
+                const uint32_t u = s_q[cur][i];
+                rb[r] = row_off[u];
+                dg[r] = row_off[u + 1] - rb[r];
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kSmallPer; ++r) local += dg[r];
+        uint32_t E;
+        uint32_t run = block_exclusive_sum<kSmallThreads>(local, E, s_scan);
+#pragma unroll
+        for (uint32_t r = 0; r < kSmallPer; ++r) {
+            const uint32_t i = tid * kSmallPer + r;
+            if (i < n) {
+                s_pre[i] = run;
+                s_rb[i] = rb[r];
+            }
+            run += dg[r];
+        }
+        if (tid == 0) s_pre[n] = E;
+        if (E > kSmallCap || it >= it_limit) {
+            // hand the frontier of iteration `it` back to the segmented queue (segment 0)
+            __syncthreads();
+            uint32_t* qo = (it & 1) ? q1 : q0;
+            uint32_t* oo = (it & 1) ? qoff1 : qoff0;
+            uint32_t keep = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < kSmallPer; ++r) keep += (tid * kSmallPer + r < n && dg[r] > 0) ? 1u : 0u;
+            uint32_t kept;
+            uint32_t at = block_exclusive_sum<kSmallThreads>(keep, kept, s_scan);
+#pragma unroll
+            for (uint32_t r = 0; r < kSmallPer; ++r) {
+                const uint32_t i = tid * kSmallPer + r;
+                if (i < n && dg[r] > 0) {
+                    qo[at] = s_q[cur][i];
+                    oo[at] = s_pre[i];
+                    ++at;
+                }
+            }
+            unsigned long long* cin = qcnt + (it % 3) * kSegments;
+            unsigned long long* cout = qcnt + ((it + 1) % 3) * kSegments;
+            if (tid < kSegments) {
+                cin[tid] = (tid == 0) ? ((static_cast<unsigned long long>(E) << 32) | kept) : 0ull;
+                cout[tid] = 0ull;
+            }
+            status = 1;
+            break;
+        }
+        if (tid == 0) {
+            hist[2 * it] = (static_cast<unsigned long long>(E) << 32) | n;
+            hist[2 * it + 1] = explored + E;
+        }
+        explored += E;
+        P p = pol.at_iteration(it);
+        if (fb) {
+            // the advance kernel's bitmap rotation: write fb[(it+1)%3], clear fb[(it+2)%3]
+            p.set_next_bitmap(fb + static_cast<size_t>((it + 1) % 3) * words);
+            uint32_t* fc = fb + static_cast<size_t>((it + 2) % 3) * words;
+            for (uint32_t i = tid; i < words; i += kSmallThreads) fc[i] = 0u;
+        }
+        __syncthreads();  // prefixes visible; bitmap cleared before any winner sets a bit in it
+        const uint32_t nxt = cur ^ 1;
+        for (uint32_t e = tid; e < E; e += kSmallThreads) {
+            const uint32_t j = search_last_le(s_pre, n, e);
+            const uint32_t u = s_q[cur][j];
+            const uint32_t k = s_rb[j] + (e - s_pre[j]);
+            const uint32_t v = col[k];
+            const uint32_t w = P::kWeighted ? wts[k] : 1u;
+            if (p.relax1(p.fresh_source_value(u), v, w)) {
+                ++activated;
+                const uint32_t pos = atomicAdd(&s_n[nxt], 1u);
+                s_q[nxt][pos] = v;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) s_n[cur] = 0;
+        cur = nxt;
+        ++it;
+    }
+    __syncthreads();
+    uint32_t total;
+    (void)block_exclusive_sum<kSmallThreads>(activated, total, s_scan);
+    if (tid == 0) {
+        stats[0] += total;
+        result[0] = it;
+        result[1] = status;
     }
 }
 

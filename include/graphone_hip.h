@@ -287,8 +287,9 @@ typedef struct g1_server g1_server;
 
 typedef struct g1_server_config {
     uint64_t hbm_budget_bytes; /* 0: 90% of the device's free memory at creation                   */
-    uint32_t batch_k;          /* session column width (8, 16, 32, 64); 0: 64                       */
-    uint32_t batch_min_sssp;   /* open an SSSP session when this many SSSP queries wait; 0: 8       */
+    uint32_t batch_k;          /* widest session (8, 16, 32, 64); 0: 64. A session opens at the power
+                                  of two >= twice the queries waiting, capped here                   */
+    uint32_t batch_min_sssp;   /* open an SSSP session when this many SSSP queries wait; 0: 32      */
     uint32_t batch_min_ppr;    /* open a PPR session when this many PPR queries wait; 0: 2           */
     float ppr_restart;         /* restart probability of every PPR query; 0: 0.15                   */
     uint32_t single_workers;   /* single-source runs in flight per graph (graph views); 0: 4         */
