@@ -37,7 +37,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def roofline_from_profile(prof: list[dict]) -> dict | None:
+def measured_traffic(kernel: str, scale: int):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters (calibrated FETCH_SIZE +
+    WRITE_SIZE) on this workload, recorded in profiles/ (None when not measured for it)."""
+    path = os.path.join(ROOT, "profiles", f"rmat{scale}_{kernel}_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def roofline_from_profile(prof: list[dict], scale: int = 22) -> dict | None:
     if not prof:
         return None
     dom = max(prof, key=lambda e: e["total_ms"])
@@ -51,7 +63,8 @@ def roofline_from_profile(prof: list[dict]) -> dict | None:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(dom["name"], scale),
+        "traffic_source": f"profiles/rmat{scale}_{dom['name']}_pmc_traffic.md (rocprofv3 PMC, calibrated)",
         "launches": dom["launches"],
         "avg_launch_us": round(ms_per_launch * 1e3, 3),
         "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
@@ -384,7 +397,7 @@ def main() -> int:
     step()
     prof = g.read_profile()
     g.profile(False)
-    roof = roofline_from_profile(prof)
+    roof = roofline_from_profile(prof, args.scale)
 
     extras = {}
     if not args.no_extras:
