@@ -220,6 +220,10 @@ def pagerank_benchmark(g1, scale: int, seed: int, with_oracle: bool) -> dict:
     return out
 
 
+def lib_msg(g1) -> str:
+    return g1._lib.lib().g1_last_error_message().decode(errors="replace")
+
+
 def serving_benchmark(g1, args) -> dict:
     """Query serving over four resident graphs: RMAT scale 18, 20 and 23 (edge factor 16, a/b/c/d as the
     main workload, u32 weights [1, 255]) and a 2048 x 2048 grid with edge retention 0.6 (symmetric u32
@@ -306,6 +310,10 @@ def serving_benchmark(g1, args) -> dict:
         by = {}
         for (gi, kind, _), st_q in zip(qs, sts):
             by.setdefault(f"{specs[gi][0]}/{kind}", []).append(st_q["t_done_ms"] - st_q["t_submit_ms"])
+        bad = [(specs[gi][0], kind, st_q["state_name"], st_q["status"]) for (gi, kind, _), st_q in zip(qs, sts)
+               if st_q["state_name"] != "done"]
+        if bad:
+            log(f"not done: {bad[:8]} ({lib_msg(g1)})")
         r = {"rate_qps": rate, "queries": count, "completed": ok, "achieved_qps": round(ok / span, 1),
              "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
              "admission_waits": st["admission_waits"], "batched": sum(1 for s in sts if s["path"] == 1),

@@ -317,6 +317,21 @@ class Graph:
         vals = d.numpy()[: src.size * self.n].reshape(src.size, self.n) if to_host else d
         return RunResult(vals, st.as_dict())
 
+    def road_batch(self, sources, kinds=None, out: DeviceArray | None = None, stream=None, to_host: bool = True):
+        """BFS ('bfs') / SSSP ('sssp') queries on a low-degree graph, one workgroup per query: values[c]
+        equals bfs/sssp(sources[c]) bit for bit."""
+        src = self._sources(sources)
+        kk = None
+        if kinds is not None:
+            kk = np.ascontiguousarray([1 if k in ("sssp", 1) else 0 for k in kinds], dtype=np.uint32)
+        d = out if out is not None else DeviceArray(max(1, src.size * self.n), np.uint32)
+        its = np.zeros(max(1, src.size), np.uint32)
+        call("g1_road_batch", self._h, None if kk is None else kk.ctypes.data_as(C.POINTER(C.c_uint32)),
+             src.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_uint32(src.size), C.c_void_p(d.ptr),
+             its.ctypes.data_as(C.POINTER(C.c_uint32)), stream)
+        vals = d.numpy()[: src.size * self.n].reshape(src.size, self.n) if to_host else d
+        return RunResult(vals, {"iterations": its[: src.size].tolist()})
+
     def ppr_batch(self, seeds, restart: float = 0.15, max_iter: int = 50, tol: float = 1e-7, k: int = 0,
                   out: DeviceArray | None = None, stream=None, to_host: bool = True):
         """Personalized PageRank of every seed: values[c] equals ppr(seeds[c]) bit for bit."""

@@ -202,6 +202,7 @@ _SIGNATURES = {
     "g1_pagerank": (_st, [_vp, C.c_float, _u32, C.c_double, _i32, _i32, _vp, C.POINTER(RunStats), _vp]),
     "g1_ppr": (_st, [_vp, _u32, C.c_float, _u32, C.c_double, _vp, C.POINTER(RunStats), _vp]),
     "g1_batch_autotune": (_st, [_vp, _i32, _pu32, _u32, _i32, _pu32, _vp]),
+    "g1_road_batch": (_st, [_vp, _pu32, _pu32, _u32, _vp, _pu32, _vp]),
     "g1_sssp_batch": (_st, [_vp, _pu32, _u32, _u32, _i32, _vp, C.POINTER(BatchStats), _vp]),
     "g1_ppr_batch": (_st, [_vp, _pu32, _u32, C.c_float, _u32, C.c_double, _u32, _vp, C.POINTER(BatchStats), _vp]),
     "g1_server_create": (_st, [C.POINTER(ServerConfig), C.POINTER(_vp)]),

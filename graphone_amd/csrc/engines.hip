@@ -13,6 +13,7 @@
 
 #include "g1/apply.hpp"
 #include "g1/push.hpp"
+#include "g1/road.hpp"
 #include "g1/spmv_merge.hpp"
 #include "graph.hpp"
 #include "spmv_launch.hpp"
@@ -203,7 +204,16 @@ g1_status seed_push(g1_graph* g, uint32_t src, hipStream_t s) {
     return read_ctr(g, s, 0, 8);  // also syncs the host mirror with the cleared counters
 }
 
+g1_status run_bfs_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats* st, hipStream_t s, bool* ok);
+bool road_graph(const g1_graph* g);
+
 g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, g1_run_stats* st, hipStream_t s) {
+    if (road_graph(g)) {
+        bool ok = false;
+        G1_TRY(run_bfs_road(g, src, depth, st, s, &ok));
+        if (ok) return G1_OK;
+        memset(st, 0, sizeof(*st));
+    }
     G1_TRY(ensure_push_work(g));
     if (diropt) G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "g1_bfs: direction-optimizing BFS needs the transpose");
     const uint32_t n = g->n;
@@ -282,7 +292,186 @@ __global__ void dstamp_extract_kernel(uint32_t n, const unsigned long long* __re
         dist[v] = static_cast<uint32_t>(ds[v] >> 32);
 }
 
-g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s) {
+// ---------------------------------------------------------------- road-like graphs (one workgroup)
+
+// Low-degree graphs (at most kRoadMaxDeg out-edges per vertex: grids, road networks) take the
+// single-workgroup engine of g1/road.hpp; it reports an overflow of its far list (never expected) and the
+// query then runs on the push engine.
+constexpr uint32_t kRoadMaxDeg = 16;
+
+bool road_graph(const g1_graph* g) { return g->n > 0 && g->max_out_deg <= kRoadMaxDeg; }
+
+// near-far bucket width: sixteen heaviest edges (a wide bucket: few splits, few extra relaxations)
+uint32_t road_delta(const g1_graph* g) { return std::max<uint32_t>(16u * std::max<uint32_t>(g->max_w, 1u), 16u); }
+
+constexpr uint32_t kRoadSlots = 64;  // queries per batched launch
+
+g1_status ensure_road_work(g1_graph* g) {
+    G1_TRY(ensure_push_work(g));
+    if (g->road_q) return G1_OK;
+    const uint64_t cap = 2ull * g->n + 1024;
+    G1_CHECK(cap < 0xFFFFFFFFull, G1_ERR_INVALID_ARG, "graph too large for the road engine");
+    g->road_far_cap = static_cast<uint32_t>(cap);
+    for (int i = 0; i < 2; ++i) {
+        void* p = nullptr;
+        G1_TRY(graph_alloc(g, &p, cap * sizeof(uint32_t)));
+        g->road_far[i] = static_cast<uint32_t*>(p);
+    }
+    G1_TRY(graph_alloc(g, &g->road_q, kRoadSlots * sizeof(RoadQuery)));
+    void* p = nullptr;
+    G1_TRY(graph_alloc(g, &p, kRoadSlots * sizeof(uint32_t*)));
+    g->road_outs = static_cast<uint32_t**>(p);
+    return G1_OK;
+}
+
+// Runs one query on the road engine; *ok = false when it must be run elsewhere.
+template <bool kSssp>
+g1_status run_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats* st, hipStream_t s, bool* ok) {
+    G1_TRY(ensure_road_work(g));
+    RoadQuery* hq = reinterpret_cast<RoadQuery*>(g->h_ctr + 128);  // pinned staging
+    memset(hq, 0, sizeof(RoadQuery));
+    G1_CHECK(kSssp || depth != nullptr, G1_ERR_INTERNAL, "road engine: BFS without a depth array");
+    hq->sssp = kSssp ? 1u : 0u;
+    hq->dstamp = g->dstamp;
+    hq->depth = depth;
+    hq->q[0] = g->q[0];
+    hq->q[1] = g->q[1];
+    hq->far[0] = g->road_far[0];
+    hq->far[1] = g->road_far[1];
+    hq->src = src;
+    G1_HIP(hipMemcpyAsync(g->road_q, hq, sizeof(RoadQuery), hipMemcpyHostToDevice, s));
+    const uint64_t qcap = std::min<uint64_t>(static_cast<uint64_t>(g->seg_cap) * kSegments, 0xFFFFFFFFull);
+    const int tok = prof_begin(g, kSssp ? "road_sssp" : "road_bfs", s);
+    road_query_kernel<<<1, kRoadThreads, 0, s>>>(g->row_off, g->col, g->w, static_cast<RoadQuery*>(g->road_q),
+                                                        road_delta(g), g->road_far_cap, static_cast<uint32_t>(qcap),
+                                                        kHistIters);
+    G1_LAUNCHED();
+    prof_end(g, tok, s, 0.0);
+    G1_HIP(hipMemcpyAsync(hq, g->road_q, sizeof(RoadQuery), hipMemcpyDeviceToHost, s));
+    G1_HIP(hipStreamSynchronize(s));
+    *ok = hq->status == 0;
+    if (!*ok) return G1_OK;
+    prof_set_bytes(g, tok, 16.0 * static_cast<double>(hq->frontier_sum) + 12.0 * static_cast<double>(hq->edges_sum));
+    st->iterations = hq->iterations;
+    st->push_iterations = hq->iterations;
+    st->frontier_sum = hq->frontier_sum;
+    st->edges_sum = hq->edges_sum;
+    return G1_OK;
+}
+
+// Batched road-graph runs: query i (kinds[i]: 1 SSSP, 0 BFS) from sources[i] into outs[i] (n u32), one
+// workgroup per query, up to kRoadSlots per launch. Slot state (SSSP distance words, near buffers, far
+// lists) is allocated on first use and kept with the graph. ok[i] = 0 asks the caller to rerun query i
+// on the push engine.
+__global__ void road_init_kernel(uint32_t n, RoadQuery* __restrict__ qs, uint32_t count) {
+    const uint32_t i = blockIdx.y;
+    if (i >= count) return;
+    const RoadQuery& Q = qs[i];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        if (Q.sssp) Q.dstamp[v] = (v == Q.src) ? 0ull : (static_cast<unsigned long long>(kInf) << 32);
+        else Q.depth[v] = (v == Q.src) ? 0u : kInf;
+    }
+}
+
+__global__ void road_extract_kernel(uint32_t n, const RoadQuery* __restrict__ qs, uint32_t* const* __restrict__ outs,
+                                    uint32_t count) {
+    const uint32_t i = blockIdx.y;
+    if (i >= count || !qs[i].sssp) return;
+    const unsigned long long* ds = qs[i].dstamp;
+    uint32_t* out = outs[i];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) out[v] = static_cast<uint32_t>(ds[v] >> 32);
+}
+
+g1_status road_batch_run(g1_graph* g, const uint32_t* kinds, const uint32_t* sources, uint32_t count, uint32_t** outs,
+                         uint8_t* ok, uint32_t* iters, hipStream_t s) {
+    G1_TRY(ensure_road_work(g));
+    const size_t n = g->n;
+    const size_t per = n * sizeof(unsigned long long) + 2 * n * sizeof(uint32_t) +
+                       2ull * g->road_far_cap * sizeof(uint32_t);
+    while (g->road_slots.size() < std::min<uint32_t>(count, kRoadSlots)) {
+        void* p = nullptr;
+        G1_TRY(graph_alloc(g, &p, per));
+        g->road_slots.push_back(p);
+    }
+    const uint64_t qcap = n;
+    for (uint32_t b0 = 0; b0 < count; b0 += kRoadSlots) {
+        const uint32_t c = std::min<uint32_t>(kRoadSlots, count - b0);
+        std::vector<RoadQuery> hq(c);
+        std::vector<uint32_t*> hout(c);
+        for (uint32_t i = 0; i < c; ++i) {
+            uint8_t* base = static_cast<uint8_t*>(g->road_slots[i]);
+            RoadQuery& Q = hq[i];
+            memset(&Q, 0, sizeof(Q));
+            Q.sssp = kinds[b0 + i] ? 1u : 0u;
+            Q.dstamp = reinterpret_cast<unsigned long long*>(base);
+            uint32_t* u32 = reinterpret_cast<uint32_t*>(base + n * sizeof(unsigned long long));
+            Q.q[0] = u32;
+            Q.q[1] = u32 + n;
+            Q.far[0] = u32 + 2 * n;
+            Q.far[1] = u32 + 2 * n + g->road_far_cap;
+            Q.depth = outs[b0 + i];
+            Q.src = sources[b0 + i];
+            hout[i] = outs[b0 + i];
+        }
+        RoadQuery* dq = static_cast<RoadQuery*>(g->road_q);
+        G1_HIP(hipMemcpyAsync(dq, hq.data(), c * sizeof(RoadQuery), hipMemcpyHostToDevice, s));
+        uint32_t** douts = g->road_outs;
+        G1_HIP(hipMemcpyAsync(douts, hout.data(), c * sizeof(uint32_t*), hipMemcpyHostToDevice, s));
+        road_init_kernel<<<dim3(std::min<uint32_t>(grid_for(g->n, 256, 1024), 1024), c), 256, 0, s>>>(g->n, dq, c);
+        G1_LAUNCHED();
+        road_query_kernel<<<c, kRoadThreads, 0, s>>>(g->row_off, g->col, g->w, dq, road_delta(g), g->road_far_cap,
+                                                     static_cast<uint32_t>(qcap), kHistIters);
+        G1_LAUNCHED();
+        road_extract_kernel<<<dim3(std::min<uint32_t>(grid_for(g->n, 256, 1024), 1024), c), 256, 0, s>>>(g->n, dq, douts, c);
+        G1_LAUNCHED();
+        G1_HIP(hipMemcpyAsync(hq.data(), dq, c * sizeof(RoadQuery), hipMemcpyDeviceToHost, s));
+        G1_HIP(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < c; ++i) {
+            ok[b0 + i] = hq[i].status == 0 ? 1 : 0;
+            if (iters) iters[b0 + i] = hq[i].iterations;
+        }
+    }
+    return G1_OK;
+}
+
+g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s,
+                        bool allow_road = true);
+
+g1_status run_sssp_road(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s) {
+    G1_TRY(ensure_road_work(g));
+    const uint32_t n = g->n;
+    dstamp_init_kernel<<<grid_for(n, 256, kMaxPushBlocks), 256, 0, s>>>(n, src, g->dstamp);
+    G1_LAUNCHED();
+    G1_HIP(hipEventRecord(g->ev[0], s));
+    bool ok = false;
+    G1_TRY(run_road<true>(g, src, nullptr, st, s, &ok));
+    if (!ok) return run_sssp_push(g, src, dist, st, s, false);
+    G1_HIP(hipEventRecord(g->ev[1], s));
+    G1_TRY(elapsed(g, st));
+    G1_TRY(prof_flush(g));
+    dstamp_extract_kernel<<<grid_for(n, 256, kMaxPushBlocks), 256, 0, s>>>(n, g->dstamp, dist);
+    G1_LAUNCHED();
+    return finish_stats(g, dist, st, s);
+}
+
+g1_status run_bfs_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats* st, hipStream_t s, bool* ok) {
+    G1_TRY(ensure_road_work(g));
+    G1_TRY(g1_fill_u32(depth, kInf, g->n, s));
+    set_u32_kernel<<<1, 64, 0, s>>>(depth, src, 0u);
+    G1_LAUNCHED();
+    G1_HIP(hipEventRecord(g->ev[0], s));
+    G1_TRY(run_road<false>(g, src, depth, st, s, ok));
+    if (!*ok) return G1_OK;
+    G1_HIP(hipEventRecord(g->ev[1], s));
+    G1_TRY(elapsed(g, st));
+    G1_TRY(prof_flush(g));
+    return finish_stats(g, depth, st, s);
+}
+
+g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s, bool allow_road) {
+    if (allow_road && road_graph(g) && g->w) return run_sssp_road(g, src, dist, st, s);
     G1_TRY(ensure_push_work(g));
     G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
     const uint32_t n = g->n;
@@ -492,6 +681,15 @@ g1_status check_run(g1_graph* g, uint32_t src, const void* out, g1_run_stats* st
 }
 
 }  // namespace
+
+bool is_road_graph(const g1_graph* g) { return road_graph(g); }
+
+g1_status road_queries(g1_graph* g, const uint32_t* kinds, const uint32_t* sources, uint32_t count, uint32_t** outs,
+                       uint8_t* ok, uint32_t* iters, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g->mu);
+    return road_batch_run(g, kinds, sources, count, outs, ok, iters, s);
+}
+
 }  // namespace g1
 
 extern "C" {
@@ -527,6 +725,41 @@ g1_status g1_sssp(g1_graph* g, uint32_t src, uint32_t* dist, int32_t engine, g1_
         case G1_ENGINE_BLAS_CSRVEC: return run_minplus_pull(g, src, dist, true, engine, stats, s);
         default: return set_error(G1_ERR_INVALID_ARG, "g1_sssp: unknown engine %d", engine);
     }
+}
+
+g1_status g1_road_batch(g1_graph* g, const uint32_t* kinds, const uint32_t* sources, uint32_t count, uint32_t* out,
+                        uint32_t* iterations, g1_stream_t stream) {
+    using namespace g1;
+    G1_CHECK(g != nullptr && (count == 0 || (sources != nullptr && out != nullptr)), G1_ERR_INVALID_ARG,
+             "g1_road_batch: null argument");
+    G1_CHECK(road_graph(g), G1_ERR_INVALID_ARG,
+             "g1_road_batch: the graph has a vertex of out-degree %u (the road engine takes at most %u)",
+             g->max_out_deg, kRoadMaxDeg);
+    for (uint32_t i = 0; i < count; ++i) {
+        G1_CHECK(sources[i] < g->n, G1_ERR_INVALID_ARG, "g1_road_batch: source %u out of range", sources[i]);
+        G1_CHECK(!(kinds ? kinds[i] : 1u) || g->w != nullptr, G1_ERR_INVALID_ARG, "g1_road_batch: SSSP without weights");
+    }
+    std::lock_guard<std::mutex> lock(g->mu);
+    const hipStream_t s = as_stream(stream);
+    std::vector<uint32_t> k(count, 1u);
+    if (kinds)
+        for (uint32_t i = 0; i < count; ++i) k[i] = kinds[i] ? 1u : 0u;
+    std::vector<uint32_t*> outs(count);
+    for (uint32_t i = 0; i < count; ++i) outs[i] = out + static_cast<size_t>(i) * g->n;
+    std::vector<uint8_t> ok(count, 0);
+    std::vector<uint32_t> it(count, 0);
+    G1_TRY(road_batch_run(g, k.data(), sources, count, outs.data(), ok.data(), it.data(), s));
+    for (uint32_t i = 0; i < count; ++i) {
+        if (ok[i]) continue;  // a far-list overflow: this query runs on the push engine
+        g1_run_stats st;
+        memset(&st, 0, sizeof(st));
+        if (k[i]) G1_TRY(run_sssp_push(g, sources[i], outs[i], &st, s, false));
+        else G1_TRY(run_bfs_push(g, sources[i], outs[i], false, &st, s));
+        it[i] = st.iterations;
+    }
+    if (iterations)
+        for (uint32_t i = 0; i < count; ++i) iterations[i] = it[i];
+    return G1_OK;
 }
 
 g1_status g1_pagerank(g1_graph* g, float alpha, uint32_t max_iter, double tol, int32_t converge_mode,

@@ -187,6 +187,8 @@ g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags
     g->flags = flags;
     for (uint32_t v = 0; v < csr->n; ++v)
         g->max_out_deg = std::max(g->max_out_deg, csr->row_offsets[v + 1] - csr->row_offsets[v]);
+    if (csr->weights)
+        for (uint32_t e = 0; e < csr->m; ++e) g->max_w = std::max(g->max_w, csr->weights[e]);
     auto fail = [&](g1_status s) {
         graph_free_all(g);
         delete g;
@@ -258,6 +260,7 @@ g1_status g1_graph_create_view(g1_graph* base, g1_graph** out) {
     g->m = base->m;
     g->flags = base->flags;
     g->max_out_deg = base->max_out_deg;
+    g->max_w = base->max_w;
     g->row_off = base->row_off;
     g->col = base->col;
     g->w = base->w;

@@ -15,6 +15,7 @@ struct g1_graph {
     uint32_t m = 0;
     uint32_t flags = 0;
     uint32_t max_out_deg = 0;  // largest out-degree (set at creation)
+    uint32_t max_w = 0;        // largest edge weight (set at creation; 0 without weights)
     // out-edge CSR ("normal" view)
     uint32_t* row_off = nullptr;
     uint32_t* col = nullptr;
@@ -41,6 +42,12 @@ struct g1_graph {
     uint32_t* fb3 = nullptr;                    // three rotating frontier bitmaps (device-driven BFS)
     unsigned long long* dstamp = nullptr;       // SSSP (dist << 32 | stamp) words
     unsigned long long* hist = nullptr;         // per-iteration history of the push engines
+    uint32_t* road_far[2] = {nullptr, nullptr};  // far lists of the road-graph kernel (road_far_cap each)
+    uint32_t road_far_cap = 0;
+    void* road_q = nullptr;                     // RoadQuery records (kRoadSlots)
+    uint32_t** road_outs = nullptr;             // per-record output vectors (kRoadSlots)
+    // per-query state of batched road-graph runs (slot i: dstamp, 2 near buffers, 2 far lists)
+    std::vector<void*> road_slots;
     unsigned long long* ctr = nullptr;  // [0..2] frontier counters, [3] activated, [4..5] reached stats, [6] pull count
     // pull work areas
     uint32_t* xu = nullptr;
@@ -92,6 +99,13 @@ g1_status ensure_rank_partials(g1_graph* g);
 void batch_release(g1_graph* g);  // captured graphs and host records of the batched engines
 
 g1_status graph_alloc(g1_graph* g, void** p, size_t bytes);
+
+// Low-degree graphs (grids, road networks) run BFS / SSSP on the single-workgroup road engine;
+// road_queries runs count such queries (kinds[i]: 1 SSSP, 0 BFS) together, ok[i] = 0 when query i must be
+// run again on the push engine.
+bool is_road_graph(const g1_graph* g);
+g1_status road_queries(g1_graph* g, const uint32_t* kinds, const uint32_t* sources, uint32_t count, uint32_t** outs,
+                       uint8_t* ok, uint32_t* iters, hipStream_t s);
 g1_status ensure_push_work(g1_graph* g);
 g1_status ensure_pull_work(g1_graph* g, bool pagerank);
 

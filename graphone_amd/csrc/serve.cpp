@@ -156,6 +156,9 @@ void finish(g1_server* sv, Query* q, g1_status status, uint32_t iterations, bool
     sv->cv_done.notify_all();
 }
 
+// Road-graph queries one single-source thread runs together in one launch.
+constexpr size_t kRoadBatch = 32;
+
 // A single-source thread: takes routed queries of its graph and runs them on its own view and stream.
 void single_loop(g1_server* sv, GraphSlot* slot, g1_graph* view) {
     hipStream_t stream = nullptr;
@@ -163,6 +166,7 @@ void single_loop(g1_server* sv, GraphSlot* slot, g1_graph* view) {
     const g1_stream_t s = reinterpret_cast<g1_stream_t>(stream);
     for (;;) {
         Query* q = nullptr;
+        std::vector<Query*> road;  // BFS / SSSP on a low-degree graph: several run in one launch
         {
             std::unique_lock<std::mutex> lock(sv->mu);
             sv->cv_work.wait(lock, [&] { return sv->stop || !slot->singles.empty(); });
@@ -174,6 +178,44 @@ void single_loop(g1_server* sv, GraphSlot* slot, g1_graph* view) {
                 finish(sv, q, G1_OK, 0);
                 continue;
             }
+            if (g1::is_road_graph(view) && q->q.kind != G1_QUERY_PPR) {
+                road.push_back(q);
+                for (auto it = slot->singles.begin(); it != slot->singles.end() && road.size() < kRoadBatch;) {
+                    if ((*it)->q.kind != G1_QUERY_PPR && !(*it)->cancel) {
+                        road.push_back(*it);
+                        it = slot->singles.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+            }
+        }
+        if (!road.empty()) {
+            const uint32_t c = static_cast<uint32_t>(road.size());
+            std::vector<uint32_t> kinds(c), srcs(c), iters(c, 0);
+            std::vector<uint32_t*> outs(c);
+            std::vector<uint8_t> ok(c, 0);
+            for (uint32_t i = 0; i < c; ++i) {
+                kinds[i] = road[i]->q.kind == G1_QUERY_SSSP ? 1u : 0u;
+                srcs[i] = road[i]->q.source;
+                outs[i] = static_cast<uint32_t*>(road[i]->result);
+            }
+            g1_status st = g1::road_queries(view, kinds.data(), srcs.data(), c, outs.data(), ok.data(), iters.data(),
+                                            stream);
+            for (uint32_t i = 0; i < c; ++i) {
+                g1_status qs = st;
+                uint32_t its = iters[i];
+                if (qs == G1_OK && !ok[i]) {  // rerun on the push engine
+                    g1_run_stats rs;
+                    memset(&rs, 0, sizeof(rs));
+                    qs = kinds[i] ? g1_sssp(view, srcs[i], outs[i], G1_ENGINE_ADVANCE, &rs, s)
+                                  : g1_bfs(view, srcs[i], outs[i], G1_ENGINE_ADVANCE, &rs, s);
+                    if (qs == G1_OK && hipStreamSynchronize(stream) != hipSuccess) qs = G1_ERR_HIP;
+                    its = rs.iterations;
+                }
+                finish(sv, road[i], qs, its);
+            }
+            continue;
         }
         g1_run_stats rs;
         memset(&rs, 0, sizeof(rs));

@@ -235,6 +235,15 @@ g1_status g1_pagerank(g1_graph* graph, float alpha, uint32_t max_iter, double to
 g1_status g1_ppr(g1_graph* graph, uint32_t seed_vertex, float restart, uint32_t max_iter, double tol,
                  float* rank_dev, g1_run_stats* stats, g1_stream_t stream);
 
+/* Many BFS (kinds[i] = 0) / SSSP (kinds[i] = 1; kinds NULL: all SSSP) queries on a low-degree graph (every
+ * out-degree <= 16: grids, road networks), one workgroup per query, 64 per launch: each runs to the end
+ * inside its workgroup (SSSP near-far, bucket width 16x the heaviest edge), which suits graphs of
+ * thousands of levels. out_dev holds count x n u32, source-major; iterations (host, may be NULL) gets
+ * each query's iteration count. Results equal g1_bfs / g1_sssp bit for bit. No counterpart in the
+ * reference. */
+g1_status g1_road_batch(g1_graph* graph, const uint32_t* kinds, const uint32_t* sources, uint32_t count,
+                        uint32_t* out_dev, uint32_t* iterations, g1_stream_t stream);
+
 /* ---- batched multi-source engines ------------------------------------------------------------
  * No single counterpart in the reference: they run many single-source queries of the reference's apps
  * (apps/sssp_blas.cu:60-80 for min-plus, apps/pagerank_blas.cu:93-123 for the PageRank iteration)

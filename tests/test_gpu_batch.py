@@ -122,3 +122,23 @@ def test_batch_edge_cases(graphs):
         g.sssp_batch([0, 1], k=12)
     with pytest.raises(g1.G1Error):
         g.ppr_batch([0], restart=0.0, k=8)
+
+
+def test_road_batch_matches_oracle(graphs):
+    """One workgroup per query on a low-degree graph (the grid): mixed BFS / SSSP queries, bit-exact
+    against the oracle, including more queries than one launch takes."""
+    import graphone_amd as g1
+    name, h, g = graphs[1]
+    assert name == "grid48"
+    rng = np.random.default_rng(5)
+    srcs = [int(s) for s in rng.integers(0, h.n, size=70)]
+    kinds = ["sssp" if i % 3 else "bfs" for i in range(len(srcs))]
+    got = g.road_batch(srcs, kinds)
+    for c, (s, k) in enumerate(zip(srcs, kinds)):
+        if k == "sssp":
+            want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, s)
+        else:
+            want = oracle.bfs_queue(h.row_offsets, h.col_indices, s)
+        assert np.array_equal(got.values[c], want), (c, s, k)
+    with pytest.raises(g1.G1Error):
+        graphs[0][2].road_batch([0])  # RMAT: hubs far above the road engine's degree bound
