@@ -48,7 +48,7 @@ __global__ void clear_u64_kernel(unsigned long long* p, uint32_t count) {
 }
 
 g1_status finish_stats(g1_graph* g, const uint32_t* d, g1_run_stats* st, hipStream_t s) {
-    G1_TRY(ensure_push_work(g));  // the stats slots live with the push work areas
+    G1_TRY(ensure_push_work(g, s));  // the stats slots live with the push work areas
     clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr + 4, 2);
     G1_LAUNCHED();
     reached_stats_kernel<<<grid_for(g->n, 256, kMaxPushBlocks), 256, 0, s>>>(g->n, d, g->row_off, g->slots,
@@ -119,7 +119,10 @@ g1_status check_overflow(g1_graph* g, hipStream_t s) {
     G1_HIP(hipMemcpyAsync(g->h_ctr + 8, g->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     G1_HIP(hipStreamSynchronize(s));
     memcpy(&flag, g->h_ctr + 8, sizeof(flag));
-    G1_CHECK(flag == 0, G1_ERR_INTERNAL, "frontier queue segment overflow");
+    if (flag != 0) {  // cleared, so the next run on this graph starts clean
+        G1_TRY(g1_fill_u32(g->overflow, 0u, 1, reinterpret_cast<g1_stream_t>(s)));
+        return set_error(G1_ERR_INTERNAL, "frontier queue segment overflow");
+    }
     return G1_OK;
 }
 
@@ -214,7 +217,7 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
         if (ok) return G1_OK;
         memset(st, 0, sizeof(*st));
     }
-    G1_TRY(ensure_push_work(g));
+    G1_TRY(ensure_push_work(g, s));
     if (diropt) G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "g1_bfs: direction-optimizing BFS needs the transpose");
     const uint32_t n = g->n;
     const uint32_t words = (n + 31) / 32;
@@ -306,8 +309,8 @@ uint32_t road_delta(const g1_graph* g) { return std::max<uint32_t>(16u * std::ma
 
 constexpr uint32_t kRoadSlots = 64;  // queries per batched launch
 
-g1_status ensure_road_work(g1_graph* g) {
-    G1_TRY(ensure_push_work(g));
+g1_status ensure_road_work(g1_graph* g, hipStream_t s) {
+    G1_TRY(ensure_push_work(g, s));
     if (g->road_q) return G1_OK;
     const uint64_t cap = 2ull * g->n + 1024;
     G1_CHECK(cap < 0xFFFFFFFFull, G1_ERR_INVALID_ARG, "graph too large for the road engine");
@@ -327,7 +330,7 @@ g1_status ensure_road_work(g1_graph* g) {
 // Runs one query on the road engine; *ok = false when it must be run elsewhere.
 template <bool kSssp>
 g1_status run_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats* st, hipStream_t s, bool* ok) {
-    G1_TRY(ensure_road_work(g));
+    G1_TRY(ensure_road_work(g, s));
     RoadQuery* hq = reinterpret_cast<RoadQuery*>(g->h_ctr + 128);  // pinned staging
     memset(hq, 0, sizeof(RoadQuery));
     G1_CHECK(kSssp || depth != nullptr, G1_ERR_INTERNAL, "road engine: BFS without a depth array");
@@ -386,7 +389,7 @@ __global__ void road_extract_kernel(uint32_t n, const RoadQuery* __restrict__ qs
 
 g1_status road_batch_run(g1_graph* g, const uint32_t* kinds, const uint32_t* sources, uint32_t count, uint32_t** outs,
                          uint8_t* ok, uint32_t* iters, hipStream_t s) {
-    G1_TRY(ensure_road_work(g));
+    G1_TRY(ensure_road_work(g, s));
     const size_t n = g->n;
     const size_t per = n * sizeof(unsigned long long) + 2 * n * sizeof(uint32_t) +
                        2ull * g->road_far_cap * sizeof(uint32_t);
@@ -440,7 +443,7 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
                         bool allow_road = true);
 
 g1_status run_sssp_road(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s) {
-    G1_TRY(ensure_road_work(g));
+    G1_TRY(ensure_road_work(g, s));
     const uint32_t n = g->n;
     dstamp_init_kernel<<<grid_for(n, 256, kMaxPushBlocks), 256, 0, s>>>(n, src, g->dstamp);
     G1_LAUNCHED();
@@ -457,7 +460,7 @@ g1_status run_sssp_road(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
 }
 
 g1_status run_bfs_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats* st, hipStream_t s, bool* ok) {
-    G1_TRY(ensure_road_work(g));
+    G1_TRY(ensure_road_work(g, s));
     G1_TRY(g1_fill_u32(depth, kInf, g->n, s));
     set_u32_kernel<<<1, 64, 0, s>>>(depth, src, 0u);
     G1_LAUNCHED();
@@ -472,7 +475,7 @@ g1_status run_bfs_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats*
 
 g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s, bool allow_road) {
     if (allow_road && road_graph(g) && g->w) return run_sssp_road(g, src, dist, st, s);
-    G1_TRY(ensure_push_work(g));
+    G1_TRY(ensure_push_work(g, s));
     G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
     const uint32_t n = g->n;
     const uint32_t fill_blocks = grid_for(n, 256, kMaxPushBlocks);
@@ -527,7 +530,7 @@ g1_status run_minplus_pull(g1_graph* g, uint32_t src, uint32_t* d, bool weighted
     G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "pull engine needs the graph's transpose");
     if (weighted) G1_CHECK(g->t_w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
     G1_TRY(ensure_pull_work(g, false));
-    G1_TRY(ensure_push_work(g));  // stats slots
+    G1_TRY(ensure_push_work(g, s));  // stats slots
     const uint32_t n = g->n;
     const uint32_t cap = std::min<uint32_t>(persistent_cap(), kMaxPushBlocks);
     minplus_seed_kernel<<<grid_for(n, 256, cap), 256, 0, s>>>(n, src, d, g->xu);

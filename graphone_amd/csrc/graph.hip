@@ -32,7 +32,7 @@ static g1_status alloc_n(g1_graph* g, T** p, size_t count) {
     return G1_OK;
 }
 
-g1_status ensure_push_work(g1_graph* g) {
+g1_status ensure_push_work(g1_graph* g, hipStream_t s) {
     if (g->q[0]) return G1_OK;
     const size_t n = g->n ? g->n : 1;
     const size_t words = (n + 31) / 32 + 1;
@@ -52,8 +52,11 @@ g1_status ensure_push_work(g1_graph* g) {
     G1_TRY(alloc_n(g, &g->qcnt, 3 * kSegments));
     G1_TRY(alloc_n(g, &g->slots, 2 * kMaxPushBlocks));
     G1_TRY(alloc_n(g, &g->overflow, 4));
-    G1_HIP(hipMemset(g->slots, 0, sizeof(unsigned long long) * 2 * kMaxPushBlocks));
-    G1_HIP(hipMemset(g->overflow, 0, 16));
+    // Cleared on the caller's stream, ahead of its first launch: a memset on the null stream is not
+    // ordered before work on a non-blocking stream, and the allocation may hold an earlier buffer's data.
+    const g1_stream_t gs = reinterpret_cast<g1_stream_t>(s);
+    G1_TRY(g1_fill_u32(reinterpret_cast<uint32_t*>(g->slots), 0u, 4ull * kMaxPushBlocks, gs));
+    G1_TRY(g1_fill_u32(g->overflow, 0u, 4, gs));
     return G1_OK;
 }
 
@@ -196,16 +199,22 @@ g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags
     };
     g1_status s;
     const size_t n1 = static_cast<size_t>(csr->n) + 1;
+    // Column and weight arrays carry kCsrPad zero entries past m: kernels that read a fixed run of
+    // entries past a tile's end (masking the extra ones) need no bounds branch.
+    const std::vector<uint32_t> zeros(kCsrPad, 0u);
+    auto upload_padded = [&](uint32_t** dst, const uint32_t* src) -> g1_status {
+        G1_TRY(alloc_n(g, dst, static_cast<size_t>(csr->m) + kCsrPad));
+        G1_CHECK(!csr->m || hipMemcpy(*dst, src, size_t(csr->m) * 4, hipMemcpyHostToDevice) == hipSuccess, G1_ERR_HIP,
+                 "g1_graph_create: upload failed");
+        G1_CHECK(hipMemcpy(*dst + csr->m, zeros.data(), kCsrPad * 4, hipMemcpyHostToDevice) == hipSuccess, G1_ERR_HIP,
+                 "g1_graph_create: upload failed");
+        return G1_OK;
+    };
     if ((s = alloc_n(g, &g->row_off, n1)) != G1_OK) return fail(s);
-    if ((s = alloc_n(g, &g->col, csr->m ? csr->m : 1)) != G1_OK) return fail(s);
-    if (hipMemcpy(g->row_off, csr->row_offsets, n1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        (csr->m && hipMemcpy(g->col, csr->col_indices, size_t(csr->m) * 4, hipMemcpyHostToDevice) != hipSuccess))
+    if (hipMemcpy(g->row_off, csr->row_offsets, n1 * 4, hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_error(G1_ERR_HIP, "g1_graph_create: upload failed"));
-    if (csr->weights) {
-        if ((s = alloc_n(g, &g->w, csr->m ? csr->m : 1)) != G1_OK) return fail(s);
-        if (csr->m && hipMemcpy(g->w, csr->weights, size_t(csr->m) * 4, hipMemcpyHostToDevice) != hipSuccess)
-            return fail(set_error(G1_ERR_HIP, "g1_graph_create: weight upload failed"));
-    }
+    if ((s = upload_padded(&g->col, csr->col_indices)) != G1_OK) return fail(s);
+    if (csr->weights && (s = upload_padded(&g->w, csr->weights)) != G1_OK) return fail(s);
     if (flags & G1_GRAPH_SYMMETRIC) {
         g->has_t = true;
         g->t_alias = true;
@@ -217,15 +226,10 @@ g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags
         if ((s = g1_host_csr_transpose(csr, &t)) != G1_OK) return fail(s);
         g->has_t = true;
         s = alloc_n(g, &g->t_row_off, n1);
-        if (s == G1_OK) s = alloc_n(g, &g->t_col, csr->m ? csr->m : 1);
-        if (s == G1_OK && t.weights) s = alloc_n(g, &g->t_w, csr->m ? csr->m : 1);
-        if (s == G1_OK) {
-            if (hipMemcpy(g->t_row_off, t.row_offsets, n1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
-                (csr->m && hipMemcpy(g->t_col, t.col_indices, size_t(csr->m) * 4, hipMemcpyHostToDevice) != hipSuccess) ||
-                (t.weights && csr->m &&
-                 hipMemcpy(g->t_w, t.weights, size_t(csr->m) * 4, hipMemcpyHostToDevice) != hipSuccess))
-                s = set_error(G1_ERR_HIP, "g1_graph_create: transpose upload failed");
-        }
+        if (s == G1_OK && hipMemcpy(g->t_row_off, t.row_offsets, n1 * 4, hipMemcpyHostToDevice) != hipSuccess)
+            s = set_error(G1_ERR_HIP, "g1_graph_create: transpose upload failed");
+        if (s == G1_OK) s = upload_padded(&g->t_col, t.col_indices);
+        if (s == G1_OK && t.weights) s = upload_padded(&g->t_w, t.weights);
         g1_host_csr_free(&t);
         if (s != G1_OK) return fail(s);
     }

@@ -94,6 +94,7 @@ namespace g1 {
 constexpr uint32_t kPartials = 4096;     // max blocks of the apply kernels (and partial slots)
 constexpr uint32_t kHistIters = 1u << 17;  // iterations a push run may take
 constexpr uint32_t kBatchCols = 64;        // widest column tile of the batched engines
+constexpr uint32_t kCsrPad = 64;           // zero entries past m in the column and weight arrays
 inline uint32_t rank_blocks(uint32_t n) { return (n + 255) / 256; }
 g1_status ensure_rank_partials(g1_graph* g);
 void batch_release(g1_graph* g);  // captured graphs and host records of the batched engines
@@ -106,7 +107,7 @@ g1_status graph_alloc(g1_graph* g, void** p, size_t bytes);
 bool is_road_graph(const g1_graph* g);
 g1_status road_queries(g1_graph* g, const uint32_t* kinds, const uint32_t* sources, uint32_t count, uint32_t** outs,
                        uint8_t* ok, uint32_t* iters, hipStream_t s);
-g1_status ensure_push_work(g1_graph* g);
+g1_status ensure_push_work(g1_graph* g, hipStream_t s);  // work areas cleared on stream s
 g1_status ensure_pull_work(g1_graph* g, bool pagerank);
 
 // Profiling: prof_begin records a start event (returns -1 when profiling is off), prof_end the stop

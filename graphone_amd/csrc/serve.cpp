@@ -18,6 +18,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -42,6 +43,7 @@ struct Query {
     void* result = nullptr;    // device buffer, owned by the server until fetched
     bool cancel = false;       // cancel requested while running
     bool reserved = false;     // its bytes are counted in hbm_used
+    std::string error;         // the engine's error message when the query failed
 };
 
 struct GraphSlot {
@@ -127,8 +129,10 @@ int session_kind_of(int32_t qkind) { return qkind == G1_QUERY_PPR ? g1::kSessPpr
 
 // ---- worker side ---------------------------------------------------------------------------------
 
-// Marks q final under the server lock and releases what it holds when it did not complete.
+// Marks q final under the server lock and releases what it holds when it did not complete. Called on
+// the thread whose engine call returned `status`, so a failure's message is that thread's last error.
 void finish(g1_server* sv, Query* q, g1_status status, uint32_t iterations, bool cancel = false) {
+    const std::string message = status != G1_OK ? std::string(g1_last_error_message()) : std::string();
     std::lock_guard<std::mutex> lock(sv->mu);
     if (cancel) q->cancel = true;
     q->iterations = iterations;
@@ -139,6 +143,7 @@ void finish(g1_server* sv, Query* q, g1_status status, uint32_t iterations, bool
     } else if (status != G1_OK) {
         q->state = G1_Q_FAILED;
         q->status = status;
+        q->error = message;
         ++sv->st.failed;
     } else {
         q->state = G1_Q_DONE;
@@ -686,6 +691,8 @@ g1_status g1_server_fetch(g1_server* sv, uint64_t query_id, void* dst, uint64_t 
         Query* q = find(sv, query_id);
         G1_CHECK(q != nullptr, G1_ERR_NOT_FOUND, "g1_server_fetch: unknown query %llu",
                  static_cast<unsigned long long>(query_id));
+        G1_CHECK(q->state != G1_Q_FAILED, q->status, "g1_server_fetch: query %llu failed: %s",
+                 static_cast<unsigned long long>(query_id), q->error.c_str());
         G1_CHECK(q->state == G1_Q_DONE, q->state == G1_Q_FETCHED ? G1_ERR_INVALID_ARG : G1_ERR_BUSY,
                  "g1_server_fetch: query %llu is not done (state %d)", static_cast<unsigned long long>(query_id),
                  q->state);

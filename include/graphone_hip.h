@@ -356,7 +356,8 @@ g1_status g1_server_submit(g1_server* server, const g1_query* query, uint64_t* q
 g1_status g1_server_poll(g1_server* server, uint64_t query_id, g1_query_status* status);
 /* Waits until the query is done, failed or cancelled, or timeout_ms passes (G1_ERR_BUSY then). */
 g1_status g1_server_wait(g1_server* server, uint64_t query_id, double timeout_ms, g1_query_status* status);
-/* Copies a done query's result (n values) to dst and frees it on the device. */
+/* Copies a done query's result (n values) to dst and frees it on the device. For a failed query it
+ * returns the engine's status, and g1_last_error_message() names the engine's error. */
 g1_status g1_server_fetch(g1_server* server, uint64_t query_id, void* dst, uint64_t dst_bytes, int32_t dst_on_device);
 g1_status g1_server_cancel(g1_server* server, uint64_t query_id);
 g1_status g1_server_get_stats(g1_server* server, g1_server_stats* stats);

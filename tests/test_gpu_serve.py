@@ -139,3 +139,24 @@ def test_cancel_and_errors(graphs):
     with pytest.raises(g1.G1Error):
         srv.submit(7, "bfs", 0)
     srv.close()
+
+
+def test_fresh_server_on_recycled_memory(graphs):
+    """Device memory that held earlier data (here all ones) is handed to the next server's graph views:
+    their work areas must be cleared on the stream that uses them, so every single-source thread's first
+    query is right. Regression test for flags cleared on the null stream, which is not ordered before a
+    non-blocking stream's kernels."""
+    import graphone_amd as g1
+    junk = g1.DeviceArray(64 << 20, np.uint32)
+    junk.fill(0xFFFFFFFF)
+    g1.synchronize()
+    junk.free()
+    srv = g1.Server(batch_k=8, batch_min_sssp=64, single_workers=4)
+    ids = [srv.add_graph(g) for _, g in graphs]
+    qs = [(gid, kind, 3 + 7 * i) for i, (gid, kind) in enumerate([(0, "sssp"), (0, "bfs")] * 6 + [(1, "sssp")] * 4)]
+    qids = [srv.submit(ids[gid], kind, src) for gid, kind, src in qs]
+    for (gid, kind, src), qid in zip(qs, qids):
+        st = srv.wait(qid, 60000)
+        assert st["state_name"] == "done", (gid, kind, src, st)
+        _check(graphs, gid, kind, src, srv.fetch(qid, kind))
+    srv.close()
