@@ -28,19 +28,40 @@ struct BatchWork {
     float* X = nullptr;
     float* Y = nullptr;
     hipGraphExec_t minplus_exec[2] = {nullptr, nullptr};   // two captured iterations, [weighted]
+    int16_t* D16 = nullptr;                                // n x 64 distances, 16-bit storage (k = 64)
+    uint32_t* overflow = nullptr;                          // set when a distance does not fit 16 bits
+    hipGraphExec_t minplus16_exec[2] = {nullptr, nullptr};
 };
 
 namespace g1 {
 namespace {
 
-// Min-plus SpMM kernel: the chunked, barrier-free walk (default) or the round-based one of the
-// plus-times path (G1_MINPLUS_ROUNDS=1); both give the same bits.
+// Min-plus SpMM kernel: at k = 64 the wavefront-per-tile kernel (unless G1_MINPLUS_WAVE=0); otherwise
+// the chunked, barrier-free walk (default) or the round-based one of the plus-times path
+// (G1_MINPLUS_ROUNDS=1). All give the same bits.
 bool minplus_chunked() {
     static const bool chunked = [] {
         const char* e = getenv("G1_MINPLUS_ROUNDS");
         return !(e && e[0] == '1');
     }();
     return chunked;
+}
+
+// k = 64 batches keep distances in 16 bits while they fit (G1_MINPLUS_U16=0: always 32 bits).
+bool minplus_u16() {
+    static const bool on = [] {
+        const char* e = getenv("G1_MINPLUS_U16");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+bool minplus_wave() {
+    static const bool wave = [] {
+        const char* e = getenv("G1_MINPLUS_WAVE");
+        return !(e && e[0] == '0');
+    }();
+    return wave;
 }
 
 int k_slot(uint32_t k) { return k == 8 ? 0 : k == 16 ? 1 : k == 32 ? 2 : k == 64 ? 3 : -1; }
@@ -53,7 +74,45 @@ g1_status alloc_n(g1_graph* g, T** p, size_t count) {
     return G1_OK;
 }
 
-g1_status ensure_batch(g1_graph* g, uint32_t K, bool ppr, BatchWork** out) {
+// out[j * n + v] = D16[v * 64 + j] widened to 32 bits (-1 -> G1_INF), through LDS like
+// batch_columns_out_kernel.
+__global__ void __launch_bounds__(256) batch_columns_out16_kernel(uint32_t n, const int16_t* __restrict__ D,
+                                                                  uint32_t cols, uint32_t* __restrict__ out) {
+    __shared__ uint32_t s[64][65];
+    const uint32_t v0 = blockIdx.x * 64;
+    for (uint32_t e = threadIdx.x; e < 64u * 64u; e += 256) {
+        const uint32_t r = e / 64, c = e % 64;
+        if (v0 + r < n) s[c][r] = widen_dist<int16_t>(D[static_cast<size_t>(v0 + r) * 64 + c]);
+    }
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < 64u * cols; f += 256) {
+        const uint32_t c = f / 64, r = f % 64;
+        if (v0 + r < n) out[static_cast<size_t>(c) * n + v0 + r] = s[c][r];
+    }
+}
+
+uint32_t fill_blocks(size_t items) {
+    size_t b = (items + 255) / 256;
+    if (b > 8192) b = 8192;
+    return static_cast<uint32_t>(b ? b : 1);
+}
+
+// The row-start bitmap of the in-edge CSR used by the k = 64 min-plus kernel; built once per graph
+// (caller holds g->mu), finished before it returns so any stream may use it.
+g1_status ensure_row_starts(g1_graph* g, hipStream_t s) {
+    if (g->t_starts || !g->has_t) return G1_OK;
+    const size_t words = (static_cast<size_t>(g->m) >> 6) + 2;
+    G1_TRY(alloc_n(g, &g->t_starts, words));
+    G1_TRY(g1_fill_u32(reinterpret_cast<uint32_t*>(g->t_starts), 0u, 2 * words, reinterpret_cast<g1_stream_t>(s)));
+    if (g->n) {
+        row_starts_kernel<<<fill_blocks(g->n), 256, 0, s>>>(g->n, g->t_row_off, g->t_starts);
+        G1_LAUNCHED();
+    }
+    G1_HIP(hipStreamSynchronize(s));
+    return G1_OK;
+}
+
+g1_status ensure_batch(g1_graph* g, uint32_t K, bool ppr, BatchWork** out, hipStream_t s) {
     const int slot = k_slot(K);
     G1_CHECK(slot >= 0, G1_ERR_INVALID_ARG, "column tile k must be 8, 16, 32 or 64 (got %u)", K);
     G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "batched engines need the graph's transpose");
@@ -72,10 +131,15 @@ g1_status ensure_batch(g1_graph* g, uint32_t K, bool ppr, BatchWork** out) {
         b->recs = v;
         G1_TRY(alloc_n(g, &b->changed, 64));
     }
+    if (!ppr && K == 64) G1_TRY(ensure_row_starts(g, s));
     if (!ppr && !b->D) {
-        G1_TRY(alloc_n(g, &b->D, n * K));
+        G1_TRY(alloc_n(g, &b->D, (n + 1) * K));  // row n: all unreached (read for inactive sources)
         G1_TRY(alloc_n(g, &b->mask[0], n));
         G1_TRY(alloc_n(g, &b->mask[1], n));
+        if (K == 64 && minplus_u16() && (static_cast<uint64_t>(n) + 1) * 128 < (1ull << 32)) {
+            G1_TRY(alloc_n(g, &b->D16, (n + 1) * 64));
+            G1_TRY(alloc_n(g, &b->overflow, 64));
+        }
     }
     if (ppr && !b->X) {
         G1_TRY(alloc_n(g, &b->R[0], n * K));
@@ -89,23 +153,26 @@ g1_status ensure_batch(g1_graph* g, uint32_t K, bool ppr, BatchWork** out) {
     return G1_OK;
 }
 
-// D = INF, D[src_j][j] = 0 for the tile's valid columns; mask[v] = columns whose source is v.
-template <int K>
-__global__ void minplus_batch_init_kernel(uint32_t n, uint32_t* __restrict__ D, unsigned long long* __restrict__ mask,
-                                          const uint32_t* __restrict__ srcs, uint32_t cols) {
-    const size_t total = static_cast<size_t>(n) * K;
+// D = unreached (all ones in DT), mask = 0; then D[src_j][j] = 0 for the tile's valid columns and
+// mask[v] = the columns whose source is v. The init kernel also clears the 16-bit overflow flag.
+template <int K, typename DT>
+__global__ void minplus_batch_init_kernel(uint32_t n, DT* __restrict__ D, unsigned long long* __restrict__ mask,
+                                          uint32_t* __restrict__ overflow) {
+    const size_t total = (static_cast<size_t>(n) + 1) * K;  // rows 0..n (row n stays unreached)
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) D[i] = kInf;
-    for (size_t v = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) mask[v] = 0ull;
+    const size_t t0 = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t i = t0; i < total; i += stride) D[i] = static_cast<DT>(~DT(0));
+    for (size_t v = t0; v < n; v += stride) mask[v] = 0ull;
+    if (overflow && t0 == 0) *overflow = 0u;
 }
 
-template <int K>
-__global__ void minplus_batch_seed_kernel(uint32_t* __restrict__ D, unsigned long long* __restrict__ mask,
+template <int K, typename DT>
+__global__ void minplus_batch_seed_kernel(DT* __restrict__ D, unsigned long long* __restrict__ mask,
                                           const uint32_t* __restrict__ srcs, uint32_t cols) {
     const uint32_t j = threadIdx.x;
     if (j < cols) {
         const uint32_t s = srcs[j];
-        D[static_cast<size_t>(s) * K + j] = 0u;
+        D[static_cast<size_t>(s) * K + j] = DT(0);
         atomicOr(&mask[s], 1ull << j);
     }
 }
@@ -148,51 +215,73 @@ __global__ void __launch_bounds__(256) batch_columns_out_kernel(uint32_t n, cons
     }
 }
 
-uint32_t fill_blocks(size_t items) {
-    size_t b = (items + 255) / 256;
-    if (b > 8192) b = 8192;
-    return static_cast<uint32_t>(b ? b : 1);
-}
-
-// One min-plus iteration: SpMM over the in-edges with the change masks (prev -> next), its fixup, and the
-// count of changed rows into b->active.
-template <int K>
-g1_status minplus_iteration(g1_graph* g, BatchWork* b, int prev, bool weighted, hipStream_t s) {
-    unsigned long long* flag = b->changed + prev;
+// One min-plus SpMM iteration over the in-edges: D relaxed in place, change masks prev -> next, the
+// columns that changed ORed into *flag; then the fixup of rows that cross tiles. DT is the distance
+// storage (int16_t only at k = 64 with the wavefront kernel; overflow is then raised instead of storing
+// a distance >= 0xFFFF).
+template <int K, typename DT = uint32_t>
+g1_status launch_minplus(g1_graph* g, DT* D, unsigned long long* mask_prev, unsigned long long* mask_next,
+                         void* recs_v, bool weighted, unsigned long long* flag, hipStream_t s,
+                         uint32_t* overflow = nullptr) {
     const uint32_t tiles = merge_num_tiles(g->n, g->m);
-    MinPlusBatchPolicy<K> pol{b->D, b->mask[prev], b->mask[prev ^ 1]};
-    auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(b->recs);
+    auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(recs_v);
     const uint32_t* vals = weighted ? g->t_w : nullptr;
-    if (minplus_chunked())
-        spmm_minplus_kernel<K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
-            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, flag);
-    else
-        spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
-            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, flag);
-    G1_LAUNCHED();
-    if (tiles > 1) {
-        const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
-        spmm_fixup_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<blocks, 256, 0, s>>>(tiles, merge_coords(g->t_ws),
-                                                                                     recs, pol, flag);
+    const MergeCoord* coords = merge_coords(g->t_ws);
+    const uint32_t fix_blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
+    if constexpr (K == 64) {
+        if (sizeof(DT) == 2 || minplus_wave()) {
+            const uint32_t blocks = (tiles + 3) / 4;
+            G1_CHECK(g->t_starts != nullptr, G1_ERR_INTERNAL, "min-plus k = 64: row-start bitmap not built");
+            if (weighted)
+                spmm_minplus_wave_kernel<true, DT><<<blocks, 256, 0, s>>>(g->n, tiles, g->t_row_off, g->t_col, vals,
+                                                                          g->t_starts, coords, recs, D, mask_prev,
+                                                                          mask_next, flag, overflow);
+            else
+                spmm_minplus_wave_kernel<false, DT><<<blocks, 256, 0, s>>>(g->n, tiles, g->t_row_off, g->t_col, vals,
+                                                                           g->t_starts, coords, recs, D, mask_prev,
+                                                                           mask_next, flag, overflow);
+            G1_LAUNCHED();
+            if (tiles > 1) {
+                if constexpr (sizeof(DT) == 2) {
+                    MinPlusBatchPolicy16 pol{D, mask_prev, mask_next, overflow};
+                    spmm_fixup_kernel<MinPlusU32, 64, MinPlusBatchPolicy16><<<fix_blocks, 256, 0, s>>>(
+                        tiles, coords, recs, pol, flag);
+                } else {
+                    MinPlusBatchPolicy<64> pol{D, mask_prev, mask_next};
+                    spmm_fixup_kernel<MinPlusU32, 64, MinPlusBatchPolicy<64>><<<fix_blocks, 256, 0, s>>>(
+                        tiles, coords, recs, pol, flag);
+                }
+                G1_LAUNCHED();
+            }
+            return G1_OK;
+        }
+    }
+    if constexpr (sizeof(DT) == 4) {
+        MinPlusBatchPolicy<K> pol{D, mask_prev, mask_next};
+        if (minplus_chunked())
+            spmm_minplus_kernel<K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+                g->n, g->m, g->t_row_off, g->t_col, vals, coords, recs, pol, flag);
+        else
+            spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+                g->n, g->m, g->t_row_off, g->t_col, vals, coords, recs, pol, flag);
         G1_LAUNCHED();
+        if (tiles > 1) {
+            spmm_fixup_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<fix_blocks, 256, 0, s>>>(tiles, coords, recs,
+                                                                                             pol, flag);
+            G1_LAUNCHED();
+        }
     }
     return G1_OK;
 }
 
-template <int K>
-g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host, uint32_t cols, bool weighted,
-                           uint32_t* out, uint32_t* iterations, hipStream_t s) {
-    const uint32_t n = g->n;
-    G1_HIP(hipMemcpyAsync(g->seeds_dev, srcs_host, cols * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    minplus_batch_init_kernel<K><<<fill_blocks(static_cast<size_t>(n) * K), 256, 0, s>>>(n, b->D, b->mask[0],
-                                                                                          g->seeds_dev, cols);
-    G1_LAUNCHED();
-    minplus_batch_seed_kernel<K><<<1, 64, 0, s>>>(b->D, b->mask[0], g->seeds_dev, cols);
-    G1_LAUNCHED();
-    // Two iterations per replay: mask[0] -> mask[1] -> mask[0].
-    hipGraphExec_t& exec = b->minplus_exec[weighted ? 1 : 0];
+// Replays the captured pair of iterations (mask[0] -> mask[1] -> mask[0]) until an iteration changes
+// nothing; the count includes that last, unchanged iteration. The pair is captured once per (graph, k,
+// storage, weighted) on a private stream; the pointers it holds never change.
+template <int K, typename DT>
+g1_status minplus_replay(g1_graph* g, BatchWork* b, DT* D, hipGraphExec_t& exec, bool weighted, uint32_t* iterations,
+                         hipStream_t s) {
+    uint32_t* ovf = sizeof(DT) == 2 ? b->overflow : nullptr;
     if (!exec) {
-        // Captured once per (graph, k, weighted) on a private stream; the pointers it holds never change.
         hipStream_t cs = nullptr;
         G1_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         hipGraph_t graph = nullptr;
@@ -200,8 +289,9 @@ g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host,
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             clear_flags_kernel<<<1, 64, 0, cs>>>(b->changed);
-            st = minplus_iteration<K>(g, b, 0, weighted, cs);
-            if (st == G1_OK) st = minplus_iteration<K>(g, b, 1, weighted, cs);
+            st = launch_minplus<K, DT>(g, D, b->mask[0], b->mask[1], b->recs, weighted, b->changed, cs, ovf);
+            if (st == G1_OK)
+                st = launch_minplus<K, DT>(g, D, b->mask[1], b->mask[0], b->recs, weighted, b->changed + 1, cs, ovf);
             e = hipStreamEndCapture(cs, &graph);
         }
         (void)hipStreamDestroy(cs);
@@ -211,7 +301,6 @@ g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host,
         (void)hipGraphDestroy(graph);
         G1_HIP(e);
     }
-    // Replays until an iteration changes nothing; the count includes that last, unchanged iteration.
     uint32_t it = 0;
     unsigned long long* hf = g->h_ctr + 12;
     for (;;) {
@@ -223,6 +312,40 @@ g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host,
         if (hf[1] == 0 || it >= kHistIters) break;
     }
     *iterations = it;
+    return G1_OK;
+}
+
+template <int K>
+g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host, uint32_t cols, bool weighted,
+                           uint32_t* out, uint32_t* iterations, hipStream_t s) {
+    const uint32_t n = g->n;
+    G1_HIP(hipMemcpyAsync(g->seeds_dev, srcs_host, cols * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if constexpr (K == 64) {
+        if (b->D16 && (!weighted || g->max_w <= 0xFFFF0000u)) {
+            // 16-bit storage first; a distance that does not fit voids the run, which is redone in 32 bits.
+            minplus_batch_init_kernel<64, int16_t><<<fill_blocks(static_cast<size_t>(n) * 64), 256, 0, s>>>(
+                n, b->D16, b->mask[0], b->overflow);
+            G1_LAUNCHED();
+            minplus_batch_seed_kernel<64, int16_t><<<1, 64, 0, s>>>(b->D16, b->mask[0], g->seeds_dev, cols);
+            G1_LAUNCHED();
+            G1_TRY((minplus_replay<64, int16_t>(g, b, b->D16, b->minplus16_exec[weighted ? 1 : 0], weighted,
+                                                 iterations, s)));
+            uint32_t* hov = reinterpret_cast<uint32_t*>(g->h_ctr + 14);
+            G1_HIP(hipMemcpyAsync(hov, b->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            G1_HIP(hipStreamSynchronize(s));
+            if (*hov == 0) {
+                batch_columns_out16_kernel<<<(n + 63) / 64, 256, 0, s>>>(n, b->D16, cols, out);
+                G1_LAUNCHED();
+                return G1_OK;
+            }
+        }
+    }
+    minplus_batch_init_kernel<K, uint32_t><<<fill_blocks(static_cast<size_t>(n) * K), 256, 0, s>>>(
+        n, b->D, b->mask[0], nullptr);
+    G1_LAUNCHED();
+    minplus_batch_seed_kernel<K, uint32_t><<<1, 64, 0, s>>>(b->D, b->mask[0], g->seeds_dev, cols);
+    G1_LAUNCHED();
+    G1_TRY((minplus_replay<K, uint32_t>(g, b, b->D, b->minplus_exec[weighted ? 1 : 0], weighted, iterations, s)));
     batch_columns_out_kernel<K, uint32_t><<<(n + 63) / 64, 256, 0, s>>>(n, b->D, cols, out);
     G1_LAUNCHED();
     return G1_OK;
@@ -290,7 +413,7 @@ template <int K>
 g1_status sssp_batch_k(g1_graph* g, const uint32_t* srcs, uint32_t count, bool weighted, uint32_t* out,
                        g1_batch_stats* st, hipStream_t s) {
     BatchWork* b = nullptr;
-    G1_TRY(ensure_batch(g, K, false, &b));
+    G1_TRY(ensure_batch(g, K, false, &b, s));
     G1_TRY(ensure_rank_partials(g));  // seeds_dev
     for (uint32_t c0 = 0; c0 < count; c0 += K) {
         const uint32_t cols = std::min<uint32_t>(K, count - c0);
@@ -306,7 +429,7 @@ template <int K>
 g1_status ppr_batch_k(g1_graph* g, const uint32_t* seeds, uint32_t count, float restart, uint32_t max_iter, double tol,
                       float* out, g1_batch_stats* st, hipStream_t s) {
     BatchWork* b = nullptr;
-    G1_TRY(ensure_batch(g, K, true, &b));
+    G1_TRY(ensure_batch(g, K, true, &b, s));
     for (uint32_t c0 = 0; c0 < count; c0 += K) {
         const uint32_t cols = std::min<uint32_t>(K, count - c0);
         uint32_t iters = 0;
@@ -501,26 +624,11 @@ g1_status sess_alloc(Session* ss, T** p, size_t count) {
 template <int K>
 g1_status session_step_minplus(Session* ss, uint64_t* done) {
     g1_graph* g = ss->g;
-    const uint32_t tiles = merge_num_tiles(g->n, g->m);
     const hipStream_t s = ss->s;
     clear_flags_kernel<<<1, 64, 0, s>>>(ss->changed);
     G1_LAUNCHED();
-    MinPlusBatchPolicy<K> pol{ss->D, ss->mask[ss->mcur], ss->mask[ss->mcur ^ 1]};
-    auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(ss->recs);
-    const uint32_t* vals = ss->kind == kSessMinPlusWeighted ? g->t_w : nullptr;
-    if (minplus_chunked())
-        spmm_minplus_kernel<K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
-            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, ss->changed);
-    else
-        spmm_tile_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<tiles, 256, 0, s>>>(
-            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, ss->changed);
-    G1_LAUNCHED();
-    if (tiles > 1) {
-        const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
-        spmm_fixup_kernel<MinPlusU32, K, MinPlusBatchPolicy<K>><<<blocks, 256, 0, s>>>(tiles, merge_coords(g->t_ws), recs,
-                                                                                     pol, ss->changed);
-        G1_LAUNCHED();
-    }
+    G1_TRY(launch_minplus<K>(g, ss->D, ss->mask[ss->mcur], ss->mask[ss->mcur ^ 1], ss->recs,
+                             ss->kind == kSessMinPlusWeighted, ss->changed, s));
     G1_HIP(hipMemcpyAsync(ss->h_changed, ss->changed, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     G1_HIP(hipStreamSynchronize(s));
     ss->mcur ^= 1;
@@ -620,7 +728,7 @@ size_t session_bytes(const g1_graph* g, int kind, uint32_t k) {
         b += 4 * n * k * sizeof(float) + n * sizeof(float) +
              (static_cast<size_t>(rank_blocks(g->n)) * k + k) * sizeof(double) + k * sizeof(uint32_t);
     else
-        b += n * k * sizeof(uint32_t) + 2 * n * sizeof(unsigned long long);
+        b += (n + 1) * k * sizeof(uint32_t) + 2 * n * sizeof(unsigned long long);
     return b;
 }
 
@@ -629,9 +737,10 @@ g1_status session_create(g1_graph* g, int kind, uint32_t k, float restart, hipSt
     G1_CHECK(k_slot(k) >= 0, G1_ERR_INVALID_ARG, "column width must be 8, 16, 32 or 64 (got %u)", k);
     G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "batched engines need the graph's transpose");
     G1_CHECK(kind != kSessMinPlusWeighted || g->t_w != nullptr, G1_ERR_INVALID_ARG, "graph has no weights");
-    if (kind == kSessPpr) {
-        std::lock_guard<std::mutex> lock(g->mu);  // lazily built graph arrays (float weights, out-weight sums)
-        G1_TRY(ensure_pull_work(g, true));
+    {
+        std::lock_guard<std::mutex> lock(g->mu);  // lazily built graph arrays
+        if (kind == kSessPpr) G1_TRY(ensure_pull_work(g, true));  // float weights, out-weight sums
+        else if (k == 64) G1_TRY(ensure_row_starts(g, stream));
     }
     Session* ss = new Session();
     ss->g = g;
@@ -664,11 +773,11 @@ g1_status session_create(g1_graph* g, int kind, uint32_t k, float restart, hipSt
         inv_weight_kernel<<<fill_blocks(g->n), 256, 0, stream>>>(g->n, g->osum, 1.0f - restart, ss->iw);
         if (hipGetLastError() != hipSuccess) return fail(set_error(G1_ERR_HIP, "session_create: launch"));
     } else {
-        if ((st = sess_alloc(ss, &ss->D, n * k)) != G1_OK) return fail(st);
+        if ((st = sess_alloc(ss, &ss->D, (n + 1) * k)) != G1_OK) return fail(st);  // row n: unreached
         for (int i = 0; i < 2 && st == G1_OK; ++i) st = sess_alloc(ss, &ss->mask[i], n);
         if (st != G1_OK) return fail(st);
         // every column empty: D = INF, no source bits in either mask
-        if ((st = g1_fill_u32(ss->D, kInf, n * k, stream)) != G1_OK) return fail(st);
+        if ((st = g1_fill_u32(ss->D, kInf, (n + 1) * k, stream)) != G1_OK) return fail(st);
         for (int i = 0; i < 2 && st == G1_OK; ++i) st = g1_fill_u32(reinterpret_cast<uint32_t*>(ss->mask[i]), 0u, 2 * n, stream);
         if (st != G1_OK) return fail(st);
     }
@@ -751,6 +860,8 @@ void batch_release(g1_graph* g) {
     for (BatchWork*& b : g->batch) {
         if (!b) continue;
         for (hipGraphExec_t& e : b->minplus_exec)
+            if (e) (void)hipGraphExecDestroy(e);
+        for (hipGraphExec_t& e : b->minplus16_exec)
             if (e) (void)hipGraphExecDestroy(e);
         delete b;  // device areas belong to the graph's allocation list
         b = nullptr;

@@ -276,6 +276,7 @@ g1_status g1_graph_create_view(g1_graph* base, g1_graph** out) {
     g->t_alias = base->t_alias;
     g->t_ws = base->t_ws;
     g->t_ws_bytes = base->t_ws_bytes;
+    g->t_starts = base->t_starts;
     g->osum = base->osum;
     g->best_k[0] = base->best_k[0];
     g->best_k[1] = base->best_k[1];

@@ -29,6 +29,7 @@ struct g1_graph {
     bool t_alias = false;
     void* t_ws = nullptr;  // merge-path partition of the in-edge CSR
     size_t t_ws_bytes = 0;
+    unsigned long long* t_starts = nullptr;  // bitmap of in-edge CSR row starts (lazy, batched min-plus)
 
     // push work areas
     uint32_t* q[2] = {nullptr, nullptr};

@@ -46,9 +46,12 @@ inline size_t spmm_records_bytes(uint32_t n_rows, uint32_t nnz) {
 // Lanes of the calling thread's K-lane group as a mask of the group's ballot bits.
 template <int K>
 __device__ __forceinline__ uint64_t group_bits(uint64_t wave_ballot) {
-    if constexpr (K == 64) return wave_ballot;
-    const uint32_t shift = (lane_id() / K) * K;
-    return (wave_ballot >> shift) & ((1ull << K) - 1ull);
+    if constexpr (K == 64) {
+        return wave_ballot;
+    } else {
+        const uint32_t shift = (lane_id() / K) * K;
+        return (wave_ballot >> shift) & ((1ull << K) - 1ull);
+    }
 }
 
 template <typename S, int K, typename P>
@@ -520,16 +523,290 @@ __global__ void __launch_bounds__(256) spmm_fixup_kernel(uint32_t num_tiles, con
     const bool live = t < num_tiles && t > 0 && (recs[t].flags & kTileDeferred) != 0u;
     // Whole groups agree on `live` (t is uniform within a group), so row_done is called by full groups.
     if (!live) return;
+    // The nearest tile u < t that closes a row (or tile 0): the group's K lanes test K tiles per step.
     uint32_t u = t - 1;
-    while (u > 0 && (recs[u].flags & kTileHasEnd) == 0u) --u;
+    for (;;) {
+        const bool has_end = j <= u && (recs[u - j].flags & kTileHasEnd) != 0u;
+        const uint64_t hits = group_bits<K>(ballot(has_end));
+        if (hits) {
+            u -= static_cast<uint32_t>(__builtin_ctzll(hits));
+            break;
+        }
+        if (u < static_cast<uint32_t>(K)) {
+            u = 0;
+            break;
+        }
+        u -= K;
+    }
+    // Carries left to right (the fold order of spmv_merge_fixup_kernel), loaded eight at a time.
     acc_t acc = S::identity();
-    for (uint32_t v = u; v < t; ++v) acc = S::merge(acc, recs[v].carry[j]);
+    uint32_t v = u;
+    for (; v + 8 <= t; v += 8) {
+        acc_t cv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) cv[i] = recs[v + i].carry[j];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc = S::merge(acc, cv[i]);
+    }
+    for (; v < t; ++v) acc = S::merge(acc, recs[v].carry[j]);
     acc = S::merge(acc, recs[t].head[j]);
     const uint32_t row = coords[t].x;
     typename S::x_t old = typename S::x_t(0);
     if constexpr (P::kRowOld) old = pol.row_old(row, j);
     const uint64_t c = pol.row_done(row, j, S::finalize(acc), old);
-    if (c && changed) atomicOr(changed, static_cast<unsigned long long>(c));
+    // Thousands of tiles may report changed columns; the shared word is only written with new bits.
+    if (c && changed) {
+        const unsigned long long cur = *reinterpret_cast<volatile unsigned long long*>(changed);
+        if (c & ~cur) atomicOr(changed, static_cast<unsigned long long>(c));
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Min-plus SpMM at k = 64 with one wavefront per merge-path tile: lane j owns column j.
+//
+// The tile's nonzeros are taken in chunks of 64 aligned to multiples of 64 (lane t <-> nonzero
+// 64c + t): one coalesced load brings the chunk's column indices and weights, one gather the change
+// masks of their sources. A source whose mask is empty costs nothing more; for an active one the wave
+// reads its index and mask out of the lane that holds them (readlane) and loads the source's 64
+// distances with one 256-byte access, the lanes of the unchanged columns masked off. The loads of chunk
+// c + 1 (masks) and c + 2 (indices) are issued before those of chunk c, so they are in flight while
+// chunk c is folded. Row boundaries inside a chunk come from a per-graph bitmap of row starts (bit q:
+// a non-empty row begins at nonzero q): the fold keeps one running minimum and, at each boundary, parks
+// the finished row's 64 values in LDS; the rows are then applied in order (d = min(d, y), the columns
+// that improved go to mask_next). A chunk with more boundaries than the LDS slots is folded in parts.
+//
+// The column and weight arrays must be readable up to the next multiple of 64 past the last nonzero
+// (the graph keeps kCsrPad = 64 padding entries).
+//
+// Min is order-free, so the result does not depend on how rows are cut into tiles or chunks; rows that
+// cross tiles leave head / carry records in the SpmmTileRec layout and are finished by spmm_fixup_kernel
+// with MinPlusBatchPolicy<64>, as for the other min-plus kernels. Distances are updated in place (a
+// source row read during the iteration may already hold this iteration's smaller value): each column
+// still decreases monotonically to the same fixed point, and an iteration with no change ends the run.
+constexpr uint32_t kWaveStage = 16;  // finished rows parked per fold part
+constexpr uint32_t kCloseBatch = 4;  // rows applied together (their old distances loaded together)
+
+// Bit q of starts[q >> 6]: a non-empty row of the CSR begins at nonzero q. starts has (nnz >> 6) + 2
+// words and must be zero on entry.
+__global__ void __launch_bounds__(256) row_starts_kernel(uint32_t n_rows, const uint32_t* __restrict__ row_offsets,
+                                                         unsigned long long* __restrict__ starts) {
+    for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < n_rows; r += gridDim.x * 256) {
+        const uint32_t b = row_offsets[r];
+        if (row_offsets[r + 1] > b) atomicOr(&starts[b >> 6], 1ull << (b & 63u));
+    }
+}
+
+// Distances are stored as DT: uint32_t, or int16_t when every finite distance of the run is below
+// kCompactLimit (then a source row is one 128-byte line instead of two). 16-bit storage is exact or
+// void: -1 is the unreached distance, so a sign-extending load yields G1_INF for it and the distance
+// itself otherwise; a relaxation that would store a finite value >= kCompactLimit sets *overflow instead
+// (the caller then reruns the batch with 32-bit storage). With 16-bit storage the weight is added with
+// unsigned saturation, which equals the semiring's combine because x + w cannot wrap (the caller only
+// picks 16-bit storage when every weight is <= 0xFFFF0000).
+constexpr uint32_t kCompactLimit = 0x7FFFu;
+
+template <typename DT>
+__device__ __forceinline__ uint32_t widen_dist(DT raw) {
+    if constexpr (sizeof(DT) == 2) return static_cast<uint32_t>(static_cast<int32_t>(raw));
+    return static_cast<uint32_t>(raw);
+}
+
+template <bool kWeighted, typename DT>
+__global__ void __launch_bounds__(256, 5)  // five waves per SIMD: at most 96 VGPRs
+spmm_minplus_wave_kernel(uint32_t n_rows, uint32_t num_tiles, const uint32_t* __restrict__ row_offsets,
+                         const uint32_t* __restrict__ col, const uint32_t* __restrict__ vals,
+                         const unsigned long long* __restrict__ starts, const MergeCoord* __restrict__ coords,
+                         SpmmTileRec<MinPlusU32, 64>* __restrict__ recs, DT* __restrict__ D,
+                         const unsigned long long* __restrict__ mask_prev, unsigned long long* __restrict__ mask_next,
+                         unsigned long long* __restrict__ changed, uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t s_stage[4][kWaveStage][64];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t tile = blockIdx.x * 4u + wave;
+    if (tile >= num_tiles) return;
+    const uint32_t r0 = coords[tile].x, r_last = coords[tile + 1].x;  // rows [r0, r_last) end in the tile
+    const uint32_t q0 = coords[tile].y, q1 = coords[tile + 1].y;      // nonzeros [q0, q1)
+    const bool deferred = r0 < r_last && row_offsets[r0] < q0;         // the first row began before the tile
+
+    // Row ends of 64 consecutive rows, one per lane, read out with readlane.
+    uint32_t rbase = r0;
+    uint32_t rwin = row_offsets[rbase + 1 + lane <= n_rows ? rbase + 1 + lane : n_rows];
+    auto row_end = [&](uint32_t r) -> uint32_t {
+        if (r - rbase >= 64u) {
+            rbase = r;
+            rwin = row_offsets[rbase + 1 + lane <= n_rows ? rbase + 1 + lane : n_rows];
+        }
+        return __builtin_amdgcn_readlane(rwin, r - rbase);
+    };
+
+    uint32_t r = r0;
+    uint32_t rs = r0 < r_last ? row_offsets[r0] : 0u;  // start of row r
+    uint32_t re = r0 < r_last ? row_end(r0) : kInf;    // end of row r (kInf: the tile's open tail)
+    uint32_t acc = kInf;                               // running minimum of the open row
+    uint64_t counted = 0;
+
+    // Closes every row that ends at or before nonzero position lim, kCloseBatch rows at a time: the old
+    // distances of a batch are loaded together, then the rows are applied in order (d = min(d, y), the
+    // columns that improved go to mask_next; the deferred first row goes to the tile record). A non-empty
+    // row that ends before lim was parked in the next LDS slot; the one that ends at lim is the open
+    // minimum; an empty row (or every row when all_inf) is unreached.
+    auto close_upto = [&](uint32_t lim, bool all_inf) {
+        uint32_t k = 0;  // next LDS slot
+        while (re <= lim) {
+            uint32_t rid[kCloseBatch], src[kCloseBatch], slot[kCloseBatch];  // src: 0 none, 1 unreached, 2 slot, 3 open
+            uint32_t open_val = kInf;
+#pragma unroll
+            for (uint32_t b = 0; b < kCloseBatch; ++b) {
+                src[b] = 0;
+                rid[b] = 0;
+                slot[b] = 0;
+                if (re <= lim) {
+                    rid[b] = r;
+                    if (re == rs || all_inf) {
+                        src[b] = 1;
+                    } else if (re < lim) {
+                        src[b] = 2;
+                        slot[b] = k++;
+                    } else {
+                        src[b] = 3;
+                        open_val = acc;
+                        acc = kInf;
+                    }
+                    rs = re;
+                    ++r;
+                    re = r < r_last ? row_end(r) : kInf;
+                }
+            }
+            uint32_t old[kCloseBatch];
+#pragma unroll
+            for (uint32_t b = 0; b < kCloseBatch; ++b) {
+                old[b] = kInf;
+                if (src[b] >= 2 && !(rid[b] == r0 && deferred))
+                    old[b] = widen_dist<DT>(D[static_cast<size_t>(rid[b]) * 64 + lane]);
+            }
+#pragma unroll
+            for (uint32_t b = 0; b < kCloseBatch; ++b) {
+                if (src[b] == 0) continue;
+                const uint32_t val = src[b] == 2 ? s_stage[wave][slot[b]][lane] : (src[b] == 3 ? open_val : kInf);
+                if (rid[b] == r0 && deferred) {
+                    recs[tile].head[lane] = MinPlusU32::acc_t{val};
+                    continue;
+                }
+                const bool improved = val < old[b];
+                if (improved) {
+                    if (sizeof(DT) == 2 && val >= kCompactLimit) *overflow = 1u;
+                    else D[static_cast<size_t>(rid[b]) * 64 + lane] = static_cast<DT>(val);
+                }
+                const uint64_t bits = ballot(improved);
+                if (lane == 0) mask_next[rid[b]] = bits;
+                counted |= bits;
+            }
+        }
+    };
+
+    close_upto(q0, false);
+    if (q0 < q1) {
+        const uint32_t c_first = q0 >> 6, c_end = (q1 + 63u) >> 6;
+        auto in_tile = [&](uint32_t c) {
+            const uint32_t q = c * 64u + lane;
+            return q >= q0 && q < q1;
+        };
+        uint32_t col0 = col[c_first * 64u + lane], w0 = kWeighted ? vals[c_first * 64u + lane] : 1u;
+        uint32_t col1 = 0, w1 = 0;
+        if (c_first + 1 < c_end) {
+            col1 = col[(c_first + 1) * 64u + lane];
+            w1 = kWeighted ? vals[(c_first + 1) * 64u + lane] : 1u;
+        }
+        unsigned long long m0 = in_tile(c_first) ? mask_prev[col0] : 0ull;
+        for (uint32_t c = c_first; c < c_end; ++c) {
+            const uint64_t act = ballot(m0 != 0ull);
+            // Prefetch, in this order (vector loads complete in order): masks of chunk c + 1, indices of
+            // chunk c + 2; both finish before this chunk's distance loads.
+            unsigned long long m1 = 0;
+            if (c + 1 < c_end && in_tile(c + 1)) m1 = mask_prev[col1];
+            uint32_t col2 = 0, w2 = 0;
+            if (c + 2 < c_end) {
+                col2 = col[(c + 2) * 64u + lane];
+                w2 = kWeighted ? vals[(c + 2) * 64u + lane] : 1u;
+            }
+            const unsigned long long bnd = starts[c];
+            const uint32_t lo = (q0 > c * 64u ? q0 - c * 64u : 0u);
+            const uint32_t hi = (q1 - c * 64u < 64u ? q1 - c * 64u : 64u);
+            if (act == 0ull && ballot(acc != kInf) == 0ull) {
+                close_upto(c * 64u + hi, true);  // nothing to fold: every row closing here is unreached
+            } else {
+                // The 64 sources' distance rows: an active source's row, or for an inactive one the
+                // all-unreached row n_rows (kept in cache), so every entry folds without a test. Every
+                // column of an active source is relaxed: a column whose distance did not change last
+                // iteration was relaxed through this edge when it last changed, so it cannot improve
+                // the row now, and the result (a fixed point of min) is the same.
+                uint32_t x[64];
+                const uint32_t lane_off = lane * static_cast<uint32_t>(sizeof(DT));
+#pragma unroll
+                for (uint32_t t = 0; t < 64; ++t) {
+                    uint32_t cu = __builtin_amdgcn_readlane(col0, t);
+                    asm volatile("" : "+s"(cu));  // keep the readlane out of a branch: select, not jump
+                    const uint32_t u = ((act >> t) & 1ull) ? cu : n_rows;
+                    if constexpr (sizeof(DT) == 2) {
+                        // 32-bit byte offsets: the caller keeps (n_rows + 1) * 128 below 2^32
+                        x[t] = widen_dist<DT>(*reinterpret_cast<const DT*>(reinterpret_cast<const char*>(D) +
+                                                                          (u << 7) + lane_off));
+                    } else {
+                        x[t] = D[static_cast<size_t>(u) * 64 + lane];
+                    }
+                }
+                uint32_t lo_p = lo;
+                while (lo_p < hi) {
+                    // A fresh copy of the boundary bits for this part's fold, so the bits are tested
+                    // where they are used instead of 64 saved predicates being kept alive.
+                    uint64_t bnd_f = bnd;
+                    asm volatile("" : "+s"(bnd_f));
+                    // Boundaries strictly inside the part; at most kWaveStage of them.
+                    uint64_t inner = (lo_p + 1 < 64u) ? (bnd >> (lo_p + 1)) << (lo_p + 1) : 0ull;
+                    if (hi < 64u) inner &= (1ull << hi) - 1ull;
+                    uint32_t hi_p = hi;
+                    if (static_cast<uint32_t>(__popcll(inner)) > kWaveStage) {
+#pragma unroll
+                        for (uint32_t k = 0; k < kWaveStage; ++k) inner &= inner - 1ull;
+                        hi_p = static_cast<uint32_t>(__builtin_ctzll(inner));
+                    }
+                    uint32_t ns = 0;
+                    auto fold = [&](uint32_t t, bool first) {
+                        if (!first && ((bnd_f >> t) & 1ull)) {
+                            s_stage[wave][ns][lane] = acc;
+                            ++ns;
+                            acc = kInf;
+                        }
+                        const uint32_t wt = kWeighted ? __builtin_amdgcn_readlane(w0, t) : 1u;
+                        const uint32_t p = sizeof(DT) == 2 ? __builtin_elementwise_add_sat(x[t], wt)
+                                                           : MinPlusU32::combine(wt, x[t]);
+                        acc = p < acc ? p : acc;
+                    };
+                    if (lo_p == 0 && hi_p == 64) {  // the whole chunk in one part: no range tests
+#pragma unroll
+                        for (uint32_t t = 0; t < 64; ++t) fold(t, t == 0);
+                    } else {
+#pragma unroll
+                        for (uint32_t t = 0; t < 64; ++t)
+                            if (t >= lo_p && t < hi_p) fold(t, t == lo_p);
+                    }
+                    close_upto(c * 64u + hi_p, false);
+                    lo_p = hi_p;
+                }
+            }
+            col0 = col1;
+            w0 = w1;
+            m0 = m1;
+            col1 = col2;
+            w1 = w2;
+        }
+    }
+    recs[tile].carry[lane] = MinPlusU32::acc_t{acc};
+    if (lane == 0) recs[tile].flags = (r_last > r0 ? kTileHasEnd : 0u) | (deferred ? kTileDeferred : 0u);
+    if (counted && lane == 0) {
+        const unsigned long long cur = *reinterpret_cast<volatile unsigned long long*>(changed);
+        if (counted & ~cur) atomicOr(changed, static_cast<unsigned long long>(counted));
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -563,6 +840,30 @@ struct MinPlusBatchPolicy {
         const bool improved = y < old;
         if (improved) D[static_cast<size_t>(row) * K + j] = y;
         const uint64_t bits = group_bits<K>(ballot(improved));
+        if (j == 0) mask_next[row] = bits;
+        return (j == 0) ? bits : 0ull;
+    }
+};
+
+// The fixup side of spmm_minplus_wave_kernel with 16-bit distance storage (row_old / row_done only).
+struct MinPlusBatchPolicy16 {
+    using aux_t = unsigned long long;
+    static constexpr bool kHasAux = true;
+    static constexpr bool kRowOld = true;
+    int16_t* D;
+    const unsigned long long* mask_prev;
+    unsigned long long* mask_next;
+    uint32_t* overflow;
+    __device__ __forceinline__ uint32_t row_old(uint32_t row, uint32_t j) const {
+        return widen_dist<int16_t>(D[static_cast<size_t>(row) * 64 + j]);
+    }
+    __device__ __forceinline__ uint64_t row_done(uint32_t row, uint32_t j, uint32_t y, uint32_t old) const {
+        const bool improved = y < old;
+        if (improved) {
+            if (y >= kCompactLimit) *overflow = 1u;
+            else D[static_cast<size_t>(row) * 64 + j] = static_cast<int16_t>(y);
+        }
+        const uint64_t bits = ballot(improved);
         if (j == 0) mask_next[row] = bits;
         return (j == 0) ? bits : 0ull;
     }

@@ -287,7 +287,8 @@ g1_status g1_ppr_batch(g1_graph* graph, const uint32_t* seeds, uint32_t count, f
  * (a multi-source SpMM whose columns are freed and refilled one query at a time while the others keep
  * iterating) when one is open or when enough of them wait (the session opens if its arrays fit the budget;
  * SSSP only on graphs whose single-source runs took at most 64 iterations, where a full SpMM per iteration
- * pays), and run on the single-source engines otherwise. Every result is the one the
+ * pays), and run on the single-source engines otherwise; on low-degree graphs (g1_road_batch) a
+ * single-source thread runs up to 32 waiting BFS / SSSP queries in one launch. Every result is the one the
  * single-source call returns (bit-identical), whichever path and whichever neighbours it ran with.
  * Admission: a query is admitted when its result vector fits in the HBM budget, counting the registered
  * graphs, open sessions and results not yet fetched; otherwise it waits, in order, for memory to be freed

@@ -142,3 +142,23 @@ def test_road_batch_matches_oracle(graphs):
         assert np.array_equal(got.values[c], want), (c, s, k)
     with pytest.raises(g1.G1Error):
         graphs[0][2].road_batch([0])  # RMAT: hubs far above the road engine's degree bound
+
+
+def test_sssp_batch_distances_past_16_bits(gpu):
+    """k = 64 keeps distances in 16 bits while they fit; on a long heavy path (distances up to
+    ~100,000 > 0xFFFF) the batch must notice and redo the run with 32-bit storage, still bit-exact.
+    A second, light path checks the 16-bit run itself (all distances < 0xFFFF)."""
+    import graphone_amd as g1
+    n = 400
+    src = np.arange(n - 1, dtype=np.uint32)
+    dst = src + 1
+    for wv in (250, 3):
+        h = g1.HostGraph.from_edges(n, src, dst, np.full(n - 1, wv, dtype=np.uint32), directed=False)
+        g = g1.Graph(h)
+        srcs = [int(s) for s in np.linspace(0, n - 1, 64).astype(np.uint32)]
+        got = g.sssp_batch(srcs, k=64)
+        for c, s in enumerate(srcs):
+            want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, s)
+            assert np.array_equal(got.values[c], want), (wv, c, s)
+        if wv == 250:
+            assert int(got.values.max()) > 0xFFFF
