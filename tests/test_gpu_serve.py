@@ -48,9 +48,11 @@ def _check(graphs, gid, kind, src, res):
         assert np.array_equal(res, _want(h, kind, src)), (gid, kind, src)
 
 
-def test_served_results_match_single_source(graphs):
+@pytest.mark.parametrize("batch_k", [16, 64])
+def test_served_results_match_single_source(graphs, batch_k):
+    """batch_k 64 runs SSSP sessions on the k = 64 wavefront kernel, narrower ones on the group kernel."""
     import graphone_amd as g1
-    srv = g1.Server(batch_k=16, batch_min_sssp=2, batch_min_ppr=2)
+    srv = g1.Server(batch_k=batch_k, batch_min_sssp=2, batch_min_ppr=2)
     ids = [srv.add_graph(g) for _, g in graphs]
     qs = _queries(graphs, 48, seed=1)
     qids = [srv.submit(ids[gid], kind, src) for gid, kind, src in qs]
@@ -159,4 +161,26 @@ def test_fresh_server_on_recycled_memory(graphs):
         st = srv.wait(qid, 60000)
         assert st["state_name"] == "done", (gid, kind, src, st)
         _check(graphs, gid, kind, src, srv.fetch(qid, kind))
+    srv.close()
+
+
+def test_sssp_session_at_width_64(graphs):
+    """A backlog of 40 SSSP queries opens a 64-column session (the k = 64 wavefront kernel, 32-bit
+    distances, continuous batching); every served column equals the oracle's Dijkstra."""
+    import graphone_amd as g1
+    h, g = graphs[0]
+    srv = g1.Server(batch_k=64, batch_min_sssp=2)
+    gid = srv.add_graph(g)
+    first = srv.submit(gid, "sssp", 1)  # a single-source run first: the graph's iteration count is known
+    assert srv.wait(first, 60000)["state_name"] == "done"
+    _check(graphs, 0, "sssp", 1, srv.fetch(first, "sssp"))
+    srcs = [int(s) for s in np.random.default_rng(8).integers(0, h.n, size=40)]
+    qids = [srv.submit(gid, "sssp", s) for s in srcs]
+    batched = 0
+    for s, q in zip(srcs, qids):
+        st = srv.wait(q, 60000)
+        assert st["state_name"] == "done", st
+        batched += st["path"] == 1
+        _check(graphs, 0, "sssp", s, srv.fetch(q, "sssp"))
+    assert batched > 0
     srv.close()
