@@ -122,3 +122,28 @@ def test_bad_source_rejected(graphs):
     name, h, g = graphs[0]
     with pytest.raises(g1.G1Error):
         g.bfs(h.n + 5)
+
+
+def test_grid213_sssp_from_vertex_1(gpu):
+    """The sssp_adv workload: 213 x 213 grid, p = 0.6, symmetric weights in [1, 64], source 1; the
+    advance engine (the road engine on this low-degree graph) and the merge-path BLAS engine both equal
+    the oracle's Dijkstra bit for bit."""
+    import graphone_amd as g1
+    h = g1.HostGraph.grid(213, 213, 0.6, seed=7).assign_weights(1, 64, seed=8, symmetric=True)
+    g = g1.Graph(h, symmetric=True)
+    want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, 1)
+    for eng in ("advance", "merge", "csrvec"):
+        assert np.array_equal(g.sssp(1, eng).values, want), eng
+
+
+def test_pagerank_rmat20_l1(gpu):
+    """PageRank on unweighted RMAT scale 20 (16.1M edges) within L1 1e-5 of the FP64 oracle."""
+    import graphone_amd as g1
+    h = g1.HostGraph.rmat(20, 16, 0.57, 0.19, 0.19, seed=20)
+    t = h.transpose()
+    want, ost = oracle.pagerank_f64(h.row_offsets, None, t.row_offsets, t.col_indices, None, 0.85,
+                                    max_iter=50, tol=1e-7)
+    got = g1.Graph(h).pagerank(0.85, 50, 1e-7, "l1", "merge")
+    diff = np.abs(got.values.astype(np.float64) - want)
+    assert float(diff.sum()) <= 1e-5, (float(diff.sum()), float(diff.max()))
+    assert abs(got.stats["iterations"] - ost["iterations"]) <= 1
