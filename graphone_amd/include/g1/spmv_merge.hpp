@@ -92,57 +92,78 @@ static __global__ void __launch_bounds__(256) merge_partition_kernel(const uint3
     coords[t] = merge_path_search(diag, row_offsets + 1, n_rows, nnz);
 }
 
-// One workgroup per tile. LDS: row ends and products of the tile, then one tail per thread.
-template <typename S, bool kHasValues>
-__global__ void __launch_bounds__(kMergeThreads)
-spmv_merge_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row_offsets,
-                       const uint32_t* __restrict__ col, const typename S::value_t* __restrict__ vals,
-                       const typename S::x_t* __restrict__ x, typename S::y_t* __restrict__ y,
-                       const MergeCoord* __restrict__ coords, MergeTileRec<S>* __restrict__ recs) {
-    using acc_t = typename S::acc_t;
-    using y_t = typename S::y_t;
-    __shared__ uint32_t s_rowend[kMergeTile + 1];
-    __shared__ y_t s_prod[kMergeTile];
-    __shared__ acc_t s_tail[kMergeThreads];
-    __shared__ uint32_t s_hasend[kMergeThreads];
+// A tile's loads, staged in registers: its coordinates, row ends, column indices, values and gathered
+// x values (rows + nzs <= kMergeTile, so each thread owns at most kMergeItems row ends and kMergeItems
+// nonzeros).
+template <typename S>
+struct MergeStage {
+    MergeCoord c0, c1;
+    uint32_t re[kMergeItems];
+    uint32_t ci[kMergeItems];
+    typename S::value_t wv[kMergeItems];
+    typename S::x_t xv[kMergeItems];
+};
 
-    const uint32_t tile = blockIdx.x;
+// Issues the tile's coordinate, row-end, column and value loads.
+template <typename S, bool kHasValues>
+__device__ __forceinline__ void merge_load_index(uint32_t tile, const uint32_t* __restrict__ row_offsets,
+                                                 const uint32_t* __restrict__ col,
+                                                 const typename S::value_t* __restrict__ vals,
+                                                 const MergeCoord* __restrict__ coords, MergeStage<S>& st) {
     const uint32_t tid = threadIdx.x;
-    const MergeCoord c0 = coords[tile];
-    const MergeCoord c1 = coords[tile + 1];
+    st.c0 = coords[tile];
+    st.c1 = coords[tile + 1];
+    const uint32_t rows = st.c1.x - st.c0.x, nzs = st.c1.y - st.c0.y;
+#pragma unroll
+    for (int r = 0; r < kMergeItems; ++r) {
+        const uint32_t k = r * kMergeThreads + tid;
+        if (k < rows) st.re[r] = row_offsets[st.c0.x + 1 + k];
+        if (k < nzs) {
+            st.ci[r] = col[st.c0.y + k];
+            st.wv[r] = kHasValues ? vals[st.c0.y + k] : typename S::value_t(1);
+        }
+    }
+}
+
+// Issues the gathers of the tile's x values (its column loads must have arrived).
+template <typename S>
+__device__ __forceinline__ void merge_load_x(const typename S::x_t* __restrict__ x, MergeStage<S>& st) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nzs = st.c1.y - st.c0.y;
+#pragma unroll
+    for (int r = 0; r < kMergeItems; ++r) {
+        const uint32_t k = r * kMergeThreads + tid;
+        if (k < nzs) st.xv[r] = x[st.ci[r]];
+    }
+}
+
+// Writes the staged row ends and products to LDS.
+template <typename S>
+__device__ __forceinline__ void merge_to_lds(const MergeStage<S>& st, uint32_t* s_rowend, typename S::y_t* s_prod) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t rows = st.c1.x - st.c0.x, nzs = st.c1.y - st.c0.y;
+#pragma unroll
+    for (int r = 0; r < kMergeItems; ++r) {
+        const uint32_t k = r * kMergeThreads + tid;
+        if (k < rows) s_rowend[k] = st.re[r];
+        if (k < nzs) s_prod[k] = S::combine(st.wv[r], st.xv[r]);
+    }
+    if (tid == 0) s_rowend[rows] = kInf;  // no further row end inside this tile
+}
+
+// The walk of one staged tile (LDS filled, after a barrier): each thread's kMergeItems steps, the rows
+// it closes, the tails chain, and the tile record. Contains one barrier; every thread must call it.
+template <typename S>
+__device__ __forceinline__ void merge_walk(uint32_t tile, MergeCoord c0, MergeCoord c1,
+                                           const uint32_t* __restrict__ row_offsets, typename S::y_t* __restrict__ y,
+                                           MergeTileRec<S>* __restrict__ recs, const uint32_t* s_rowend,
+                                           const typename S::y_t* s_prod, typename S::acc_t* s_tail,
+                                           uint32_t* s_hasend) {
+    using acc_t = typename S::acc_t;
+    const uint32_t tid = threadIdx.x;
     const uint32_t rows = c1.x - c0.x;
     const uint32_t nzs = c1.y - c0.y;
     const uint32_t total = rows + nzs;
-
-    // Stage the tile with every load of a thread in flight at once (rows + nzs <= kMergeTile, so each
-    // thread owns at most kMergeItems row ends and kMergeItems nonzeros).
-    {
-        uint32_t re[kMergeItems], ci[kMergeItems];
-        typename S::value_t wv[kMergeItems];
-        typename S::x_t xv[kMergeItems];
-#pragma unroll
-        for (int r = 0; r < kMergeItems; ++r) {
-            const uint32_t k = r * kMergeThreads + tid;
-            if (k < rows) re[r] = row_offsets[c0.x + 1 + k];
-            if (k < nzs) {
-                ci[r] = col[c0.y + k];
-                wv[r] = kHasValues ? vals[c0.y + k] : typename S::value_t(1);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < kMergeItems; ++r) {
-            const uint32_t k = r * kMergeThreads + tid;
-            if (k < nzs) xv[r] = x[ci[r]];
-        }
-#pragma unroll
-        for (int r = 0; r < kMergeItems; ++r) {
-            const uint32_t k = r * kMergeThreads + tid;
-            if (k < rows) s_rowend[k] = re[r];
-            if (k < nzs) s_prod[k] = S::combine(wv[r], xv[r]);
-        }
-        if (tid == 0) s_rowend[rows] = kInf;  // no further row end inside this tile
-    }
-    __syncthreads();
 
     // This thread's first step inside the tile, then its kMergeItems steps.
     uint32_t d = tid * kMergeItems;
@@ -208,6 +229,28 @@ spmv_merge_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict
         if (u >= 0 && row_offsets[c0.x] < c0.y) flags |= kTileDeferred;
         recs[tile].flags = flags;
     }
+}
+
+// One workgroup per tile. LDS: row ends and products of the tile, then one tail per thread.
+template <typename S, bool kHasValues>
+__global__ void __launch_bounds__(kMergeThreads)
+spmv_merge_tile_kernel(uint32_t n_rows, uint32_t nnz, const uint32_t* __restrict__ row_offsets,
+                       const uint32_t* __restrict__ col, const typename S::value_t* __restrict__ vals,
+                       const typename S::x_t* __restrict__ x, typename S::y_t* __restrict__ y,
+                       const MergeCoord* __restrict__ coords, MergeTileRec<S>* __restrict__ recs) {
+    using acc_t = typename S::acc_t;
+    using y_t = typename S::y_t;
+    __shared__ uint32_t s_rowend[kMergeTile + 1];
+    __shared__ y_t s_prod[kMergeTile];
+    __shared__ acc_t s_tail[kMergeThreads];
+    __shared__ uint32_t s_hasend[kMergeThreads];
+    const uint32_t tile = blockIdx.x;
+    MergeStage<S> st;
+    merge_load_index<S, kHasValues>(tile, row_offsets, col, vals, coords, st);
+    merge_load_x<S>(x, st);
+    merge_to_lds<S>(st, s_rowend, s_prod);
+    __syncthreads();
+    merge_walk<S>(tile, st.c0, st.c1, row_offsets, y, recs, s_rowend, s_prod, s_tail, s_hasend);
 }
 
 // One thread per tile: finish the first closed row of every deferred tile by folding, left to right,
