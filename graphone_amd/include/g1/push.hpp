@@ -283,18 +283,23 @@ struct SsspPush {
     __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
         return static_cast<uint32_t>(dstamp[u] >> 32);
     }
-    // v[r] == kInf marks an empty slot. Slots are relaxed one after another: with the gather, the
-    // atomicMin and its result test kept together per slot, fewer atomics are in flight per wave, which
-    // measured faster on RMAT scale 22 than issuing all of a thread's atomics before testing any.
+    // v[r] == kInf marks an empty slot. The targets' distances are read for every slot first (all of a
+    // thread's random reads in flight at once); then the slots that would improve their target take
+    // their atomicMin one after another, each tested as it returns (fewer atomics in flight per wave
+    // measured faster than issuing them all before testing any). A distance read early may be stale by
+    // its atomic: the atomic then returns the smaller value and the slot does not win.
     template <int N>
     __device__ __forceinline__ void relax_many(const uint32_t (&)[N], const uint32_t (&du)[N], const uint32_t (&v)[N],
                                                const uint32_t (&w)[N], const uint32_t (&)[N], bool (&won)[N]) const {
+        uint32_t cur[N];
+#pragma unroll
+        for (int r = 0; r < N; ++r) cur[r] = v[r] != kInf ? static_cast<uint32_t>(dstamp[v[r]] >> 32) : 0u;
 #pragma unroll
         for (int r = 0; r < N; ++r) {
             won[r] = false;
             if (v[r] == kInf) continue;
             const uint32_t nd = du[r] + w[r];
-            if (nd >= static_cast<uint32_t>(dstamp[v[r]] >> 32)) continue;
+            if (nd >= cur[r]) continue;
             const unsigned long long old = atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | iter);
             won[r] = nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
         }
