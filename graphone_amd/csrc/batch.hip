@@ -31,6 +31,7 @@ struct BatchWork {
     int16_t* D16 = nullptr;                                // n x 64 distances, 16-bit storage (k = 64)
     uint32_t* overflow = nullptr;                          // set when a distance does not fit 16 bits
     hipGraphExec_t minplus16_exec[2] = {nullptr, nullptr};
+    uint32_t* act = nullptr;                               // k = 64: bitmap of last iteration's changed vertices
 };
 
 namespace g1 {
@@ -97,6 +98,9 @@ uint32_t fill_blocks(size_t items) {
     return static_cast<uint32_t>(b ? b : 1);
 }
 
+// Words of the k = 64 activity bitmap (act_bits_kernel): two per 64 vertices.
+size_t act_words(size_t n) { return 2 * ((n + 63) / 64) + 2; }
+
 // The row-start bitmap of the in-edge CSR used by the k = 64 min-plus kernel; built once per graph
 // (caller holds g->mu), finished before it returns so any stream may use it.
 g1_status ensure_row_starts(g1_graph* g, hipStream_t s) {
@@ -136,6 +140,7 @@ g1_status ensure_batch(g1_graph* g, uint32_t K, bool ppr, BatchWork** out, hipSt
         G1_TRY(alloc_n(g, &b->D, (n + 1) * K));  // row n: all unreached (read for inactive sources)
         G1_TRY(alloc_n(g, &b->mask[0], n));
         G1_TRY(alloc_n(g, &b->mask[1], n));
+        if (K == 64) G1_TRY(alloc_n(g, &b->act, act_words(n)));
         if (K == 64 && minplus_u16() && (static_cast<uint64_t>(n) + 1) * 128 < (1ull << 32)) {
             G1_TRY(alloc_n(g, &b->D16, (n + 1) * 64));
             G1_TRY(alloc_n(g, &b->overflow, 64));
@@ -221,7 +226,7 @@ __global__ void __launch_bounds__(256) batch_columns_out_kernel(uint32_t n, cons
 // a distance >= 0xFFFF).
 template <int K, typename DT = uint32_t>
 g1_status launch_minplus(g1_graph* g, DT* D, unsigned long long* mask_prev, unsigned long long* mask_next,
-                         void* recs_v, bool weighted, unsigned long long* flag, hipStream_t s,
+                         void* recs_v, bool weighted, unsigned long long* flag, hipStream_t s, uint32_t* act,
                          uint32_t* overflow = nullptr) {
     const uint32_t tiles = merge_num_tiles(g->n, g->m);
     auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(recs_v);
@@ -231,14 +236,19 @@ g1_status launch_minplus(g1_graph* g, DT* D, unsigned long long* mask_prev, unsi
     if constexpr (K == 64) {
         if (sizeof(DT) == 2 || minplus_wave()) {
             const uint32_t blocks = (tiles + 3) / 4;
-            G1_CHECK(g->t_starts != nullptr, G1_ERR_INTERNAL, "min-plus k = 64: row-start bitmap not built");
+            G1_CHECK(g->t_starts != nullptr && act != nullptr, G1_ERR_INTERNAL,
+                     "min-plus k = 64: row-start or activity bitmap missing");
+            // The wavefront kernel writes only non-zero change masks, and reads activity as a bitmap.
+            G1_TRY(g1_fill_u32(reinterpret_cast<uint32_t*>(mask_next), 0u, 2ull * g->n, reinterpret_cast<g1_stream_t>(s)));
+            act_bits_kernel<<<fill_blocks(static_cast<size_t>(g->n) / 4 + 64), 256, 0, s>>>(g->n, mask_prev, act);
+            G1_LAUNCHED();
             if (weighted)
                 spmm_minplus_wave_kernel<true, DT><<<blocks, 256, 0, s>>>(g->n, tiles, g->t_row_off, g->t_col, vals,
-                                                                          g->t_starts, coords, recs, D, mask_prev,
+                                                                          g->t_starts, coords, recs, D, act,
                                                                           mask_next, flag, overflow);
             else
                 spmm_minplus_wave_kernel<false, DT><<<blocks, 256, 0, s>>>(g->n, tiles, g->t_row_off, g->t_col, vals,
-                                                                           g->t_starts, coords, recs, D, mask_prev,
+                                                                           g->t_starts, coords, recs, D, act,
                                                                            mask_next, flag, overflow);
             G1_LAUNCHED();
             if (tiles > 1) {
@@ -289,9 +299,10 @@ g1_status minplus_replay(g1_graph* g, BatchWork* b, DT* D, hipGraphExec_t& exec,
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             clear_flags_kernel<<<1, 64, 0, cs>>>(b->changed);
-            st = launch_minplus<K, DT>(g, D, b->mask[0], b->mask[1], b->recs, weighted, b->changed, cs, ovf);
+            st = launch_minplus<K, DT>(g, D, b->mask[0], b->mask[1], b->recs, weighted, b->changed, cs, b->act, ovf);
             if (st == G1_OK)
-                st = launch_minplus<K, DT>(g, D, b->mask[1], b->mask[0], b->recs, weighted, b->changed + 1, cs, ovf);
+                st = launch_minplus<K, DT>(g, D, b->mask[1], b->mask[0], b->recs, weighted, b->changed + 1, cs, b->act,
+                                           ovf);
             e = hipStreamEndCapture(cs, &graph);
         }
         (void)hipStreamDestroy(cs);
@@ -593,6 +604,7 @@ struct Session {
     uint32_t iters[kBatchCols] = {};
     uint32_t max_iter[kBatchCols] = {};
     double tol[kBatchCols] = {};
+    uint32_t* act = nullptr;  // k = 64 min-plus: activity bitmap
     std::vector<void*> owned;
 };
 
@@ -628,7 +640,7 @@ g1_status session_step_minplus(Session* ss, uint64_t* done) {
     clear_flags_kernel<<<1, 64, 0, s>>>(ss->changed);
     G1_LAUNCHED();
     G1_TRY(launch_minplus<K>(g, ss->D, ss->mask[ss->mcur], ss->mask[ss->mcur ^ 1], ss->recs,
-                             ss->kind == kSessMinPlusWeighted, ss->changed, s));
+                             ss->kind == kSessMinPlusWeighted, ss->changed, s, ss->act));
     G1_HIP(hipMemcpyAsync(ss->h_changed, ss->changed, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     G1_HIP(hipStreamSynchronize(s));
     ss->mcur ^= 1;
@@ -774,6 +786,7 @@ g1_status session_create(g1_graph* g, int kind, uint32_t k, float restart, hipSt
         if (hipGetLastError() != hipSuccess) return fail(set_error(G1_ERR_HIP, "session_create: launch"));
     } else {
         if ((st = sess_alloc(ss, &ss->D, (n + 1) * k)) != G1_OK) return fail(st);  // row n: unreached
+        if (k == 64 && (st = sess_alloc(ss, &ss->act, act_words(n))) != G1_OK) return fail(st);
         for (int i = 0; i < 2 && st == G1_OK; ++i) st = sess_alloc(ss, &ss->mask[i], n);
         if (st != G1_OK) return fail(st);
         // every column empty: D = INF, no source bits in either mask

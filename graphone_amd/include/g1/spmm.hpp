@@ -611,13 +611,26 @@ __device__ __forceinline__ uint32_t widen_dist(DT raw) {
     return static_cast<uint32_t>(raw);
 }
 
+// act[v >> 5] bit (v & 31): mask[v] != 0 (the vertex changed in the last iteration). One wave per 64
+// vertices; act has 2 * ceil(n / 64) words.
+__global__ void __launch_bounds__(256) act_bits_kernel(uint32_t n, const unsigned long long* __restrict__ mask,
+                                                       uint32_t* __restrict__ act) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t groups = (n + 63u) / 64u;
+    for (uint32_t gi = (blockIdx.x * 256u + threadIdx.x) / 64u; gi < groups; gi += gridDim.x * 4u) {
+        const uint32_t v = gi * 64u + lane;
+        const uint64_t b = ballot(v < n && mask[v] != 0ull);
+        if (lane < 2) act[2 * gi + lane] = static_cast<uint32_t>(b >> (32 * lane));
+    }
+}
+
 template <bool kWeighted, typename DT>
 __global__ void __launch_bounds__(256, 5)  // five waves per SIMD: at most 96 VGPRs
 spmm_minplus_wave_kernel(uint32_t n_rows, uint32_t num_tiles, const uint32_t* __restrict__ row_offsets,
                          const uint32_t* __restrict__ col, const uint32_t* __restrict__ vals,
                          const unsigned long long* __restrict__ starts, const MergeCoord* __restrict__ coords,
                          SpmmTileRec<MinPlusU32, 64>* __restrict__ recs, DT* __restrict__ D,
-                         const unsigned long long* __restrict__ mask_prev, unsigned long long* __restrict__ mask_next,
+                         const uint32_t* __restrict__ act_prev, unsigned long long* __restrict__ mask_next,
                          unsigned long long* __restrict__ changed, uint32_t* __restrict__ overflow) {
     __shared__ uint32_t s_stage[4][kWaveStage][64];
     const uint32_t lane = threadIdx.x & 63u;
@@ -651,6 +664,27 @@ spmm_minplus_wave_kernel(uint32_t n_rows, uint32_t num_tiles, const uint32_t* __
     // row that ends before lim was parked in the next LDS slot; the one that ends at lim is the open
     // minimum; an empty row (or every row when all_inf) is unreached.
     auto close_upto = [&](uint32_t lim, bool all_inf) {
+        if (all_inf) {
+            // Every row closing here is unreached: nothing to relax and nothing to record (mask_next is
+            // cleared before the launch), so whole runs of rows are stepped over through the row-end
+            // window; only a deferred first row leaves its (unreached) head.
+            if (r == r0 && deferred && re <= lim) {
+                recs[tile].head[lane] = MinPlusU32::acc_t{kInf};
+                rs = re;
+                ++r;
+                re = r < r_last ? row_end(r) : kInf;
+            }
+            while (re <= lim) {
+                const uint32_t off = r - rbase;  // < 64: row_end(r) refilled the window
+                const uint32_t run = static_cast<uint32_t>(__popcll(ballot(lane >= off && rwin <= lim)));
+                uint32_t last = r + (run ? run : 1u) - 1u;
+                if (last >= r_last) last = r_last - 1u;
+                rs = row_end(last);
+                r = last + 1u;
+                re = r < r_last ? row_end(r) : kInf;
+            }
+            return;
+        }
         uint32_t k = 0;  // next LDS slot
         while (re <= lim) {
             uint32_t rid[kCloseBatch], src[kCloseBatch], slot[kCloseBatch];  // src: 0 none, 1 unreached, 2 slot, 3 open
@@ -698,7 +732,7 @@ spmm_minplus_wave_kernel(uint32_t n_rows, uint32_t num_tiles, const uint32_t* __
                     else D[static_cast<size_t>(rid[b]) * 64 + lane] = static_cast<DT>(val);
                 }
                 const uint64_t bits = ballot(improved);
-                if (lane == 0) mask_next[rid[b]] = bits;
+                if (lane == 0 && bits) mask_next[rid[b]] = bits;  // cleared before the launch
                 counted |= bits;
             }
         }
@@ -711,24 +745,20 @@ spmm_minplus_wave_kernel(uint32_t n_rows, uint32_t num_tiles, const uint32_t* __
             const uint32_t q = c * 64u + lane;
             return q >= q0 && q < q1;
         };
-        uint32_t col0 = col[c_first * 64u + lane], w0 = kWeighted ? vals[c_first * 64u + lane] : 1u;
-        uint32_t col1 = 0, w1 = 0;
-        if (c_first + 1 < c_end) {
-            col1 = col[(c_first + 1) * 64u + lane];
-            w1 = kWeighted ? vals[(c_first + 1) * 64u + lane] : 1u;
-        }
-        unsigned long long m0 = in_tile(c_first) ? mask_prev[col0] : 0ull;
+        // Source activity comes from a bitmap of the vertices that changed last iteration (one bit per
+        // vertex, small enough to stay in L2): the word of each lane's source, tested at use.
+        uint32_t col0 = col[c_first * 64u + lane];
+        uint32_t col1 = 0;
+        if (c_first + 1 < c_end) col1 = col[(c_first + 1) * 64u + lane];
+        uint32_t a0 = in_tile(c_first) ? act_prev[col0 >> 5] : 0u;
         for (uint32_t c = c_first; c < c_end; ++c) {
-            const uint64_t act = ballot(m0 != 0ull);
-            // Prefetch, in this order (vector loads complete in order): masks of chunk c + 1, indices of
-            // chunk c + 2; both finish before this chunk's distance loads.
-            unsigned long long m1 = 0;
-            if (c + 1 < c_end && in_tile(c + 1)) m1 = mask_prev[col1];
-            uint32_t col2 = 0, w2 = 0;
-            if (c + 2 < c_end) {
-                col2 = col[(c + 2) * 64u + lane];
-                w2 = kWeighted ? vals[(c + 2) * 64u + lane] : 1u;
-            }
+            const uint64_t act = ballot(((a0 >> (col0 & 31u)) & 1u) != 0u);
+            // Prefetch, in this order (vector loads complete in order): activity of chunk c + 1, indices
+            // of chunk c + 2; both finish before this chunk's distance loads.
+            uint32_t a1 = 0;
+            if (c + 1 < c_end && in_tile(c + 1)) a1 = act_prev[col1 >> 5];
+            uint32_t col2 = 0;
+            if (c + 2 < c_end) col2 = col[(c + 2) * 64u + lane];
             const unsigned long long bnd = starts[c];
             const uint32_t lo = (q0 > c * 64u ? q0 - c * 64u : 0u);
             const uint32_t hi = (q1 - c * 64u < 64u ? q1 - c * 64u : 64u);
@@ -740,6 +770,7 @@ spmm_minplus_wave_kernel(uint32_t n_rows, uint32_t num_tiles, const uint32_t* __
                 // column of an active source is relaxed: a column whose distance did not change last
                 // iteration was relaxed through this edge when it last changed, so it cannot improve
                 // the row now, and the result (a fixed point of min) is the same.
+                const uint32_t w0 = kWeighted ? vals[c * 64u + lane] : 1u;  // weights: active chunks only
                 uint32_t x[64];
                 const uint32_t lane_off = lane * static_cast<uint32_t>(sizeof(DT));
 #pragma unroll
@@ -795,10 +826,8 @@ spmm_minplus_wave_kernel(uint32_t n_rows, uint32_t num_tiles, const uint32_t* __
                 }
             }
             col0 = col1;
-            w0 = w1;
-            m0 = m1;
+            a0 = a1;
             col1 = col2;
-            w1 = w2;
         }
     }
     recs[tile].carry[lane] = MinPlusU32::acc_t{acc};
