@@ -101,6 +101,29 @@ def synchronize(stream=None) -> None:
     call("g1_stream_synchronize", stream)
 
 
+class Stream:
+    """A non-blocking HIP stream of the library (g1_stream_create); pass it as `stream=` to the engines."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        call("g1_stream_create", C.byref(h))
+        self._as_parameter_ = h
+
+    def synchronize(self) -> None:
+        call("g1_stream_synchronize", self._as_parameter_)
+
+    def close(self) -> None:
+        if self._as_parameter_ and self._as_parameter_.value:
+            lib().g1_stream_destroy(self._as_parameter_)
+            self._as_parameter_ = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def device_info() -> dict:
     count, cus, hbm = C.c_int32(), C.c_int32(), C.c_uint64()
     name = C.create_string_buffer(64)
@@ -343,6 +366,17 @@ class Graph:
              C.byref(st), stream)
         vals = r.numpy()[: src.size * self.n].reshape(src.size, self.n) if to_host else r
         return RunResult(vals, st.as_dict())
+
+    def view(self) -> "Graph":
+        """A second handle on this resident graph (g1_graph_create_view) with its own work areas, for runs
+        on another stream at the same time. Close it before this graph."""
+        v = Graph.__new__(Graph)
+        h = C.c_void_p()
+        call("g1_graph_create_view", self._h, C.byref(h))
+        v._h = h
+        v.n, v.m, v.weighted = self.n, self.m, self.weighted
+        v._base = self  # keeps the base alive while the view is
+        return v
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -89,6 +89,16 @@ PushQueues make_queues(g1_graph* g, uint32_t k) {
     return pq;
 }
 
+// The push SSSP engine switches an iteration to a pull step when its frontier has at least
+// G1_SSSP_PULL (default 0.8) times the graph's edges; G1_SSSP_PULL=0 disables the switch.
+double sssp_pull_fraction() {
+    static const double f = [] {
+        const char* e = getenv("G1_SSSP_PULL");
+        return e ? atof(e) : 0.8;
+    }();
+    return f;
+}
+
 // Reads the kSegments counters of counter set `set` and returns (E << 32) | nf of that frontier.
 g1_status read_frontier(g1_graph* g, uint32_t set, hipStream_t s, unsigned long long* packed) {
     G1_HIP(hipMemcpyAsync(g->h_ctr + 16, g->qcnt + set * kSegments, kSegments * sizeof(unsigned long long),
@@ -130,11 +140,17 @@ g1_status check_overflow(g1_graph* g, hipStream_t s) {
 // on the device and is a no-op once the frontier is empty, so the host enqueues iterations in batches
 // (2, 4, 8, ... up to 32) and reads back only the output counters of the last launch of a batch.
 // With profiling on, the batch is one launch so each launch's counts can be attributed to it.
-template <typename Launch, typename AfterLaunch, typename Small>
+//
+// With pull_min_edges > 0 an iteration whose frontier has at least that many out-edges runs as one pull
+// step instead (pull(it): scatter, SpMV, apply). The host then needs each frontier's edge count before
+// its iteration, so while frontiers are near that size (or not yet known) iterations go one at a time.
+template <typename Launch, typename AfterLaunch, typename Small, typename Pull>
 g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& after, Small&& small,
-                     bool small_first, uint32_t* launched) {
+                     bool small_first, uint32_t* launched, Pull&& pull, uint64_t pull_min_edges) {
     uint32_t it = 0, batch = 2;
     bool try_small = small_first;
+    bool known = false;    // next_edges holds the edge count of the frontier of iteration `it`
+    uint64_t next_edges = 0;
     for (;;) {
         // A frontier whose out-edges fit the single-workgroup kernel runs there (see push_small_kernel)
         // until it grows past it or ends; profiling keeps one launch per iteration instead.
@@ -144,22 +160,39 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
             it = res[0];
             if (res[1] == 0) break;
             batch = 2;
-        }
-        const uint32_t b = g->profile ? 1u : batch;
-        G1_CHECK(it + b <= kHistIters, G1_ERR_INTERNAL, "push engine exceeded %u iterations", kHistIters);
-        for (uint32_t k = 0; k < b; ++k) {
-            G1_TRY(launch(it));
-            G1_TRY(after(it));
-            ++it;
+            known = false;
         }
         unsigned long long next = 0;
+        if (pull_min_edges && known && next_edges >= pull_min_edges) {
+            G1_CHECK(it + 1 <= kHistIters, G1_ERR_INTERNAL, "push engine exceeded %u iterations", kHistIters);
+            G1_TRY(pull(it));
+            ++it;
+        } else {
+            uint32_t b = g->profile ? 1u : batch;
+            if (pull_min_edges && (!known || next_edges >= pull_min_edges / 4)) b = 1;
+            G1_CHECK(it + b <= kHistIters, G1_ERR_INTERNAL, "push engine exceeded %u iterations", kHistIters);
+            for (uint32_t k = 0; k < b; ++k) {
+                G1_TRY(launch(it));
+                G1_TRY(after(it));
+                ++it;
+            }
+            if (batch < 32) batch *= 2;
+        }
         G1_TRY(read_frontier(g, it % 3, s, &next));
         if (static_cast<uint32_t>(next) == 0) break;
-        try_small = (next >> 32) <= kSmallCap;
-        if (batch < 32) batch *= 2;
+        next_edges = next >> 32;
+        known = true;
+        try_small = next_edges <= kSmallCap;
     }
     *launched = it;
     return G1_OK;
+}
+
+template <typename Launch, typename AfterLaunch, typename Small>
+g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& after, Small&& small,
+                     bool small_first, uint32_t* launched) {
+    auto no_pull = [](uint32_t) -> g1_status { return G1_OK; };
+    return drive_push(g, s, launch, after, small, small_first, launched, no_pull, 0);
 }
 
 // Runs push_small_kernel from iteration it0 and reads its result: res[0] the next iteration, res[1] 1
@@ -509,8 +542,28 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
         SsspPush pol{g->dstamp, 0};
         return run_small(g, s, pol, g->w, nullptr, 0, it0, res);
     };
+    // A frontier with at least half the graph's edges runs as a pull step (sssp_pull_scatter_kernel):
+    // one streaming pass over the in-edge CSR costs less than that many random relaxations.
+    const bool can_pull = g->has_t && g->t_w != nullptr && sssp_pull_fraction() > 0.0;
+    if (can_pull) G1_TRY(ensure_pull_work(g, false));
+    uint32_t pulls = 0;
+    auto pull = [&](uint32_t it) -> g1_status {
+        PushQueues pq = make_queues(g, it);
+        G1_TRY(g1_fill_u32(g->xu, kInf, n, reinterpret_cast<g1_stream_t>(s)));
+        tok = prof_begin(g, "sssp_pull", s);
+        sssp_pull_scatter_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(pq, g->dstamp, g->xu, it, g->hist);
+        G1_LAUNCHED();
+        G1_TRY(launch_spmv_merge<MinPlusU32>(n, g->m, g->t_row_off, g->t_col, g->t_w, g->xu, g->yu, g->t_ws, s));
+        sssp_pull_apply_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->yu, g->dstamp, it + 1, pq);
+        G1_LAUNCHED();
+        // scatter 8n + SpMV 4(n+1) + 8m + 4n (x) + 4n (y) + apply 4n (y) + 16n (dist word) + 8n (degrees)
+        prof_end(g, tok, s, 4.0 * (n + 1.0) + 8.0 * g->m + 44.0 * n);
+        ++pulls;
+        return G1_OK;
+    };
     uint32_t launched = 0;
-    G1_TRY(drive_push(g, s, launch, after, small, g->max_out_deg <= kSmallCap, &launched));
+    G1_TRY(drive_push(g, s, launch, after, small, g->max_out_deg <= kSmallCap, &launched, pull,
+                      can_pull ? std::max<uint64_t>(static_cast<uint64_t>(sssp_pull_fraction() * g->m), 1) : 0));
     dstamp_extract_kernel<<<fill_blocks, 256, 0, s>>>(n, g->dstamp, dist);
     G1_LAUNCHED();
     G1_HIP(hipEventRecord(g->ev[1], s));
@@ -519,6 +572,8 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     G1_TRY(fold_slots(g, s, true));
     G1_TRY(check_overflow(g, s));
     G1_TRY(read_history(g, launched, s, st, true));
+    st->pull_iterations += pulls;
+    st->push_iterations -= std::min(st->push_iterations, pulls);
     st->frontier_sum = 1 + g->h_ctr[3];
     return finish_stats(g, dist, st, s);
 }

@@ -274,8 +274,7 @@ g1_status g1_graph_create_view(g1_graph* base, g1_graph** out) {
     g->t_wf = base->t_wf;
     g->has_t = base->has_t;
     g->t_alias = base->t_alias;
-    g->t_ws = base->t_ws;
-    g->t_ws_bytes = base->t_ws_bytes;
+    g->t_ws_bytes = base->t_ws_bytes;  // own copy below: the workspace also holds per-call tile records
     g->t_starts = base->t_starts;
     g->osum = base->osum;
     g->best_k[0] = base->best_k[0];
@@ -286,6 +285,13 @@ g1_status g1_graph_create_view(g1_graph* base, g1_graph** out) {
         return s;
     };
     g1_status s;
+    if (base->t_ws) {
+        // The merge-path workspace holds the partition (read-only) and the tile records every SpMV writes:
+        // concurrent runs on two handles need their own records, so the view copies the whole workspace.
+        if ((s = graph_alloc(g, &g->t_ws, g->t_ws_bytes)) != G1_OK) return fail(s);
+        if (hipMemcpy(g->t_ws, base->t_ws, g->t_ws_bytes, hipMemcpyDeviceToDevice) != hipSuccess)
+            return fail(set_error(G1_ERR_HIP, "g1_graph_create_view: workspace copy failed"));
+    }
     if ((s = alloc_n(g, &g->ctr, 16)) != G1_OK) return fail(s);
     if ((s = alloc_n(g, &g->partials, kPartials + 8)) != G1_OK) return fail(s);
     if (hipHostMalloc(reinterpret_cast<void**>(&g->h_ctr), 256 * sizeof(unsigned long long)) != hipSuccess ||

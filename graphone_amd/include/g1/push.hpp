@@ -358,6 +358,67 @@ struct BfsPush {
     __device__ __forceinline__ uint32_t fresh_source_value(uint32_t) const { return 0u; }
 };
 
+// SSSP pull step, for iterations whose frontier has too many edges for the push kernel to pay: the
+// frontier's distances are scattered into a vector x (kInf elsewhere), one min-plus SpMV over the in-edge
+// CSR gives y[v] = min over in-edges (u, v) of x[u] + w, and the apply relaxes and enqueues every vertex
+// that improved. One iteration of the same relaxation the push kernel does (the frontier's out-edges),
+// in one pass over the matrix instead of random atomics per edge.
+//
+// Scatter: x[u] = dist(u) for every entry of the input frontier (x holds kInf on entry). Also the push
+// prologue's bookkeeping: hist[2*it] = (E << 32) | nf and the counters of the round after next cleared.
+static __global__ void __launch_bounds__(kPushThreads)
+sssp_pull_scatter_kernel(PushQueues pq, const unsigned long long* __restrict__ dstamp, uint32_t* __restrict__ x,
+                         uint32_t it, unsigned long long* __restrict__ hist) {
+    __shared__ uint32_t s_vb[kSegments + 1];
+    __shared__ uint32_t s_eb[kSegments + 1];
+    push_prologue(pq, s_vb, s_eb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        hist[2 * it] = (static_cast<unsigned long long>(s_eb[kSegments]) << 32) | s_vb[kSegments];
+        hist[2 * it + 1] = 0ull;
+    }
+    const uint32_t nf = s_vb[kSegments];
+    for (uint32_t i = blockIdx.x * kPushThreads + threadIdx.x; i < nf; i += gridDim.x * kPushThreads) {
+        const uint32_t sg = search_last_le(s_vb, kSegments, i);
+        const uint32_t u = pq.q_in[static_cast<size_t>(sg) * pq.cap + (i - s_vb[sg])];
+        x[u] = static_cast<uint32_t>(dstamp[u] >> 32);
+    }
+}
+
+// Apply: every vertex whose y beats its distance takes (y, stamp) and, with out-edges, joins the next
+// frontier (segment blockIdx % kSegments of the output queue). Each vertex has one owner, so no atomics.
+static __global__ void __launch_bounds__(kPushThreads)
+sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ y,
+                       unsigned long long* __restrict__ dstamp, uint32_t stamp, PushQueues pq) {
+    __shared__ unsigned long long s_scan[kPushThreads / kWave];
+    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_red[kPushThreads / kWave];
+    const uint32_t groups = (n + kPushThreads - 1) / kPushThreads;
+    const uint32_t seg_out = blockIdx.x % kSegments;
+    uint32_t activated = 0;
+    for (uint32_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform per workgroup
+        const uint32_t v = grp * kPushThreads + threadIdx.x;
+        uint32_t cand[1] = {kInf}, cdeg[1] = {0};
+        uint32_t my_cnt = 0, my_deg = 0;
+        if (v < n) {
+            const uint32_t yv = y[v];
+            if (yv < static_cast<uint32_t>(dstamp[v] >> 32)) {
+                dstamp[v] = (static_cast<unsigned long long>(yv) << 32) | stamp;
+                ++activated;
+                const uint32_t dg = row_off[v + 1] - row_off[v];
+                if (dg > 0) {
+                    cand[0] = v;
+                    cdeg[0] = dg;
+                    my_cnt = 1;
+                    my_deg = dg;
+                }
+            }
+        }
+        push_enqueue(cand, cdeg, my_cnt, my_deg, pq, seg_out, s_scan, &s_base);
+        __syncthreads();
+    }
+    block_stat_add(pq.stats + blockIdx.x, activated, s_red);
+}
+
 // BFS bottom-up step over the in-edge CSR: every unvisited vertex looks for an in-neighbour in the
 // current frontier bitmap and stops at the first one. The next frontier is written both as a bitmap
 // (every word fully written by one wave, so it needs no clearing) and as a queue for a following

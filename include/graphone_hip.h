@@ -162,7 +162,9 @@ g1_status g1_graph_create(g1_graph** graph, const g1_host_csr* csr, uint32_t fla
 g1_status g1_graph_destroy(g1_graph* graph);
 /* A second handle on the same resident graph with its own work areas: runs on a view and runs on the base
  * (or on other views) may proceed at the same time on different streams, where runs on one handle are
- * serialized. The view shares the base's device arrays (no copy) and must be destroyed before the base.
+ * serialized. The view shares the base's CSR arrays (no copy), copies the merge-path workspace (partition
+ * and per-call tile records, about 16 bytes per 2048 nonzeros) and owns its work areas; it must be
+ * destroyed before the base.
  * No counterpart in the reference (one graph, one engine per process). */
 g1_status g1_graph_create_view(g1_graph* base, g1_graph** view);
 g1_status g1_graph_info(const g1_graph* graph, uint32_t* n, uint32_t* m, uint64_t* device_bytes);

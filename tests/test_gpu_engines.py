@@ -63,7 +63,10 @@ def test_sssp_engines_match_restated_engines(graphs):
     name, h, g = graphs[0]
     src = int(np.argmax(h.out_degrees()))
     d_adv, _ = oracle.sssp_advance(h.row_offsets, h.col_indices, h.weights, src)
-    assert np.array_equal(g.sssp(src, "advance").values, d_adv)
+    adv = g.sssp(src, "advance")
+    assert np.array_equal(adv.values, d_adv)
+    # from the hub, the densest frontier holds most of the graph's edges: it runs as a pull step
+    assert adv.stats["pull_iterations"] >= 1, adv.stats
     t = h.transpose()
     d_blas, st = oracle.minplus_blas(t.row_offsets, t.col_indices, t.weights, src)
     got = g.sssp(src, "merge")
@@ -147,3 +150,38 @@ def test_pagerank_rmat20_l1(gpu):
     diff = np.abs(got.values.astype(np.float64) - want)
     assert float(diff.sum()) <= 1e-5, (float(diff.sum()), float(diff.max()))
     assert abs(got.stats["iterations"] - ost["iterations"]) <= 1
+
+
+def test_views_run_concurrently(graphs):
+    """Two views of one graph run SSSP (with its pull steps) and personalized PageRank on their own
+    streams from two threads at once; every result equals the serial run (views own their work areas,
+    the SpMV tile records included)."""
+    import threading
+    import graphone_amd as g1
+    name, h, g = graphs[0]
+    srcs = [int(np.argmax(h.out_degrees())), 1, 7, 42]
+    want_d = [g.sssp(s).values for s in srcs]
+    want_r = [g.ppr(s, 0.15, 30, 1e-7).values for s in srcs]
+    views = [g.view(), g.view()]
+    streams = [g1.Stream(), g1.Stream()]
+    got = [[], []]
+
+    def work(i):
+        for _ in range(3):
+            for s in srcs:
+                if i == 0:
+                    got[i].append(views[i].sssp(s, stream=streams[i]).values)
+                else:
+                    got[i].append(views[i].ppr(s, 0.15, 30, 1e-7, stream=streams[i]).values)
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    for k, d in enumerate(got[0]):
+        assert np.array_equal(d, want_d[k % len(srcs)]), k
+    for k, r in enumerate(got[1]):
+        assert np.array_equal(r.view(np.uint32), want_r[k % len(srcs)].view(np.uint32)), k
+    for v in views:
+        v.close()
