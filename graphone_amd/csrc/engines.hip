@@ -542,21 +542,21 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
         SsspPush pol{g->dstamp, 0};
         return run_small(g, s, pol, g->w, nullptr, 0, it0, res);
     };
-    // A frontier with at least half the graph's edges runs as a pull step (sssp_pull_scatter_kernel):
+    // A dense frontier (G1_SSSP_PULL x m edges, 0.8 by default) runs as a pull step (sssp_pull_x_kernel):
     // one streaming pass over the in-edge CSR costs less than that many random relaxations.
     const bool can_pull = g->has_t && g->t_w != nullptr && sssp_pull_fraction() > 0.0;
     if (can_pull) G1_TRY(ensure_pull_work(g, false));
     uint32_t pulls = 0;
     auto pull = [&](uint32_t it) -> g1_status {
         PushQueues pq = make_queues(g, it);
-        G1_TRY(g1_fill_u32(g->xu, kInf, n, reinterpret_cast<g1_stream_t>(s)));
         tok = prof_begin(g, "sssp_pull", s);
-        sssp_pull_scatter_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(pq, g->dstamp, g->xu, it, g->hist);
+        sssp_pull_x_kernel<<<grid_for(n, kPushThreads, kMaxPushBlocks), kPushThreads, 0, s>>>(n, g->dstamp, it, g->xu,
+                                                                                           pq, g->hist);
         G1_LAUNCHED();
         G1_TRY(launch_spmv_merge<MinPlusU32>(n, g->m, g->t_row_off, g->t_col, g->t_w, g->xu, g->yu, g->t_ws, s));
         sssp_pull_apply_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->yu, g->dstamp, it + 1, pq);
         G1_LAUNCHED();
-        // scatter 8n + SpMV 4(n+1) + 8m + 4n (x) + 4n (y) + apply 4n (y) + 16n (dist word) + 8n (degrees)
+        // x 12n + SpMV 4(n+1) + 8m + 4n (x) + 4n (y) + apply 4n (y) + 16n (dist word) + 8n (degrees)
         prof_end(g, tok, s, 4.0 * (n + 1.0) + 8.0 * g->m + 44.0 * n);
         ++pulls;
         return G1_OK;

@@ -364,23 +364,26 @@ struct BfsPush {
 // that improved. One iteration of the same relaxation the push kernel does (the frontier's out-edges),
 // in one pass over the matrix instead of random atomics per edge.
 //
-// Scatter: x[u] = dist(u) for every entry of the input frontier (x holds kInf on entry). Also the push
-// prologue's bookkeeping: hist[2*it] = (E << 32) | nf and the counters of the round after next cleared.
+// x: the input frontier of iteration `it` is the set of vertices whose stamp is `it` (improved during
+// iteration it - 1; every push and pull step stamps its winners it + 1), so x[v] = dist(v) for those and
+// kInf for the rest, in one streaming pass (vertices without out-edges are never read through x). Block 0
+// also does the push prologue's bookkeeping: hist[2*it] = (E << 32) | nf of the input frontier, and the
+// counters of the round after next cleared.
 static __global__ void __launch_bounds__(kPushThreads)
-sssp_pull_scatter_kernel(PushQueues pq, const unsigned long long* __restrict__ dstamp, uint32_t* __restrict__ x,
-                         uint32_t it, unsigned long long* __restrict__ hist) {
-    __shared__ uint32_t s_vb[kSegments + 1];
-    __shared__ uint32_t s_eb[kSegments + 1];
-    push_prologue(pq, s_vb, s_eb);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        hist[2 * it] = (static_cast<unsigned long long>(s_eb[kSegments]) << 32) | s_vb[kSegments];
-        hist[2 * it + 1] = 0ull;
+sssp_pull_x_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, uint32_t it, uint32_t* __restrict__ x,
+                   PushQueues pq, unsigned long long* __restrict__ hist) {
+    if (blockIdx.x == 0) {
+        __shared__ uint32_t s_vb[kSegments + 1];
+        __shared__ uint32_t s_eb[kSegments + 1];
+        push_prologue(pq, s_vb, s_eb);
+        if (threadIdx.x == 0) {
+            hist[2 * it] = (static_cast<unsigned long long>(s_eb[kSegments]) << 32) | s_vb[kSegments];
+            hist[2 * it + 1] = 0ull;
+        }
     }
-    const uint32_t nf = s_vb[kSegments];
-    for (uint32_t i = blockIdx.x * kPushThreads + threadIdx.x; i < nf; i += gridDim.x * kPushThreads) {
-        const uint32_t sg = search_last_le(s_vb, kSegments, i);
-        const uint32_t u = pq.q_in[static_cast<size_t>(sg) * pq.cap + (i - s_vb[sg])];
-        x[u] = static_cast<uint32_t>(dstamp[u] >> 32);
+    for (uint32_t v = blockIdx.x * kPushThreads + threadIdx.x; v < n; v += gridDim.x * kPushThreads) {
+        const unsigned long long ds = dstamp[v];
+        x[v] = static_cast<uint32_t>(ds) == it ? static_cast<uint32_t>(ds >> 32) : kInf;
     }
 }
 
