@@ -220,6 +220,52 @@ __global__ void __launch_bounds__(256) batch_columns_out_kernel(uint32_t n, cons
     }
 }
 
+// k = 64 plus-times SpMM on the wavefront kernel unless G1_PPR_WAVE=0 (same bits either way).
+bool ppr_wave() {
+    static const bool on = [] {
+        const char* e = getenv("G1_PPR_WAVE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// One plus-times SpMM iteration: Y = A^T (x) X over the in-edges (k columns, Kahan sums in the fold order
+// of the single-source SpMV), then the fixup of rows that cross tiles.
+template <int K>
+g1_status launch_plustimes(g1_graph* g, const float* X, float* Y, void* recs_v, hipStream_t s) {
+    const uint32_t tiles = merge_num_tiles(g->n, g->m);
+    PlusTimesBatchPolicy<K> pol{X, Y};
+    auto* recs = static_cast<SpmmTileRec<PlusTimesKahanF32, K>*>(recs_v);
+    const MergeCoord* coords = merge_coords(g->t_ws);
+    const float* vals = g->t_wf;
+    bool done = false;
+    if constexpr (K == 64) {
+        if (ppr_wave()) {
+            const uint32_t blocks = (tiles + 3) / 4;
+            if (vals)
+                spmm_plustimes_wave_kernel<true><<<blocks, 256, 0, s>>>(g->n, tiles, g->t_row_off, g->t_col, vals,
+                                                                        coords, recs, X, Y);
+            else
+                spmm_plustimes_wave_kernel<false><<<blocks, 256, 0, s>>>(g->n, tiles, g->t_row_off, g->t_col, vals,
+                                                                         coords, recs, X, Y);
+            G1_LAUNCHED();
+            done = true;
+        }
+    }
+    if (!done) {
+        spmm_tile_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<tiles, 256, 0, s>>>(
+            g->n, g->m, g->t_row_off, g->t_col, vals, coords, recs, pol, nullptr);
+        G1_LAUNCHED();
+    }
+    if (tiles > 1) {
+        const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
+        spmm_fixup_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<blocks, 256, 0, s>>>(tiles, coords, recs,
+                                                                                              pol, nullptr);
+        G1_LAUNCHED();
+    }
+    return G1_OK;
+}
+
 // One min-plus SpMM iteration over the in-edges: D relaxed in place, change masks prev -> next, the
 // columns that changed ORed into *flag; then the fixup of rows that cross tiles. DT is the distance
 // storage (int16_t only at k = 64 with the wavefront kernel; overflow is then raised instead of storing
@@ -366,9 +412,7 @@ template <int K>
 g1_status ppr_tile_run(g1_graph* g, BatchWork* b, const uint32_t* seeds_host, uint32_t cols, float restart,
                        uint32_t max_iter, double tol, float* out, uint32_t* iterations, hipStream_t s) {
     const uint32_t n = g->n;
-    const uint32_t tiles = merge_num_tiles(g->n, g->m);
     const uint32_t nb = rank_blocks(n);
-    const float* vals = g->t_wf;
     std::vector<uint32_t> seeds(K, kInf);
     for (uint32_t j = 0; j < cols; ++j) seeds[j] = seeds_host[j];
     G1_HIP(hipMemcpyAsync(g->seeds_dev, seeds.data(), K * sizeof(uint32_t), hipMemcpyHostToDevice, s));
@@ -384,19 +428,9 @@ g1_status ppr_tile_run(g1_graph* g, BatchWork* b, const uint32_t* seeds_host, ui
     std::vector<double> deltas(K);
     int cur = 0;
     uint32_t it = 0;
-    auto* recs = static_cast<SpmmTileRec<PlusTimesKahanF32, K>*>(b->recs);
     const unsigned long long all = (K == 64) ? ~0ull : ((1ull << K) - 1ull);
     while (frozen != all && it < max_iter) {
-        PlusTimesBatchPolicy<K> pol{b->X, b->Y};
-        spmm_tile_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<tiles, 256, 0, s>>>(
-            g->n, g->m, g->t_row_off, g->t_col, vals, merge_coords(g->t_ws), recs, pol, nullptr);
-        G1_LAUNCHED();
-        if (tiles > 1) {
-            const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
-            spmm_fixup_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<blocks, 256, 0, s>>>(
-                tiles, merge_coords(g->t_ws), recs, pol, nullptr);
-            G1_LAUNCHED();
-        }
+        G1_TRY(launch_plustimes<K>(g, b->X, b->Y, b->recs, s));
         rank_apply_batch_kernel<K><<<(nb + 256 / K - 1) / (256 / K), 256, 0, s>>>(
             n, b->Y, 0.0f, g->seeds_dev, restart, g->iw, b->R[cur], b->R[cur ^ 1], b->X, g->rpart, nb, frozen);
         G1_LAUNCHED();
@@ -661,20 +695,9 @@ template <int K>
 g1_status session_step_ppr(Session* ss, uint64_t* done) {
     g1_graph* g = ss->g;
     const uint32_t n = g->n;
-    const uint32_t tiles = merge_num_tiles(g->n, g->m);
     const uint32_t nb = rank_blocks(n);
     const hipStream_t s = ss->s;
-    PlusTimesBatchPolicy<K> pol{ss->X, ss->Y};
-    auto* recs = static_cast<SpmmTileRec<PlusTimesKahanF32, K>*>(ss->recs);
-    spmm_tile_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<tiles, 256, 0, s>>>(
-        g->n, g->m, g->t_row_off, g->t_col, g->t_wf, merge_coords(g->t_ws), recs, pol, nullptr);
-    G1_LAUNCHED();
-    if (tiles > 1) {
-        const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
-        spmm_fixup_kernel<PlusTimesKahanF32, K, PlusTimesBatchPolicy<K>><<<blocks, 256, 0, s>>>(
-            tiles, merge_coords(g->t_ws), recs, pol, nullptr);
-        G1_LAUNCHED();
-    }
+    G1_TRY(launch_plustimes<K>(g, ss->X, ss->Y, ss->recs, s));
     const unsigned long long live = ss->active & ~ss->converged;
     rank_apply_batch_kernel<K><<<(nb + 256 / K - 1) / (256 / K), 256, 0, s>>>(
         n, ss->Y, 0.0f, ss->seeds_dev, ss->restart, ss->iw, ss->R[ss->rcur], ss->R[ss->rcur ^ 1], ss->X, ss->part, nb,

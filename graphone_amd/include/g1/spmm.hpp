@@ -917,4 +917,144 @@ struct PlusTimesBatchPolicy {
     }
 };
 
+// ------------------------------------------------------------------------------------------------
+// Batched plus-times (Kahan) at k = 64 with one wavefront per merge-path tile, lane j = column j, the
+// walk on the scalar unit. It reproduces the fold of spmv_merge_tile_kernel exactly, per column: the
+// tile's 2048 merge steps are cut into the same 256 runs of 8 ("virtual threads"); inside a run a row's
+// products are added left to right; a run that closes a row finishes its first closed row by merging,
+// left to right, the tails of the runs since the last run that closed a row and then its own head; the
+// tile's open row leaves as the carry, a first row that began before the tile as the deferred head
+// (finished by spmm_fixup_kernel). Batched personalized PageRank columns therefore stay bit-identical
+// to the single-source runs.
+//
+// The x rows of 32 nonzeros are loaded at a time (one 256-byte access per source, issued a chunk ahead)
+// and parked in LDS, so the walk indexes them at run time; weights and column indices come with one
+// coalesced load per chunk and are read out of their lanes.
+constexpr uint32_t kPprChunk = 32;
+
+template <bool kWeighted>
+__global__ void __launch_bounds__(256)
+spmm_plustimes_wave_kernel(uint32_t n_rows, uint32_t num_tiles, const uint32_t* __restrict__ row_offsets,
+                           const uint32_t* __restrict__ col, const float* __restrict__ vals,
+                           const MergeCoord* __restrict__ coords,
+                           SpmmTileRec<PlusTimesKahanF32, 64>* __restrict__ recs, const float* __restrict__ X,
+                           float* __restrict__ Y) {
+    using S = PlusTimesKahanF32;
+    using acc_t = S::acc_t;
+    __shared__ float s_x[4][kPprChunk][64];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t tile = blockIdx.x * 4u + wave;
+    if (tile >= num_tiles) return;
+    const uint32_t r0 = coords[tile].x, r_last = coords[tile + 1].x;
+    const uint32_t q0 = coords[tile].y, q1 = coords[tile + 1].y;
+
+    uint32_t rbase = r0;
+    uint32_t rwin = row_offsets[rbase + 1 + lane <= n_rows ? rbase + 1 + lane : n_rows];
+    auto row_end = [&](uint32_t r) -> uint32_t {
+        if (r - rbase >= 64u) {
+            rbase = r;
+            rwin = row_offsets[rbase + 1 + lane <= n_rows ? rbase + 1 + lane : n_rows];
+        }
+        return __builtin_amdgcn_readlane(rwin, r - rbase);
+    };
+
+    uint32_t r = r0;                                    // next row to close
+    uint32_t re = r0 < r_last ? row_end(r0) : kInf;     // its end (kInf: no further row end in the tile)
+    uint32_t d = 0;                                     // merge steps done in the tile
+    acc_t acc = S::identity(), head = S::identity(), chain = S::identity();
+    bool hasend = false, any_end = false;
+    uint32_t first_row = 0;
+
+    // End of a run of 8 steps (the same folds as merge_walk's, per column).
+    auto run_end = [&]() {
+        if (hasend) {
+            const acc_t whole = S::merge(chain, head);
+            if (!any_end && row_offsets[first_row] < q0) {
+                recs[tile].head[lane] = whole;
+            } else {
+                Y[static_cast<size_t>(first_row) * 64 + lane] = S::finalize(whole);
+            }
+            chain = S::merge(S::identity(), acc);
+            any_end = true;
+        } else {
+            chain = S::merge(chain, acc);
+        }
+        acc = S::identity();
+        head = S::identity();
+        hasend = false;
+    };
+    auto step = [&]() {
+        ++d;
+        if ((d & (kMergeItems - 1)) == 0) run_end();
+    };
+    auto close_row = [&]() {
+        if (!hasend) {
+            head = acc;
+            hasend = true;
+            first_row = r;
+        } else {
+            Y[static_cast<size_t>(r) * 64 + lane] = S::finalize(acc);
+        }
+        acc = S::identity();
+        ++r;
+        re = r < r_last ? row_end(r) : kInf;
+        step();
+    };
+
+    if (q0 < q1) {
+        const uint32_t c_first = q0 / kPprChunk, c_end = (q1 + kPprChunk - 1) / kPprChunk;
+        auto load_cw = [&](uint32_t c, uint32_t& cv, uint32_t& wv) {
+            cv = 0;
+            wv = __float_as_uint(1.0f);
+            const uint32_t q = c * kPprChunk + lane;
+            if (lane < kPprChunk && q >= q0 && q < q1) {
+                cv = col[q];
+                if (kWeighted) wv = __float_as_uint(vals[q]);
+            }
+        };
+        float xr[kPprChunk];
+        auto load_x = [&](uint32_t cv) {
+#pragma unroll
+            for (uint32_t t = 0; t < kPprChunk; ++t) {
+                const uint32_t u = __builtin_amdgcn_readlane(cv, t);  // inactive lanes hold source 0
+                xr[t] = X[static_cast<size_t>(u) * 64 + lane];
+            }
+        };
+        uint32_t colA, wA, colB = 0, wB = 0;
+        load_cw(c_first, colA, wA);
+        load_x(colA);
+        if (c_first + 1 < c_end) load_cw(c_first + 1, colB, wB);
+        for (uint32_t c = c_first; c < c_end; ++c) {
+#pragma unroll
+            for (uint32_t t = 0; t < kPprChunk; ++t) s_x[wave][t][lane] = xr[t];
+            if (c + 1 < c_end) load_x(colB);
+            uint32_t colC = 0, wC = 0;
+            if (c + 2 < c_end) load_cw(c + 2, colC, wC);
+            const uint32_t qa = (q0 > c * kPprChunk) ? q0 : c * kPprChunk;
+            const uint32_t qb = (q1 < (c + 1) * kPprChunk) ? q1 : (c + 1) * kPprChunk;
+            for (uint32_t q = qa; q < qb; ++q) {
+                while (re <= q) close_row();
+                const uint32_t t = q - c * kPprChunk;
+                const float w = kWeighted ? __uint_as_float(__builtin_amdgcn_readlane(wA, t)) : 1.0f;
+                S::add(acc, S::combine(w, s_x[wave][t][lane]));
+                step();
+            }
+            wA = wB;
+            colB = colC;
+            wB = wC;
+        }
+    }
+    while (r < r_last) close_row();
+    // The runs after the tile's last step (the last tile only) are empty: their tails still join the chain.
+    if ((d & (kMergeItems - 1)) != 0) run_end();
+    for (uint32_t v = (d + kMergeItems - 1) / kMergeItems; v < static_cast<uint32_t>(kMergeThreads); ++v) run_end();
+    recs[tile].carry[lane] = chain;
+    if (lane == 0) {
+        uint32_t flags = any_end ? kTileHasEnd : 0u;
+        if (any_end && row_offsets[r0] < q0) flags |= kTileDeferred;
+        recs[tile].flags = flags;
+    }
+}
+
 }  // namespace g1
