@@ -8,6 +8,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <immintrin.h>
+
 #include <algorithm>
 #include <vector>
 
@@ -99,17 +101,46 @@ double sssp_pull_fraction() {
     return f;
 }
 
-// Reads the kSegments counters of counter set `set` and returns (E << 32) | nf of that frontier.
-g1_status read_frontier(g1_graph* g, uint32_t set, hipStream_t s, unsigned long long* packed) {
-    G1_HIP(hipMemcpyAsync(g->h_ctr + 16, g->qcnt + set * kSegments, kSegments * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s));
-    G1_HIP(hipStreamSynchronize(s));
-    unsigned long long nf = 0, E = 0;
-    for (int i = 0; i < kSegments; ++i) {
-        nf += static_cast<uint32_t>(g->h_ctr[16 + i]);
-        E += g->h_ctr[16 + i] >> 32;
+// One wave sums the kSegments counters of a frontier and posts nf and E to the graph's coherent pinned
+// mailbox, then the sequence number with a system-scope release store, so the host learns a frontier's
+// size by watching host memory instead of a device-to-host copy plus a stream synchronization.
+__global__ void __launch_bounds__(64) frontier_post_kernel(const unsigned long long* __restrict__ cnt,
+                                                           unsigned long long* mbox, unsigned long long seq) {
+    const unsigned long long c = cnt[lane_id()];
+    const unsigned long long nf = wave_sum<unsigned long long>(c & 0xFFFFFFFFull);
+    const unsigned long long E = wave_sum<unsigned long long>(c >> 32);
+    if (lane_id() == 0) {
+        __hip_atomic_store(mbox + 1, nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(mbox + 2, E, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(mbox, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    G1_CHECK(E < 0xFFFFFFFFull, G1_ERR_INTERNAL, "frontier edge count overflow");
+}
+
+// Returns (E << 32) | nf of the frontier in counter set `set`, once the work enqueued on s before it is
+// done. The host spins on the mailbox; every 1024 polls it asks the stream, and a stream that has drained
+// (or failed) without the post showing is an error.
+g1_status read_frontier(g1_graph* g, uint32_t set, hipStream_t s, unsigned long long* packed) {
+    const unsigned long long seq = ++g->post_seq;
+    frontier_post_kernel<<<1, 64, 0, s>>>(g->qcnt + set * kSegments, g->h_mbox, seq);
+    G1_LAUNCHED();
+    uint32_t polls = 0;
+    while (__atomic_load_n(g->h_mbox, __ATOMIC_ACQUIRE) != seq) {
+        if ((++polls & 1023u) == 0) {
+            const hipError_t q = hipStreamQuery(s);
+            if (q != hipErrorNotReady) {
+                if (__atomic_load_n(g->h_mbox, __ATOMIC_ACQUIRE) == seq) break;
+                G1_HIP(q);
+                G1_HIP(hipStreamSynchronize(s));
+                G1_CHECK(__atomic_load_n(g->h_mbox, __ATOMIC_ACQUIRE) == seq, G1_ERR_INTERNAL,
+                         "frontier post missing after the stream drained");
+                break;
+            }
+        }
+        _mm_pause();
+    }
+    const unsigned long long nf = __atomic_load_n(g->h_mbox + 1, __ATOMIC_ACQUIRE);
+    const unsigned long long E = __atomic_load_n(g->h_mbox + 2, __ATOMIC_ACQUIRE);
+    G1_CHECK(E < 0xFFFFFFFFull && nf <= 0xFFFFFFFFull, G1_ERR_INTERNAL, "frontier edge count overflow");
     *packed = (E << 32) | nf;
     return G1_OK;
 }
@@ -136,20 +167,35 @@ g1_status check_overflow(g1_graph* g, hipStream_t s) {
     return G1_OK;
 }
 
+// Launches to enqueue for a frontier of `edges` out-edges that is already shrinking: enough for it to
+// fall to the single-workgroup kernel's size if each iteration keeps a quarter of the edges (a shrinking
+// RMAT frontier keeps 10-25 %), at most 32. Launches past the frontier's end are no-ops but not free (a
+// few microseconds each, and as long again for the host to enqueue).
+uint32_t tail_batch(uint64_t edges) {
+    uint32_t k = 1;
+    while (edges > kSmallCap && k < 32) {
+        edges /= 4;
+        ++k;
+    }
+    return k;
+}
+
 // Device-driven iteration loop of the push engines. Every launch reads its input frontier's counters
 // on the device and is a no-op once the frontier is empty, so the host enqueues iterations in batches
-// (2, 4, 8, ... up to 32) and reads back only the output counters of the last launch of a batch.
-// With profiling on, the batch is one launch so each launch's counts can be attributed to it.
+// and reads back only the output counters of the last launch of a batch: 2, 4, 8, ... launches while the
+// frontier grows, tail_batch() once it shrinks. With profiling on, the batch is one launch so each
+// launch's counts can be attributed to it.
 //
 // With pull_min_edges > 0 an iteration whose frontier has at least that many out-edges runs as one pull
 // step instead (pull(it): scatter, SpMV, apply). The host then needs each frontier's edge count before
-// its iteration, so while frontiers are near that size (or not yet known) iterations go one at a time.
+// its iteration, so until the frontier shrinks below a quarter of that size iterations go one at a time.
 template <typename Launch, typename AfterLaunch, typename Small, typename Pull>
 g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& after, Small&& small,
                      bool small_first, uint32_t* launched, Pull&& pull, uint64_t pull_min_edges) {
     uint32_t it = 0, batch = 2;
     bool try_small = small_first;
-    bool known = false;    // next_edges holds the edge count of the frontier of iteration `it`
+    bool known = false;      // next_edges holds the edge count of the frontier of iteration `it`
+    bool shrinking = false;  // ... and it is below the count read before it
     uint64_t next_edges = 0;
     for (;;) {
         // A frontier whose out-edges fit the single-workgroup kernel runs there (see push_small_kernel)
@@ -161,6 +207,7 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
             if (res[1] == 0) break;
             batch = 2;
             known = false;
+            shrinking = false;
         }
         unsigned long long next = 0;
         if (pull_min_edges && known && next_edges >= pull_min_edges) {
@@ -168,8 +215,9 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
             G1_TRY(pull(it));
             ++it;
         } else {
-            uint32_t b = g->profile ? 1u : batch;
-            if (pull_min_edges && (!known || next_edges >= pull_min_edges / 4)) b = 1;
+            uint32_t b = shrinking ? tail_batch(next_edges) : batch;
+            if (g->profile) b = 1;
+            if (pull_min_edges && (!known || !shrinking || next_edges >= pull_min_edges / 4)) b = 1;
             G1_CHECK(it + b <= kHistIters, G1_ERR_INTERNAL, "push engine exceeded %u iterations", kHistIters);
             for (uint32_t k = 0; k < b; ++k) {
                 G1_TRY(launch(it));
@@ -180,6 +228,7 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
         }
         G1_TRY(read_frontier(g, it % 3, s, &next));
         if (static_cast<uint32_t>(next) == 0) break;
+        shrinking = known && (next >> 32) < next_edges;
         next_edges = next >> 32;
         known = true;
         try_small = next_edges <= kSmallCap;

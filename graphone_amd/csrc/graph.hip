@@ -160,6 +160,7 @@ static void graph_free_all(g1_graph* g) {
     g->owned.clear();
     if (g->h_ctr) (void)hipHostFree(g->h_ctr);
     if (g->h_scalar) (void)hipHostFree(g->h_scalar);
+    if (g->h_mbox) (void)hipHostFree(g->h_mbox);
     for (auto& e : g->ev)
         if (e) (void)hipEventDestroy(e);
 }
@@ -241,8 +242,11 @@ g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags
     if ((s = alloc_n(g, &g->ctr, 16)) != G1_OK) return fail(s);
     if ((s = alloc_n(g, &g->partials, kPartials + 8)) != G1_OK) return fail(s);
     if (hipHostMalloc(reinterpret_cast<void**>(&g->h_ctr), 256 * sizeof(unsigned long long)) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&g->h_scalar), 8 * sizeof(double)) != hipSuccess)
+        hipHostMalloc(reinterpret_cast<void**>(&g->h_scalar), 8 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&g->h_mbox), 4 * sizeof(unsigned long long), hipHostMallocCoherent) !=
+            hipSuccess)
         return fail(set_error(G1_ERR_HIP, "g1_graph_create: pinned host allocation failed"));
+    memset(g->h_mbox, 0, 4 * sizeof(unsigned long long));
     if (hipEventCreate(&g->ev[0]) != hipSuccess || hipEventCreate(&g->ev[1]) != hipSuccess)
         return fail(set_error(G1_ERR_HIP, "g1_graph_create: event creation failed"));
     if (hipDeviceSynchronize() != hipSuccess) return fail(set_error(G1_ERR_HIP, "g1_graph_create: sync failed"));
@@ -295,8 +299,11 @@ g1_status g1_graph_create_view(g1_graph* base, g1_graph** out) {
     if ((s = alloc_n(g, &g->ctr, 16)) != G1_OK) return fail(s);
     if ((s = alloc_n(g, &g->partials, kPartials + 8)) != G1_OK) return fail(s);
     if (hipHostMalloc(reinterpret_cast<void**>(&g->h_ctr), 256 * sizeof(unsigned long long)) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&g->h_scalar), 8 * sizeof(double)) != hipSuccess)
+        hipHostMalloc(reinterpret_cast<void**>(&g->h_scalar), 8 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&g->h_mbox), 4 * sizeof(unsigned long long), hipHostMallocCoherent) !=
+            hipSuccess)
         return fail(set_error(G1_ERR_HIP, "g1_graph_create_view: pinned host allocation failed"));
+    memset(g->h_mbox, 0, 4 * sizeof(unsigned long long));
     if (hipEventCreate(&g->ev[0]) != hipSuccess || hipEventCreate(&g->ev[1]) != hipSuccess)
         return fail(set_error(G1_ERR_HIP, "g1_graph_create_view: event creation failed"));
     *out = g;

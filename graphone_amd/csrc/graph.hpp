@@ -65,6 +65,8 @@ struct g1_graph {
     uint32_t best_k[2] = {0, 0};                                        // autotuned k: [0] min-plus, [1] PPR
     unsigned long long* h_ctr = nullptr;  // pinned host mirror of ctr (16) and a counter set (at [16])
     double* h_scalar = nullptr;           // pinned host scalar
+    unsigned long long* h_mbox = nullptr;  // coherent pinned mailbox: [0] post sequence, [1] nf, [2] E
+    unsigned long long post_seq = 0;       // last sequence number posted to h_mbox
     hipEvent_t ev[2] = {nullptr, nullptr};
 
     // per-kernel profile (g1_profile_enable / g1_profile_read)
