@@ -424,16 +424,30 @@ g1_status run_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats* st,
     hq->far[0] = g->road_far[0];
     hq->far[1] = g->road_far[1];
     hq->src = src;
-    G1_HIP(hipMemcpyAsync(g->road_q, hq, sizeof(RoadQuery), hipMemcpyHostToDevice, s));
+    const RoadQuery staged = *hq;
     const uint64_t qcap = std::min<uint64_t>(static_cast<uint64_t>(g->seg_cap) * kSegments, 0xFFFFFFFFull);
-    const int tok = prof_begin(g, kSssp ? "road_sssp" : "road_bfs", s);
-    road_query_kernel<<<1, kRoadThreads, 0, s>>>(g->row_off, g->col, g->w, static_cast<RoadQuery*>(g->road_q),
-                                                        road_delta(g), g->road_far_cap, static_cast<uint32_t>(qcap),
-                                                        kHistIters);
-    G1_LAUNCHED();
-    prof_end(g, tok, s, 0.0);
-    G1_HIP(hipMemcpyAsync(hq, g->road_q, sizeof(RoadQuery), hipMemcpyDeviceToHost, s));
-    G1_HIP(hipStreamSynchronize(s));
+    // A small graph's SSSP runs with its distances in LDS (road_lds_sssp_kernel); if that reports status 2
+    // (a distance past 16 bits, or a far-list overflow) it leaves the distance words untouched and the
+    // query runs on the global-memory kernel.
+    int tok = -1;
+    for (int pass = (kSssp && g->n <= kRoadLdsMaxN) ? 0 : 1; pass < 2; ++pass) {
+        *hq = staged;
+        G1_HIP(hipMemcpyAsync(g->road_q, hq, sizeof(RoadQuery), hipMemcpyHostToDevice, s));
+        tok = prof_begin(g, kSssp ? "road_sssp" : "road_bfs", s);
+        RoadQuery* dq = static_cast<RoadQuery*>(g->road_q);
+        if (pass == 0) {
+            road_lds_sssp_kernel<<<1, kRoadThreads, 0, s>>>(g->n, g->row_off, g->col, g->w, dq, road_delta(g),
+                                                            g->road_far_cap, static_cast<uint32_t>(qcap), kHistIters);
+        } else {
+            road_query_kernel<<<1, kRoadThreads, 0, s>>>(g->row_off, g->col, g->w, dq, road_delta(g), g->road_far_cap,
+                                                         static_cast<uint32_t>(qcap), kHistIters);
+        }
+        G1_LAUNCHED();
+        prof_end(g, tok, s, 0.0);
+        G1_HIP(hipMemcpyAsync(hq, g->road_q, sizeof(RoadQuery), hipMemcpyDeviceToHost, s));
+        G1_HIP(hipStreamSynchronize(s));
+        if (hq->status == 0) break;
+    }
     *ok = hq->status == 0;
     if (!*ok) return G1_OK;
     prof_set_bytes(g, tok, 16.0 * static_cast<double>(hq->frontier_sum) + 12.0 * static_cast<double>(hq->edges_sum));

@@ -139,6 +139,20 @@ def test_grid213_sssp_from_vertex_1(gpu):
         assert np.array_equal(g.sssp(1, eng).values, want), eng
 
 
+def test_road_sssp_past_16_bit_distances(gpu):
+    """A small grid (its distances fit the on-chip road kernel's LDS) with weights up to 60000: distances
+    pass 0xFFFF, the on-chip kernel hands the query to the global-memory road kernel, and the result still
+    equals Dijkstra; the same grid with light weights stays on chip."""
+    import graphone_amd as g1
+    for hi in (60000, 9):
+        h = g1.HostGraph.grid(40, 40, 0.7, seed=11).assign_weights(1, hi, seed=12, symmetric=True)
+        want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, 5)
+        if hi > 65535 // 40:
+            assert int(want[want != 0xFFFFFFFF].max()) > 0xFFFF
+        got = g1.Graph(h, symmetric=True).sssp(5, "advance")
+        assert np.array_equal(got.values, want), hi
+
+
 def test_pagerank_rmat20_l1(gpu):
     """PageRank on unweighted RMAT scale 20 (16.1M edges) within L1 1e-5 of the FP64 oracle."""
     import graphone_amd as g1
