@@ -155,18 +155,6 @@ g1_status fold_slots(g1_graph* g, hipStream_t s, bool read) {
     return G1_OK;
 }
 
-g1_status check_overflow(g1_graph* g, hipStream_t s) {
-    uint32_t flag = 0;
-    G1_HIP(hipMemcpyAsync(g->h_ctr + 8, g->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    G1_HIP(hipStreamSynchronize(s));
-    memcpy(&flag, g->h_ctr + 8, sizeof(flag));
-    if (flag != 0) {  // cleared, so the next run on this graph starts clean
-        G1_TRY(g1_fill_u32(g->overflow, 0u, 1, reinterpret_cast<g1_stream_t>(s)));
-        return set_error(G1_ERR_INTERNAL, "frontier queue segment overflow");
-    }
-    return G1_OK;
-}
-
 // Launches to enqueue for a frontier of `edges` out-edges that is already shrinking: enough for it to
 // fall to the single-workgroup kernel's size if each iteration keeps a quarter of the edges (a shrinking
 // RMAT frontier keeps 10-25 %), at most 32. Launches past the frontier's end are no-ops but not free (a
@@ -264,29 +252,42 @@ g1_status run_small(g1_graph* g, hipStream_t s, const P& pol, const uint32_t* wt
 
 // Reads the history of `launched` launches and fills iteration statistics: the iterations that had a
 // non-empty input, their frontier entries and edges.
-g1_status read_history(g1_graph* g, uint32_t launched, hipStream_t s, g1_run_stats* st, bool count_edges) {
-    std::vector<unsigned long long> h(2ull * launched);
-    G1_HIP(hipMemcpyAsync(h.data(), g->hist, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+// The end of a push-engine run (BFS or SSSP) in two launches and one stream synchronization: the reached
+// vertices of the result d are counted into the third and fourth stats-slot regions, then
+// push_finish_kernel folds every slot region and writes the counters, the overflow flag and the
+// iteration history to pinned memory. The run's end event must already be recorded.
+g1_status collect_push_run(g1_graph* g, hipStream_t s, uint32_t launched, const uint32_t* d, g1_run_stats* st,
+                           bool count_edges) {
+    reached_stats_kernel<<<grid_for(g->n, 256, kMaxPushBlocks), 256, 0, s>>>(
+        g->n, d, g->row_off, g->slots + 2 * kMaxPushBlocks, g->slots + 3 * kMaxPushBlocks);
+    G1_LAUNCHED();
+    if (!g->h_hist) {
+        G1_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->h_hist), 2ull * kHistIters * sizeof(unsigned long long)));
+    }
+    push_finish_kernel<<<1, 256, 0, s>>>(g->slots, g->ctr, g->overflow, g->hist, launched, g->h_ctr, g->h_hist);
+    G1_LAUNCHED();
     G1_HIP(hipStreamSynchronize(s));
+    G1_TRY(elapsed(g, st));
+    G1_TRY(prof_flush(g));
+    uint32_t flag = 0;
+    memcpy(&flag, g->h_ctr + 8, sizeof(flag));
+    if (flag != 0) {  // cleared, so the next run on this graph starts clean
+        G1_TRY(g1_fill_u32(g->overflow, 0u, 1, reinterpret_cast<g1_stream_t>(s)));
+        return set_error(G1_ERR_INTERNAL, "frontier queue segment overflow");
+    }
     uint32_t iters = 0;
     for (uint32_t i = 0; i < launched; ++i) {
-        const uint32_t nf = static_cast<uint32_t>(h[2 * i]);
-        if (nf == 0) break;
+        const unsigned long long* h = g->h_hist + 2ull * i;
+        if (static_cast<uint32_t>(h[0]) == 0) break;
         ++iters;
-        const bool pull = (h[2 * i + 1] >> 63) != 0;
+        const bool pull = (h[1] >> 63) != 0;
         if (pull) ++st->pull_iterations; else ++st->push_iterations;
-        if (count_edges && !pull) st->edges_sum += h[2 * i] >> 32;
+        if (count_edges && !pull) st->edges_sum += h[0] >> 32;
     }
     st->iterations = iters;
+    st->reached = g->h_ctr[4];
+    st->reached_edges = g->h_ctr[5];
     return G1_OK;
-}
-
-g1_status seed_push(g1_graph* g, uint32_t src, hipStream_t s) {
-    push_seed_kernel<<<1, 256, 0, s>>>(g->row_off, src, g->q[0], g->qoff[0], g->qcnt);
-    G1_LAUNCHED();
-    clear_u64_kernel<<<1, 64, 0, s>>>(g->ctr, 8);
-    G1_LAUNCHED();
-    return read_ctr(g, s, 0, 8);  // also syncs the host mirror with the cleared counters
 }
 
 g1_status run_bfs_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats* st, hipStream_t s, bool* ok);
@@ -303,15 +304,10 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
     if (diropt) G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "g1_bfs: direction-optimizing BFS needs the transpose");
     const uint32_t n = g->n;
     const uint32_t words = (n + 31) / 32;
-    G1_TRY(g1_fill_u32(depth, kInf, n, s));
-    set_u32_kernel<<<1, 64, 0, s>>>(depth, src, 0u);
+    push_start_kernel<false><<<grid_for(n, 256, kMaxPushBlocks), 256, 0, s>>>(
+        n, src, nullptr, depth, diropt ? g->fb3 : nullptr, words, g->row_off, g->q[0], g->qoff[0], g->qcnt, g->ctr);
     G1_LAUNCHED();
-    if (diropt) {
-        G1_TRY(g1_fill_u32(g->fb3, 0u, 2ull * words, s));
-        set_u32_kernel<<<1, 64, 0, s>>>(g->fb3, src >> 5, 1u << (src & 31u));
-        G1_LAUNCHED();
-    }
-    G1_TRY(seed_push(g, src, s));
+    memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
     G1_HIP(hipEventRecord(g->ev[0], s));
     int tok = -1;
     auto launch = [&](uint32_t it) -> g1_status {
@@ -354,14 +350,10 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
     uint32_t launched = 0;
     G1_TRY(drive_push(g, s, launch, after, small, g->max_out_deg <= kSmallCap, &launched));
     G1_HIP(hipEventRecord(g->ev[1], s));
-    G1_TRY(elapsed(g, st));
-    G1_TRY(prof_flush(g));
-    G1_TRY(fold_slots(g, s, true));
-    G1_TRY(check_overflow(g, s));
-    G1_TRY(read_history(g, launched, s, st, true));
+    G1_TRY(collect_push_run(g, s, launched, depth, st, true));
     st->frontier_sum = 1 + g->h_ctr[3];
     st->edges_sum += g->h_ctr[7];
-    return finish_stats(g, depth, st, s);
+    return G1_OK;
 }
 
 __global__ void dstamp_init_kernel(uint32_t n, uint32_t src, unsigned long long* __restrict__ ds) {
@@ -575,9 +567,10 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
     const uint32_t n = g->n;
     const uint32_t fill_blocks = grid_for(n, 256, kMaxPushBlocks);
-    dstamp_init_kernel<<<fill_blocks, 256, 0, s>>>(n, src, g->dstamp);
+    push_start_kernel<true><<<fill_blocks, 256, 0, s>>>(n, src, g->dstamp, nullptr, nullptr, 0, g->row_off, g->q[0],
+                                                        g->qoff[0], g->qcnt, g->ctr);
     G1_LAUNCHED();
-    G1_TRY(seed_push(g, src, s));
+    memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
     G1_HIP(hipEventRecord(g->ev[0], s));
     int tok = -1;
     auto launch = [&](uint32_t it) -> g1_status {
@@ -630,15 +623,11 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     dstamp_extract_kernel<<<fill_blocks, 256, 0, s>>>(n, g->dstamp, dist);
     G1_LAUNCHED();
     G1_HIP(hipEventRecord(g->ev[1], s));
-    G1_TRY(elapsed(g, st));
-    G1_TRY(prof_flush(g));
-    G1_TRY(fold_slots(g, s, true));
-    G1_TRY(check_overflow(g, s));
-    G1_TRY(read_history(g, launched, s, st, true));
+    G1_TRY(collect_push_run(g, s, launched, dist, st, true));
     st->pull_iterations += pulls;
     st->push_iterations -= std::min(st->push_iterations, pulls);
     st->frontier_sum = 1 + g->h_ctr[3];
-    return finish_stats(g, dist, st, s);
+    return G1_OK;
 }
 
 // ---------------------------------------------------------------- pull engines (SpMV)

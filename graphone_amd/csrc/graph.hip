@@ -50,12 +50,12 @@ g1_status ensure_push_work(g1_graph* g, hipStream_t s) {
     G1_TRY(alloc_n(g, &g->dstamp, n));
     G1_TRY(alloc_n(g, &g->hist, 2ull * kHistIters));
     G1_TRY(alloc_n(g, &g->qcnt, 3 * kSegments));
-    G1_TRY(alloc_n(g, &g->slots, 2 * kMaxPushBlocks));
+    G1_TRY(alloc_n(g, &g->slots, 4 * kMaxPushBlocks));  // four regions of per-workgroup stats
     G1_TRY(alloc_n(g, &g->overflow, 4));
     // Cleared on the caller's stream, ahead of its first launch: a memset on the null stream is not
     // ordered before work on a non-blocking stream, and the allocation may hold an earlier buffer's data.
     const g1_stream_t gs = reinterpret_cast<g1_stream_t>(s);
-    G1_TRY(g1_fill_u32(reinterpret_cast<uint32_t*>(g->slots), 0u, 4ull * kMaxPushBlocks, gs));
+    G1_TRY(g1_fill_u32(reinterpret_cast<uint32_t*>(g->slots), 0u, 8ull * kMaxPushBlocks, gs));
     G1_TRY(g1_fill_u32(g->overflow, 0u, 4, gs));
     return G1_OK;
 }
@@ -161,6 +161,7 @@ static void graph_free_all(g1_graph* g) {
     if (g->h_ctr) (void)hipHostFree(g->h_ctr);
     if (g->h_scalar) (void)hipHostFree(g->h_scalar);
     if (g->h_mbox) (void)hipHostFree(g->h_mbox);
+    if (g->h_hist) (void)hipHostFree(g->h_hist);
     for (auto& e : g->ev)
         if (e) (void)hipEventDestroy(e);
 }

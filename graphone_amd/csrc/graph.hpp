@@ -37,7 +37,7 @@ struct g1_graph {
     uint32_t* stamp = nullptr;
     uint32_t seg_cap = 0;                       // entries per queue segment
     unsigned long long* qcnt = nullptr;         // 3 rotating sets of kSegments queue counters
-    unsigned long long* slots = nullptr;        // per-workgroup stats slots (2 * kMaxPushBlocks)
+    unsigned long long* slots = nullptr;        // per-workgroup stats slots (4 regions of kMaxPushBlocks)
     uint32_t* overflow = nullptr;
     uint32_t* fb[2] = {nullptr, nullptr};
     uint32_t* fb3 = nullptr;                    // three rotating frontier bitmaps (device-driven BFS)
@@ -65,6 +65,7 @@ struct g1_graph {
     uint32_t best_k[2] = {0, 0};                                        // autotuned k: [0] min-plus, [1] PPR
     unsigned long long* h_ctr = nullptr;  // pinned host mirror of ctr (16) and a counter set (at [16])
     double* h_scalar = nullptr;           // pinned host scalar
+    unsigned long long* h_hist = nullptr;  // pinned copy of the push engines' iteration history (lazy)
     unsigned long long* h_mbox = nullptr;  // coherent pinned mailbox: [0] post sequence, [1] nf, [2] E
     unsigned long long post_seq = 0;       // last sequence number posted to h_mbox
     hipEvent_t ev[2] = {nullptr, nullptr};

@@ -547,6 +547,73 @@ static __global__ void push_seed_kernel(const uint32_t* __restrict__ row_off, ui
     }
 }
 
+// Start of a push-engine run in one launch: the per-vertex state (SSSP: distance words (dist << 32 |
+// stamp), all unreached but src; BFS: depths, and the direction-optimizing engine's first two frontier
+// bitmaps with src set in the first), and, in block 0, the seeded input queue (push_seed_kernel) and the
+// run counters ctr[0..8) cleared.
+template <bool kSssp>
+static __global__ void __launch_bounds__(256)
+push_start_kernel(uint32_t n, uint32_t src, unsigned long long* __restrict__ dstamp, uint32_t* __restrict__ depth,
+                  uint32_t* __restrict__ fb, uint32_t words, const uint32_t* __restrict__ row_off, uint32_t* q,
+                  uint32_t* qoff, unsigned long long* cnt3, unsigned long long* ctr) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t v = t0; v < n; v += stride) {
+        if (kSssp) dstamp[v] = (v == src) ? 0ull : (static_cast<unsigned long long>(kInf) << 32);
+        else depth[v] = (v == src) ? 0u : kInf;
+    }
+    if (!kSssp && fb)
+        for (uint32_t i = t0; i < 2 * words; i += stride) fb[i] = (i == (src >> 5)) ? (1u << (src & 31u)) : 0u;
+    if (blockIdx.x == 0) {
+        for (uint32_t i = threadIdx.x; i < 3u * kSegments; i += blockDim.x) cnt3[i] = 0ull;
+        if (threadIdx.x < 8) ctr[threadIdx.x] = 0ull;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t dg = row_off[src + 1] - row_off[src];
+            q[0] = src;
+            qoff[0] = 0;
+            cnt3[0] = dg ? ((static_cast<unsigned long long>(dg) << 32) | 1ull) : 0ull;
+        }
+    }
+}
+
+// End of a push-engine run, one workgroup: folds (and clears) the four stats slot regions -- activated and
+// scanned of the run's launches, reached vertices and their out-edges of the result -- and writes the run's
+// record to host memory: out[0..8) the counters ctr[0..8) with [3] += activated, [4] = reached,
+// [5] = reached out-edges, [7] += scanned; out[8] the overflow flag; hist_out[0 .. 2 * launched) the
+// iteration history.
+static __global__ void __launch_bounds__(256)
+push_finish_kernel(unsigned long long* __restrict__ slots, const unsigned long long* __restrict__ ctr,
+                   const uint32_t* __restrict__ overflow, const unsigned long long* __restrict__ hist, uint32_t launched,
+                   unsigned long long* out, unsigned long long* hist_out) {
+    __shared__ unsigned long long s[4][256 / kWave];
+    unsigned long long acc[4] = {0ull, 0ull, 0ull, 0ull};
+    for (uint32_t i = threadIdx.x; i < kMaxPushBlocks; i += 256) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc[k] += slots[k * kMaxPushBlocks + i];
+            slots[k * kMaxPushBlocks + i] = 0ull;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        acc[k] = wave_sum(acc[k]);
+        if (lane_id() == 0) s[k][threadIdx.x / kWave] = acc[k];
+    }
+    for (uint32_t i = threadIdx.x; i < 2 * launched; i += 256) hist_out[i] = hist[i];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        unsigned long long v = ctr[threadIdx.x];
+        const int k = threadIdx.x == 3 ? 0 : threadIdx.x == 7 ? 1 : threadIdx.x == 4 ? 2 : threadIdx.x == 5 ? 3 : -1;
+        if (k >= 0) {
+            const unsigned long long t = s[k][0] + s[k][1] + s[k][2] + s[k][3];
+            v = (k >= 2) ? t : v + t;
+        }
+        out[threadIdx.x] = v;
+    }
+    if (threadIdx.x == 8) out[8] = *overflow;
+}
+
 // Adds per-workgroup stats slots [0, count) to out[0] (one block, fixed order) and clears them.
 static __global__ void __launch_bounds__(256) sum_slots_kernel(unsigned long long* __restrict__ slots, uint32_t count,
                                                                 unsigned long long* __restrict__ out) {
