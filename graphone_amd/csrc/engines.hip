@@ -91,6 +91,16 @@ PushQueues make_queues(g1_graph* g, uint32_t k) {
     return pq;
 }
 
+// The SSSP push engine relaxes a frontier of at least G1_SSSP_NORETURN (default 0.3) times the graph's
+// edges without waiting on its atomics; G1_SSSP_NORETURN=0 disables it.
+double sssp_noreturn_fraction() {
+    static const double f = [] {
+        const char* e = getenv("G1_SSSP_NORETURN");
+        return e ? atof(e) : 0.3;
+    }();
+    return f;
+}
+
 // The push SSSP engine switches an iteration to a pull step when its frontier has at least
 // G1_SSSP_PULL (default 0.8) times the graph's edges; G1_SSSP_PULL=0 disables the switch.
 double sssp_pull_fraction() {
@@ -208,7 +218,7 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
             if (pull_min_edges && (!known || !shrinking || next_edges >= pull_min_edges / 4)) b = 1;
             G1_CHECK(it + b <= kHistIters, G1_ERR_INTERNAL, "push engine exceeded %u iterations", kHistIters);
             for (uint32_t k = 0; k < b; ++k) {
-                G1_TRY(launch(it));
+                G1_TRY(launch(it, k == 0 && known ? next_edges : 0));
                 G1_TRY(after(it));
                 ++it;
             }
@@ -310,7 +320,7 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
     memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
     G1_HIP(hipEventRecord(g->ev[0], s));
     int tok = -1;
-    auto launch = [&](uint32_t it) -> g1_status {
+    auto launch = [&](uint32_t it, uint64_t) -> g1_status {
         PushQueues pq = make_queues(g, it);
         tok = prof_begin(g, diropt ? "bfs_step_diropt" : "bfs_advance", s);
         if (diropt) {
@@ -573,12 +583,25 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
     G1_HIP(hipEventRecord(g->ev[0], s));
     int tok = -1;
-    auto launch = [&](uint32_t it) -> g1_status {
+    // A frontier known to hold at least G1_SSSP_NORETURN x m out-edges (0.3 by default) relaxes without
+    // the winner test (SsspPushNoReturn) and its winners are collected from the stamps afterwards.
+    const uint64_t noreturn_min =
+        sssp_noreturn_fraction() > 0.0 ? std::max<uint64_t>(static_cast<uint64_t>(sssp_noreturn_fraction() * g->m), 1)
+                                       : ~0ull;
+    auto launch = [&](uint32_t it, uint64_t edges) -> g1_status {
         PushQueues pq = make_queues(g, it);
-        SsspPush pol{g->dstamp, it + 1};
         tok = prof_begin(g, "sssp_advance", s);
-        push_advance_kernel<SsspPush><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq, pol, it,
-                                                                             g->hist);
+        if (edges >= noreturn_min) {
+            SsspPushNoReturn pol{g->dstamp, it + 1};
+            push_advance_kernel<SsspPushNoReturn><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w,
+                                                                                         pq, pol, it, g->hist);
+            G1_LAUNCHED();
+            sssp_collect_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->dstamp, it + 1, pq);
+        } else {
+            SsspPush pol{g->dstamp, it + 1};
+            push_advance_kernel<SsspPush><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq, pol,
+                                                                                 it, g->hist);
+        }
         G1_LAUNCHED();
         prof_end(g, tok, s, 0.0);
         return G1_OK;

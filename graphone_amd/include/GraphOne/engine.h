@@ -193,6 +193,7 @@ struct FunctorAdvance {
     using src_t = uint32_t;
     static constexpr bool kWeighted = false;
     static constexpr bool kQueueLeaves = true;  // the output frontier lists every kept vertex
+    static constexpr bool kEnqueue = true;
     GRef g;
     D d;
     __device__ __forceinline__ uint32_t source_value(uint32_t u) const { return u; }

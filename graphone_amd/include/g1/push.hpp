@@ -209,6 +209,10 @@ __device__ __forceinline__ void advance_tiles(const uint32_t* __restrict__ row_o
         }
         bool won[kPushEpt];
         pol.template relax_many<kPushEpt>(uu, su, vv, ww, ee, won);
+        if constexpr (!P::kEnqueue) {
+            __syncthreads();  // the tile's staged sources are overwritten by the next tile
+            continue;
+        }
         uint32_t cand[kPushEpt];
         uint32_t cdeg[kPushEpt];
         uint32_t rb[kPushEpt], re[kPushEpt];
@@ -278,6 +282,7 @@ struct SsspPush {
     using src_t = uint32_t;
     static constexpr bool kWeighted = true;
     static constexpr bool kQueueLeaves = false;
+    static constexpr bool kEnqueue = true;
     unsigned long long* dstamp;
     uint32_t iter;
     __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
@@ -320,12 +325,90 @@ struct SsspPush {
     __device__ __forceinline__ void set_next_bitmap(uint32_t*) {}
 };
 
+// Vertex scans that enqueue (sssp_collect_kernel, sssp_pull_apply_kernel): vertices per thread per round.
+constexpr uint32_t kScanPerThread = 8;
+constexpr uint32_t kScanRound = kScanPerThread * kPushThreads;
+
+// SSSP relaxation of a dense iteration without the winner test: each improving slot lowers its target's
+// word with an atomicMin whose result is not waited for, and the iteration's winners -- exactly the
+// vertices whose word carries this iteration's stamp afterwards, since a strict improvement installs
+// (nd, iter) and nothing else does -- are enqueued by sssp_collect_kernel in one streaming pass. A dense
+// iteration spends most of its time waiting on returning atomics one after another; this one does not.
+struct SsspPushNoReturn {
+    using src_t = uint32_t;
+    static constexpr bool kWeighted = true;
+    static constexpr bool kQueueLeaves = false;
+    static constexpr bool kEnqueue = false;
+    unsigned long long* dstamp;
+    uint32_t iter;
+    __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
+        return static_cast<uint32_t>(dstamp[u] >> 32);
+    }
+    template <int N>
+    __device__ __forceinline__ void relax_many(const uint32_t (&)[N], const uint32_t (&du)[N], const uint32_t (&v)[N],
+                                               const uint32_t (&w)[N], const uint32_t (&)[N], bool (&won)[N]) const {
+        uint32_t cur[N];
+#pragma unroll
+        for (int r = 0; r < N; ++r) cur[r] = v[r] != kInf ? static_cast<uint32_t>(dstamp[v[r]] >> 32) : 0u;
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            won[r] = false;
+            const uint32_t nd = du[r] + w[r];
+            if (v[r] != kInf && nd < cur[r])
+                (void)atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | iter);
+        }
+    }
+};
+
+// The winners of a no-return iteration: every vertex whose word carries `stamp` (the iteration's stamp)
+// and has out-edges joins the next frontier (segment blockIdx % kSegments of the output queue).
+static __global__ void __launch_bounds__(kPushThreads)
+sssp_collect_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const unsigned long long* __restrict__ dstamp,
+                    uint32_t stamp, PushQueues pq) {
+    __shared__ unsigned long long s_scan[kPushThreads / kWave];
+    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_red[kPushThreads / kWave];
+    const uint32_t seg_out = blockIdx.x % kSegments;
+    uint32_t activated = 0;
+    // kScanPerThread vertices per thread and one queue reservation per workgroup round
+    for (uint32_t base = blockIdx.x * kScanRound; base < n; base += gridDim.x * kScanRound) {  // uniform
+        uint32_t cand[kScanPerThread], cdeg[kScanPerThread];
+        uint32_t my_cnt = 0, my_deg = 0;
+        bool hit[kScanPerThread];
+#pragma unroll
+        for (uint32_t r = 0; r < kScanPerThread; ++r) {
+            const uint32_t v = base + r * kPushThreads + threadIdx.x;
+            hit[r] = v < n && static_cast<uint32_t>(dstamp[v]) == stamp;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kScanPerThread; ++r) {
+            const uint32_t v = base + r * kPushThreads + threadIdx.x;
+            cand[r] = kInf;
+            cdeg[r] = 0;
+            if (hit[r]) {
+                ++activated;
+                const uint32_t dg = row_off[v + 1] - row_off[v];
+                if (dg > 0) {
+                    cand[r] = v;
+                    cdeg[r] = dg;
+                    ++my_cnt;
+                    my_deg += dg;
+                }
+            }
+        }
+        push_enqueue(cand, cdeg, my_cnt, my_deg, pq, seg_out, s_scan, &s_base);
+        __syncthreads();
+    }
+    block_stat_add(pq.stats + blockIdx.x, activated, s_red);
+}
+
 // BFS top-down step: the first visitor of v sets its depth; optionally marks v in a bitmap that a
 // following bottom-up step reads.
 struct BfsPush {
     using src_t = uint32_t;
     static constexpr bool kWeighted = false;
     static constexpr bool kQueueLeaves = false;
+    static constexpr bool kEnqueue = true;
     uint32_t* depth;
     uint32_t* fb_next;  // may be null
     uint32_t next_level;
@@ -395,24 +478,37 @@ sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const u
     __shared__ unsigned long long s_scan[kPushThreads / kWave];
     __shared__ unsigned long long s_base;
     __shared__ uint32_t s_red[kPushThreads / kWave];
-    const uint32_t groups = (n + kPushThreads - 1) / kPushThreads;
     const uint32_t seg_out = blockIdx.x % kSegments;
     uint32_t activated = 0;
-    for (uint32_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform per workgroup
-        const uint32_t v = grp * kPushThreads + threadIdx.x;
-        uint32_t cand[1] = {kInf}, cdeg[1] = {0};
+    for (uint32_t base = blockIdx.x * kScanRound; base < n; base += gridDim.x * kScanRound) {  // uniform
+        uint32_t cand[kScanPerThread], cdeg[kScanPerThread];
         uint32_t my_cnt = 0, my_deg = 0;
-        if (v < n) {
-            const uint32_t yv = y[v];
-            if (yv < static_cast<uint32_t>(dstamp[v] >> 32)) {
-                dstamp[v] = (static_cast<unsigned long long>(yv) << 32) | stamp;
+        uint32_t yv[kScanPerThread];
+        unsigned long long ds[kScanPerThread];
+#pragma unroll
+        for (uint32_t r = 0; r < kScanPerThread; ++r) {
+            const uint32_t v = base + r * kPushThreads + threadIdx.x;
+            yv[r] = kInf;
+            ds[r] = 0ull;
+            if (v < n) {
+                yv[r] = y[v];
+                ds[r] = dstamp[v];
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kScanPerThread; ++r) {
+            const uint32_t v = base + r * kPushThreads + threadIdx.x;
+            cand[r] = kInf;
+            cdeg[r] = 0;
+            if (v < n && yv[r] < static_cast<uint32_t>(ds[r] >> 32)) {
+                dstamp[v] = (static_cast<unsigned long long>(yv[r]) << 32) | stamp;
                 ++activated;
                 const uint32_t dg = row_off[v + 1] - row_off[v];
                 if (dg > 0) {
-                    cand[0] = v;
-                    cdeg[0] = dg;
-                    my_cnt = 1;
-                    my_deg = dg;
+                    cand[r] = v;
+                    cdeg[r] = dg;
+                    ++my_cnt;
+                    my_deg += dg;
                 }
             }
         }
