@@ -588,22 +588,29 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     const uint64_t noreturn_min =
         sssp_noreturn_fraction() > 0.0 ? std::max<uint64_t>(static_cast<uint64_t>(sssp_noreturn_fraction() * g->m), 1)
                                        : ~0ull;
+    int tok_c = -1;  // the collect launch of a no-return iteration (profiling)
     auto launch = [&](uint32_t it, uint64_t edges) -> g1_status {
         PushQueues pq = make_queues(g, it);
-        tok = prof_begin(g, "sssp_advance", s);
+        tok_c = -1;
         if (edges >= noreturn_min) {
             SsspPushNoReturn pol{g->dstamp, it + 1};
+            tok = prof_begin(g, "sssp_advance_nr", s);
             push_advance_kernel<SsspPushNoReturn><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w,
                                                                                          pq, pol, it, g->hist);
             G1_LAUNCHED();
+            prof_end(g, tok, s, 0.0);
+            tok_c = prof_begin(g, "sssp_collect", s);
             sssp_collect_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->dstamp, it + 1, pq);
+            G1_LAUNCHED();
+            prof_end(g, tok_c, s, 0.0);
         } else {
             SsspPush pol{g->dstamp, it + 1};
+            tok = prof_begin(g, "sssp_advance", s);
             push_advance_kernel<SsspPush><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq, pol,
                                                                                  it, g->hist);
+            G1_LAUNCHED();
+            prof_end(g, tok, s, 0.0);
         }
-        G1_LAUNCHED();
-        prof_end(g, tok, s, 0.0);
         return G1_OK;
     };
     auto after = [&](uint32_t it) -> g1_status {
@@ -614,7 +621,16 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
         G1_HIP(hipMemcpy(h, g->hist + 2 * it, sizeof(h), hipMemcpyDeviceToHost));
         const double nf = static_cast<double>(static_cast<uint32_t>(h[0]));
         const double E = static_cast<double>(h[0] >> 32);
-        prof_set_bytes(g, tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E + 4.0 * static_cast<double>(g->h_ctr[3] - act0));
+        const double act = static_cast<double>(g->h_ctr[3] - act0);
+        // advance: frontier entries (id, offset, row start, source distance) + column, weight and target word
+        // per edge; its winners' entries (4 B each) unless a collect launch enqueued them, which reads every
+        // distance word and the winners' degrees and writes their entries
+        if (tok_c >= 0) {
+            prof_set_bytes(g, tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E);
+            prof_set_bytes(g, tok_c, 8.0 * n + 16.0 * act);
+        } else {
+            prof_set_bytes(g, tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E + 4.0 * act);
+        }
         return G1_OK;
     };
     auto small = [&](uint32_t it0, uint32_t* res) -> g1_status {
