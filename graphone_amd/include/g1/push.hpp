@@ -522,6 +522,7 @@ sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const u
 // current frontier bitmap and stops at the first one. The next frontier is written both as a bitmap
 // (every word fully written by one wave, so it needs no clearing) and as a queue for a following
 // top-down step. Each workgroup takes a contiguous run of 256-vertex groups.
+constexpr uint32_t kPullGroups = 8;
 __device__ __forceinline__ void pull_groups(uint32_t n, const uint32_t* __restrict__ t_row_off,
                                             const uint32_t* __restrict__ t_col, const uint32_t* __restrict__ row_off,
                                             uint32_t* __restrict__ depth, const uint32_t* __restrict__ fb_cur,
@@ -537,40 +538,68 @@ __device__ __forceinline__ void pull_groups(uint32_t n, const uint32_t* __restri
     const uint32_t g_beg = blockIdx.x * per;
     const uint32_t g_end = (g_beg + per < groups) ? g_beg + per : groups;
     const uint32_t seg_out = blockIdx.x % kSegments;
-    for (uint32_t grp = g_beg; grp < g_end; ++grp) {
-        const uint32_t v = grp * kPushThreads + threadIdx.x;
-        bool found = false;
-        if (v < n && depth[v] == kInf) {
-            const uint32_t b = t_row_off[v], e = t_row_off[v + 1];
+    // kPullGroups groups per round and one queue reservation per round: waves run through their vertices
+    // without waiting for each other group by group
+    for (uint32_t g0 = g_beg; g0 < g_end; g0 += kPullGroups) {  // uniform per workgroup
+        uint32_t vv[kPullGroups], rb[kPullGroups], re[kPullGroups];
+        bool found[kPullGroups];
+#pragma unroll
+        for (uint32_t r = 0; r < kPullGroups; ++r) {
+            vv[r] = (g0 + r) * kPushThreads + threadIdx.x;
+            found[r] = g0 + r < g_end && vv[r] < n && depth[vv[r]] == kInf;  // unvisited (scan below)
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kPullGroups; ++r) {
+            rb[r] = 0;
+            re[r] = 0;
+            if (found[r]) {
+                rb[r] = t_row_off[vv[r]];
+                re[r] = t_row_off[vv[r] + 1];
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kPullGroups; ++r) {
+            bool hit = false;
             // four in-edges per step, their bitmap words loaded together
-            for (uint32_t k = b; k < e && !found; k += 4) {
+            for (uint32_t k = rb[r]; k < re[r] && !hit; k += 4) {
                 uint32_t u[4], wd[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) u[q] = (k + q < e) ? t_col[k + q] : kInf;
+                for (int q = 0; q < 4; ++q) u[q] = (k + q < re[r]) ? t_col[k + q] : kInf;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) wd[q] = (u[q] != kInf) ? fb_cur[u[q] >> 5] : 0u;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    if (!found && u[q] != kInf) {
+                    if (!hit && u[q] != kInf) {
                         ++scanned;
-                        if ((wd[q] >> (u[q] & 31u)) & 1u) found = true;
+                        if ((wd[q] >> (u[q] & 31u)) & 1u) hit = true;
                     }
                 }
             }
+            found[r] = hit;
         }
-        const uint64_t m = ballot(found);
-        const uint32_t lane = lane_id();
-        const uint32_t word = v >> 5;  // lanes 0..31 and 32..63 of a wave each own one word
-        if ((lane & 31u) == 0 && word < words)
-            fb_next[word] = (lane == 0) ? static_cast<uint32_t>(m) : static_cast<uint32_t>(m >> 32);
-        uint32_t cand[1] = {kInf};
-        uint32_t cdeg[1] = {0};
+        uint32_t cand[kPullGroups], cdeg[kPullGroups];
         uint32_t my_cnt = 0, my_deg = 0;
-        if (found) {
-            depth[v] = next_level;
-            ++activated;
-            const uint32_t dg = row_off[v + 1] - row_off[v];
-            if (dg > 0) { cand[0] = v; cdeg[0] = dg; my_cnt = 1; my_deg = dg; }
+        const uint32_t lane = lane_id();
+#pragma unroll
+        for (uint32_t r = 0; r < kPullGroups; ++r) {
+            cand[r] = kInf;
+            cdeg[r] = 0;
+            if (g0 + r >= g_end) continue;  // uniform
+            const uint64_t mk = ballot(found[r]);
+            const uint32_t word = vv[r] >> 5;  // lanes 0..31 and 32..63 of a wave each own one word
+            if ((lane & 31u) == 0 && word < words)
+                fb_next[word] = (lane == 0) ? static_cast<uint32_t>(mk) : static_cast<uint32_t>(mk >> 32);
+            if (found[r]) {
+                depth[vv[r]] = next_level;
+                ++activated;
+                const uint32_t dg = row_off[vv[r] + 1] - row_off[vv[r]];
+                if (dg > 0) {
+                    cand[r] = vv[r];
+                    cdeg[r] = dg;
+                    ++my_cnt;
+                    my_deg += dg;
+                }
+            }
         }
         push_enqueue(cand, cdeg, my_cnt, my_deg, pq, seg_out, s_scan, &s_base);
     }
