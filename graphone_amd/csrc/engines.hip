@@ -11,6 +11,7 @@
 #include <immintrin.h>
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "g1/apply.hpp"
@@ -127,8 +128,10 @@ __global__ void __launch_bounds__(64) frontier_post_kernel(const unsigned long l
 }
 
 // Returns (E << 32) | nf of the frontier in counter set `set`, once the work enqueued on s before it is
-// done. The host spins on the mailbox; every 1024 polls it asks the stream, and a stream that has drained
-// (or failed) without the post showing is an error.
+// done. The host spins on the mailbox, yielding its core between polls after the first 2048 (a dense
+// iteration runs for hundreds of microseconds, and the query server polls from many threads at once);
+// every 1024 polls it asks the stream, and a stream that has drained (or failed) without the post showing
+// is an error.
 g1_status read_frontier(g1_graph* g, uint32_t set, hipStream_t s, unsigned long long* packed) {
     const unsigned long long seq = ++g->post_seq;
     frontier_post_kernel<<<1, 64, 0, s>>>(g->qcnt + set * kSegments, g->h_mbox, seq);
@@ -146,7 +149,8 @@ g1_status read_frontier(g1_graph* g, uint32_t set, hipStream_t s, unsigned long 
                 break;
             }
         }
-        _mm_pause();
+        if (polls < 2048) _mm_pause();
+        else std::this_thread::yield();
     }
     const unsigned long long nf = __atomic_load_n(g->h_mbox + 1, __ATOMIC_ACQUIRE);
     const unsigned long long E = __atomic_load_n(g->h_mbox + 2, __ATOMIC_ACQUIRE);
