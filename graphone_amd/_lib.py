@@ -69,7 +69,7 @@ class RunStats(C.Structure):
         ("iterations", C.c_uint32),
         ("push_iterations", C.c_uint32),
         ("pull_iterations", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("dense_iterations", C.c_uint32),
         ("frontier_sum", C.c_uint64),
         ("edges_sum", C.c_uint64),
         ("reached", C.c_uint64),
@@ -79,7 +79,7 @@ class RunStats(C.Structure):
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class BatchStats(C.Structure):

@@ -208,8 +208,13 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
             it = res[0];
             if (res[1] == 0) break;
             batch = 2;
-            known = false;
             shrinking = false;
+            // the frontier handed back is past the small kernel's size: read it so that this iteration can
+            // already be a pull step or a dense launch
+            unsigned long long back = 0;
+            G1_TRY(read_frontier(g, it % 3, s, &back));
+            next_edges = back >> 32;
+            known = true;
         }
         unsigned long long next = 0;
         if (pull_min_edges && known && next_edges >= pull_min_edges) {
@@ -593,10 +598,12 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
         sssp_noreturn_fraction() > 0.0 ? std::max<uint64_t>(static_cast<uint64_t>(sssp_noreturn_fraction() * g->m), 1)
                                        : ~0ull;
     int tok_c = -1;  // the collect launch of a no-return iteration (profiling)
+    uint32_t dense = 0;
     auto launch = [&](uint32_t it, uint64_t edges) -> g1_status {
         PushQueues pq = make_queues(g, it);
         tok_c = -1;
         if (edges >= noreturn_min) {
+            ++dense;
             SsspPushNoReturn pol{g->dstamp, it + 1};
             tok = prof_begin(g, "sssp_advance_nr", s);
             push_advance_kernel<SsspPushNoReturn><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w,
@@ -669,6 +676,7 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     G1_TRY(collect_push_run(g, s, launched, dist, st, true));
     st->pull_iterations += pulls;
     st->push_iterations -= std::min(st->push_iterations, pulls);
+    st->dense_iterations = dense;
     st->frontier_sum = 1 + g->h_ctr[3];
     return G1_OK;
 }

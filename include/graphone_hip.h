@@ -188,7 +188,7 @@ typedef struct g1_run_stats {
     uint32_t iterations;       /* engine iterations (Forward calls)                         */
     uint32_t push_iterations;  /* iterations run top-down (advance)                          */
     uint32_t pull_iterations;  /* iterations run bottom-up / SpMV                            */
-    uint32_t reserved;
+    uint32_t dense_iterations; /* SSSP push: iterations relaxed without the winner test      */
     uint64_t frontier_sum;     /* sum over iterations of |F_i| (vertices activated)          */
     uint64_t edges_sum;        /* sum over iterations of E(F_i) (push) or nnz scanned (pull) */
     uint64_t reached;          /* vertices with a finite result                              */
