@@ -737,19 +737,44 @@ g1_status run_minplus_pull(g1_graph* g, uint32_t src, uint32_t* d, bool weighted
 
 // ---------------------------------------------------------------- PageRank / personalized PageRank
 
+// out[perm[i]] = r[i]: ranks in the relabelled ids back to the graph's ids.
+__global__ void unpermute_f32_kernel(uint32_t n, const uint32_t* __restrict__ perm, const float* __restrict__ r,
+                                     float* __restrict__ out) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[perm[i]] = r[i];
+}
+
+// Plain PageRank relabels the vertices hub first unless G1_PR_RELABEL=0.
+bool pr_relabel() {
+    static const bool on = [] {
+        const char* e = getenv("G1_PR_RELABEL");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, uint32_t max_iter, double tol,
                        int mode, int engine, float* out, g1_run_stats* st, hipStream_t s) {
     G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "PageRank needs the graph's transpose");
     G1_TRY(ensure_pull_work(g, true));
     const uint32_t n = g->n;
     const uint32_t blocks = grid_for(n, 256, std::min<uint32_t>(kPartials, persistent_cap()));
-    const float* vals = g->t_wf;  // null for an unweighted graph: every weight is 1
     const uint32_t lanes = csrvec_lanes(n, g->m);
     const bool personalized = seed != kInf;
+    // Plain PageRank runs over the vertices relabelled hub first (ensure_pr_relabel; G1_PR_RELABEL=0 keeps
+    // the graph's ids) and un-permutes the ranks at the end; personalized PageRank keeps the graph's ids,
+    // the fold order the batched engines reproduce.
+    const bool relabel = !personalized && pr_relabel() && n > 0;
+    if (relabel) G1_TRY(ensure_pr_relabel(g, s));
+    const uint32_t* t_ro = relabel ? g->pr_row_off : g->t_row_off;
+    const uint32_t* t_ci = relabel ? g->pr_col : g->t_col;
+    const float* vals = relabel ? g->pr_wf : g->t_wf;  // null for an unweighted graph: every weight is 1
+    const float* osum = relabel ? g->pr_osum : g->osum;
+    void* ws = relabel ? g->pr_ws : g->t_ws;
     // alpha of the propagation: PageRank damping, or 1 - restart for the personalized variant.
     const float prop = personalized ? 1.0f - restart : alpha;
     const float r0 = n ? 1.0f / static_cast<float>(n) : 0.0f;
-    pr_init_kernel<<<grid_for(n, 256, persistent_cap()), 256, 0, s>>>(n, g->osum, prop, g->iw, g->r[0],
+    pr_init_kernel<<<grid_for(n, 256, persistent_cap()), 256, 0, s>>>(n, osum, prop, g->iw, g->r[0],
                                                                       r0, g->xf, seed);
     G1_LAUNCHED();
     const float base_all = personalized ? 0.0f : static_cast<float>((1.0 - alpha) / static_cast<double>(n));
@@ -774,10 +799,9 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
             delta = g->h_scalar[0];
             if (delta <= tol || it >= max_iter) break;
             if (engine == G1_ENGINE_BLAS_CSRVEC)
-                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, g->t_row_off, g->t_col, vals, g->xf, g->yf, lanes, s));
+                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, t_ro, t_ci, vals, g->xf, g->yf, lanes, s));
             else
-                G1_TRY(launch_spmv_merge<PlusTimesKahanF32>(n, g->m, g->t_row_off, g->t_col, vals, g->xf, g->yf,
-                                                            g->t_ws, s));
+                G1_TRY(launch_spmv_merge<PlusTimesKahanF32>(n, g->m, t_ro, t_ci, vals, g->xf, g->yf, ws, s));
             // In place on ranks[idx] as apply_each does; the partials of this call are unused.
             pr_apply_kernel<1><<<blocks, 256, 0, s>>>(n, g->yf, base_all, seed, base_seed, g->iw, g->r[idx ^ 1],
                                                       g->r[idx], g->xf, g->partials);
@@ -796,11 +820,10 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
             int tok;
             if (engine == G1_ENGINE_BLAS_CSRVEC) {
                 tok = prof_begin(g, "spmv_csrvec_plustimes", s);
-                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, g->t_row_off, g->t_col, vals, g->xf, g->yf, lanes, s));
+                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, t_ro, t_ci, vals, g->xf, g->yf, lanes, s));
             } else {
                 tok = prof_begin(g, "spmv_merge_plustimes", s);
-                G1_TRY(launch_spmv_merge<PlusTimesKahanF32>(n, g->m, g->t_row_off, g->t_col, vals, g->xf, g->yf,
-                                                            g->t_ws, s));
+                G1_TRY(launch_spmv_merge<PlusTimesKahanF32>(n, g->m, t_ro, t_ci, vals, g->xf, g->yf, ws, s));
             }
             prof_end(g, tok, s, spmv_bytes);
             tok = prof_begin(g, "pr_apply", s);
@@ -822,7 +845,12 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
     G1_HIP(hipEventRecord(g->ev[1], s));
     G1_TRY(elapsed(g, st));
     G1_TRY(prof_flush(g));
-    G1_HIP(hipMemcpyAsync(out, g->r[cur], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    if (relabel) {
+        unpermute_f32_kernel<<<grid_for(n, 256, persistent_cap()), 256, 0, s>>>(n, g->pr_perm, g->r[cur], out);
+        G1_LAUNCHED();
+    } else {
+        G1_HIP(hipMemcpyAsync(out, g->r[cur], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    }
     G1_HIP(hipStreamSynchronize(s));
     st->iterations = it;
     st->pull_iterations = it;

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "g1/push.hpp"
 #include "g1/spmv_merge.hpp"
@@ -57,6 +58,56 @@ g1_status ensure_push_work(g1_graph* g, hipStream_t s) {
     const g1_stream_t gs = reinterpret_cast<g1_stream_t>(s);
     G1_TRY(g1_fill_u32(reinterpret_cast<uint32_t*>(g->slots), 0u, 8ull * kMaxPushBlocks, gs));
     G1_TRY(g1_fill_u32(g->overflow, 0u, 4, gs));
+    return G1_OK;
+}
+
+// The relabelled copy of the in-edge CSR for plain PageRank, built once on the host: vertices ordered by
+// out-degree, largest first (ties by id), so the sources gathered most often -- the hubs -- share cache
+// lines; a row keeps its in-edges in their original order. Needs the out-weight sums (ensure_pull_work).
+g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s) {
+    if (g->pr_perm || !g->has_t) return G1_OK;
+    const uint32_t n = g->n, m = g->m;
+    G1_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> ro(static_cast<size_t>(n) + 1), tro(static_cast<size_t>(n) + 1), tcol(m);
+    std::vector<float> osum(n), twf(g->t_wf ? m : 0);
+    G1_HIP(hipMemcpy(ro.data(), g->row_off, sizeof(uint32_t) * (n + 1ull), hipMemcpyDeviceToHost));
+    G1_HIP(hipMemcpy(tro.data(), g->t_row_off, sizeof(uint32_t) * (n + 1ull), hipMemcpyDeviceToHost));
+    if (m) G1_HIP(hipMemcpy(tcol.data(), g->t_col, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+    if (m && g->t_wf) G1_HIP(hipMemcpy(twf.data(), g->t_wf, sizeof(float) * m, hipMemcpyDeviceToHost));
+    if (n) G1_HIP(hipMemcpy(osum.data(), g->osum, sizeof(float) * n, hipMemcpyDeviceToHost));
+    // counting sort by out-degree, descending, stable
+    uint32_t maxd = 0;
+    for (uint32_t v = 0; v < n; ++v) maxd = std::max(maxd, ro[v + 1] - ro[v]);
+    std::vector<uint32_t> start(static_cast<size_t>(maxd) + 2, 0), perm(n), inv(n);
+    for (uint32_t v = 0; v < n; ++v) ++start[maxd - (ro[v + 1] - ro[v]) + 1];
+    for (size_t d = 1; d < start.size(); ++d) start[d] += start[d - 1];
+    for (uint32_t v = 0; v < n; ++v) perm[start[maxd - (ro[v + 1] - ro[v])]++] = v;
+    for (uint32_t i = 0; i < n; ++i) inv[perm[i]] = i;
+    std::vector<uint32_t> nro(static_cast<size_t>(n) + 1), ncol(static_cast<size_t>(m) + kCsrPad, 0);
+    std::vector<float> nwf(g->t_wf ? static_cast<size_t>(m) + kCsrPad : 0, 0.0f), nosum(n);
+    nro[0] = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t v = perm[i], b = tro[v], e = tro[v + 1];
+        nro[i + 1] = nro[i] + (e - b);
+        for (uint32_t k = b; k < e; ++k) {
+            ncol[nro[i] + (k - b)] = inv[tcol[k]];
+            if (g->t_wf) nwf[nro[i] + (k - b)] = twf[k];
+        }
+        nosum[i] = osum[v];
+    }
+    G1_TRY(alloc_n(g, &g->pr_perm, n ? n : 1));
+    G1_TRY(alloc_n(g, &g->pr_row_off, static_cast<size_t>(n) + 1));
+    G1_TRY(alloc_n(g, &g->pr_col, static_cast<size_t>(m) + kCsrPad));
+    G1_TRY(alloc_n(g, &g->pr_osum, n ? n : 1));
+    if (g->t_wf) G1_TRY(alloc_n(g, &g->pr_wf, static_cast<size_t>(m) + kCsrPad));
+    if (n) G1_HIP(hipMemcpy(g->pr_perm, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    G1_HIP(hipMemcpy(g->pr_row_off, nro.data(), sizeof(uint32_t) * (n + 1ull), hipMemcpyHostToDevice));
+    G1_HIP(hipMemcpy(g->pr_col, ncol.data(), sizeof(uint32_t) * ncol.size(), hipMemcpyHostToDevice));
+    if (n) G1_HIP(hipMemcpy(g->pr_osum, nosum.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    if (g->t_wf) G1_HIP(hipMemcpy(g->pr_wf, nwf.data(), sizeof(float) * nwf.size(), hipMemcpyHostToDevice));
+    G1_TRY(graph_alloc(g, &g->pr_ws, merge_workspace_bytes(n, m)));
+    G1_TRY(launch_merge_partition(n, m, g->pr_row_off, g->pr_ws, s));
+    G1_HIP(hipStreamSynchronize(s));
     return G1_OK;
 }
 

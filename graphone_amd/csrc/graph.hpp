@@ -31,6 +31,15 @@ struct g1_graph {
     size_t t_ws_bytes = 0;
     unsigned long long* t_starts = nullptr;  // bitmap of in-edge CSR row starts (lazy, batched min-plus)
 
+    // plain PageRank over the vertices relabelled hub first (lazy, ensure_pr_relabel): new id i is old id
+    // pr_perm[i]; the in-edge CSR, its weights, the out-weight sums and the merge partition in the new ids
+    uint32_t* pr_perm = nullptr;
+    uint32_t* pr_row_off = nullptr;
+    uint32_t* pr_col = nullptr;
+    float* pr_wf = nullptr;
+    float* pr_osum = nullptr;
+    void* pr_ws = nullptr;
+
     // push work areas
     uint32_t* q[2] = {nullptr, nullptr};
     uint32_t* qoff[2] = {nullptr, nullptr};
@@ -113,6 +122,7 @@ g1_status road_queries(g1_graph* g, const uint32_t* kinds, const uint32_t* sourc
                        uint8_t* ok, uint32_t* iters, hipStream_t s);
 g1_status ensure_push_work(g1_graph* g, hipStream_t s);  // work areas cleared on stream s
 g1_status ensure_pull_work(g1_graph* g, bool pagerank);
+g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s);  // after ensure_pull_work(g, true)
 
 // Profiling: prof_begin records a start event (returns -1 when profiling is off), prof_end the stop
 // event with the launch's algorithmic bytes; prof_flush (after the stream is synchronized) adds the
