@@ -269,8 +269,6 @@ g1_status run_small(g1_graph* g, hipStream_t s, const P& pol, const uint32_t* wt
     return G1_OK;
 }
 
-// Reads the history of `launched` launches and fills iteration statistics: the iterations that had a
-// non-empty input, their frontier entries and edges.
 // The end of a push-engine run (BFS or SSSP) in two launches and one stream synchronization: the reached
 // vertices of the result d are counted into the third and fourth stats-slot regions, then
 // push_finish_kernel folds every slot region and writes the counters, the overflow flag and the
@@ -761,10 +759,10 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
     const uint32_t blocks = grid_for(n, 256, std::min<uint32_t>(kPartials, persistent_cap()));
     const uint32_t lanes = csrvec_lanes(n, g->m);
     const bool personalized = seed != kInf;
-    // Plain PageRank runs over the vertices relabelled hub first (ensure_pr_relabel; G1_PR_RELABEL=0 keeps
-    // the graph's ids) and un-permutes the ranks at the end; personalized PageRank keeps the graph's ids,
-    // the fold order the batched engines reproduce.
-    const bool relabel = !personalized && pr_relabel() && n > 0;
+    // Plain PageRank runs over the vertices relabelled hub first (ensure_pr_relabel; G1_PR_RELABEL=0 or a
+    // view handle keeps the graph's ids) and un-permutes the ranks at the end; personalized PageRank keeps
+    // the graph's ids, the fold order the batched engines reproduce.
+    const bool relabel = !personalized && pr_relabel() && n > 0 && !g->is_view;
     if (relabel) G1_TRY(ensure_pr_relabel(g, s));
     const uint32_t* t_ro = relabel ? g->pr_row_off : g->t_row_off;
     const uint32_t* t_ci = relabel ? g->pr_col : g->t_col;

@@ -329,6 +329,7 @@ g1_status g1_graph_create_view(g1_graph* base, g1_graph** out) {
     g->t_w = base->t_w;
     g->t_wf = base->t_wf;
     g->has_t = base->has_t;
+    g->is_view = true;
     g->t_alias = base->t_alias;
     g->t_ws_bytes = base->t_ws_bytes;  // own copy below: the workspace also holds per-call tile records
     g->t_starts = base->t_starts;

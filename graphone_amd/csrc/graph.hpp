@@ -26,6 +26,7 @@ struct g1_graph {
     uint32_t* t_w = nullptr;
     float* t_wf = nullptr;  // float copy of t_w for plus-times engines (lazy)
     bool has_t = false;
+    bool is_view = false;  // a handle of g1_graph_create_view (shares the base's CSR arrays)
     bool t_alias = false;
     void* t_ws = nullptr;  // merge-path partition of the in-edge CSR
     size_t t_ws_bytes = 0;
