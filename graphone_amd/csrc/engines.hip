@@ -687,6 +687,7 @@ g1_status run_minplus_pull(g1_graph* g, uint32_t src, uint32_t* d, bool weighted
     if (weighted) G1_CHECK(g->t_w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
     G1_TRY(ensure_pull_work(g, false));
     G1_TRY(ensure_push_work(g, s));  // stats slots
+    if (engine == G1_ENGINE_BLAS_CSRVEC) G1_TRY(ensure_long_rows(g, false, s));
     const uint32_t n = g->n;
     const uint32_t cap = std::min<uint32_t>(persistent_cap(), kMaxPushBlocks);
     minplus_seed_kernel<<<grid_for(n, 256, cap), 256, 0, s>>>(n, src, d, g->xu);
@@ -702,7 +703,8 @@ g1_status run_minplus_pull(g1_graph* g, uint32_t src, uint32_t* d, bool weighted
     for (;;) {
         if (engine == G1_ENGINE_BLAS_CSRVEC) {
             const int tok = prof_begin(g, "spmv_csrvec_minplus", s);
-            G1_TRY(launch_spmv_csrvec<MinPlusU32>(n, g->t_row_off, g->t_col, vals, g->xu, g->yu, lanes, s));
+            G1_TRY(launch_spmv_csrvec<MinPlusU32>(n, g->t_row_off, g->t_col, vals, g->xu, g->yu, lanes, s, g->t_long,
+                                                  g->t_nlong, g->long_min));
             prof_end(g, tok, s, spmv_bytes);
         } else {
             const int tok = prof_begin(g, "spmv_merge_minplus", s);
@@ -769,6 +771,9 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
     const float* vals = relabel ? g->pr_wf : g->t_wf;  // null for an unweighted graph: every weight is 1
     const float* osum = relabel ? g->pr_osum : g->osum;
     void* ws = relabel ? g->pr_ws : g->t_ws;
+    if (engine == G1_ENGINE_BLAS_CSRVEC) G1_TRY(ensure_long_rows(g, relabel, s));
+    const uint32_t* lrows = relabel ? g->pr_long : g->t_long;
+    const uint32_t nlong = relabel ? g->pr_nlong : g->t_nlong;
     // alpha of the propagation: PageRank damping, or 1 - restart for the personalized variant.
     const float prop = personalized ? 1.0f - restart : alpha;
     const float r0 = n ? 1.0f / static_cast<float>(n) : 0.0f;
@@ -797,7 +802,8 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
             delta = g->h_scalar[0];
             if (delta <= tol || it >= max_iter) break;
             if (engine == G1_ENGINE_BLAS_CSRVEC)
-                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, t_ro, t_ci, vals, g->xf, g->yf, lanes, s));
+                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, t_ro, t_ci, vals, g->xf, g->yf, lanes, s, lrows, nlong,
+                                                              g->long_min));
             else
                 G1_TRY(launch_spmv_merge<PlusTimesKahanF32>(n, g->m, t_ro, t_ci, vals, g->xf, g->yf, ws, s));
             // In place on ranks[idx] as apply_each does; the partials of this call are unused.
@@ -818,7 +824,8 @@ g1_status run_pagerank(g1_graph* g, float alpha, uint32_t seed, float restart, u
             int tok;
             if (engine == G1_ENGINE_BLAS_CSRVEC) {
                 tok = prof_begin(g, "spmv_csrvec_plustimes", s);
-                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, t_ro, t_ci, vals, g->xf, g->yf, lanes, s));
+                G1_TRY(launch_spmv_csrvec<PlusTimesKahanF32>(n, t_ro, t_ci, vals, g->xf, g->yf, lanes, s, lrows, nlong,
+                                                              g->long_min));
             } else {
                 tok = prof_begin(g, "spmv_merge_plustimes", s);
                 G1_TRY(launch_spmv_merge<PlusTimesKahanF32>(n, g->m, t_ro, t_ci, vals, g->xf, g->yf, ws, s));

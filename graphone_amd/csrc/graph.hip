@@ -111,6 +111,28 @@ g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s) {
     return G1_OK;
 }
 
+// The rows of the in-edge CSR (or of PageRank's relabelled copy) longer than long_min nonzeros, for the
+// CSR-vector engine's long-row kernel; built once on the host.
+g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s) {
+    uint32_t** list = relabelled ? &g->pr_long : &g->t_long;
+    uint32_t* count = relabelled ? &g->pr_nlong : &g->t_nlong;
+    if (*list || !g->has_t) return G1_OK;
+    const uint32_t* ro_dev = relabelled ? g->pr_row_off : g->t_row_off;
+    G1_CHECK(ro_dev != nullptr, G1_ERR_INTERNAL, "long rows of a missing matrix");
+    const uint32_t n = g->n;
+    g->long_min = kCsrvecLongPerLane * csrvec_lanes(n, g->m);
+    G1_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> ro(static_cast<size_t>(n) + 1), rows;
+    G1_HIP(hipMemcpy(ro.data(), ro_dev, sizeof(uint32_t) * (n + 1ull), hipMemcpyDeviceToHost));
+    for (uint32_t v = 0; v < n; ++v)
+        if (ro[v + 1] - ro[v] > g->long_min) rows.push_back(v);
+    G1_TRY(alloc_n(g, list, rows.empty() ? 1 : rows.size()));
+    if (!rows.empty())
+        G1_HIP(hipMemcpy(*list, rows.data(), sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice));
+    *count = static_cast<uint32_t>(rows.size());
+    return G1_OK;
+}
+
 g1_status ensure_pull_work(g1_graph* g, bool pagerank) {
     const size_t n = g->n ? g->n : 1;
     if (!g->xu) {

@@ -40,6 +40,13 @@ struct g1_graph {
     float* pr_wf = nullptr;
     float* pr_osum = nullptr;
     void* pr_ws = nullptr;
+    // CSR-vector long rows (lazy, ensure_long_rows): rows of the in-edge CSR, and of the relabelled one,
+    // longer than long_min nonzeros
+    uint32_t* t_long = nullptr;
+    uint32_t t_nlong = 0;
+    uint32_t* pr_long = nullptr;
+    uint32_t pr_nlong = 0;
+    uint32_t long_min = 0;
 
     // push work areas
     uint32_t* q[2] = {nullptr, nullptr};
@@ -124,6 +131,7 @@ g1_status road_queries(g1_graph* g, const uint32_t* kinds, const uint32_t* sourc
 g1_status ensure_push_work(g1_graph* g, hipStream_t s);  // work areas cleared on stream s
 g1_status ensure_pull_work(g1_graph* g, bool pagerank);
 g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s);  // after ensure_pull_work(g, true)
+g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s);
 
 // Profiling: prof_begin records a start event (returns -1 when profiling is off), prof_end the stop
 // event with the launch's algorithmic bytes; prof_flush (after the stream is synchronized) adds the
