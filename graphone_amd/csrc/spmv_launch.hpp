@@ -53,50 +53,46 @@ g1_status launch_spmv_merge(uint32_t n_rows, uint32_t nnz, const uint32_t* row_o
 template <typename S, bool V, int L>
 inline void csrvec_launch_one(uint32_t n_rows, const uint32_t* row_offsets, const uint32_t* col,
                               const typename S::value_t* vals, const typename S::x_t* x, typename S::y_t* y,
-                              uint32_t long_min, hipStream_t stream) {
+                              uint32_t long_min, const uint32_t* long_rows, uint32_t n_long, hipStream_t stream) {
     const uint64_t threads = static_cast<uint64_t>(n_rows) * L;
-    const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256);
+    const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256) + n_long;
     spmv_csrvec_kernel<S, V, L><<<dim3(blocks), dim3(256), 0, stream>>>(n_rows, row_offsets, col, vals, x, y,
-                                                                        long_min);
+                                                                        long_min, long_rows, n_long);
 }
 
 template <typename S, bool V>
 inline void csrvec_dispatch(uint32_t lanes, uint32_t n_rows, const uint32_t* row_offsets, const uint32_t* col,
                             const typename S::value_t* vals, const typename S::x_t* x, typename S::y_t* y,
-                            uint32_t long_min, hipStream_t stream) {
+                            uint32_t long_min, const uint32_t* long_rows, uint32_t n_long, hipStream_t stream) {
     switch (lanes) {
-        case 1: csrvec_launch_one<S, V, 1>(n_rows, row_offsets, col, vals, x, y, long_min, stream); break;
-        case 2: csrvec_launch_one<S, V, 2>(n_rows, row_offsets, col, vals, x, y, long_min, stream); break;
-        case 4: csrvec_launch_one<S, V, 4>(n_rows, row_offsets, col, vals, x, y, long_min, stream); break;
-        case 8: csrvec_launch_one<S, V, 8>(n_rows, row_offsets, col, vals, x, y, long_min, stream); break;
-        case 16: csrvec_launch_one<S, V, 16>(n_rows, row_offsets, col, vals, x, y, long_min, stream); break;
-        case 32: csrvec_launch_one<S, V, 32>(n_rows, row_offsets, col, vals, x, y, long_min, stream); break;
-        default: csrvec_launch_one<S, V, 64>(n_rows, row_offsets, col, vals, x, y, long_min, stream); break;
+        case 1: csrvec_launch_one<S, V, 1>(n_rows, row_offsets, col, vals, x, y, long_min, long_rows, n_long, stream); break;
+        case 2: csrvec_launch_one<S, V, 2>(n_rows, row_offsets, col, vals, x, y, long_min, long_rows, n_long, stream); break;
+        case 4: csrvec_launch_one<S, V, 4>(n_rows, row_offsets, col, vals, x, y, long_min, long_rows, n_long, stream); break;
+        case 8: csrvec_launch_one<S, V, 8>(n_rows, row_offsets, col, vals, x, y, long_min, long_rows, n_long, stream); break;
+        case 16: csrvec_launch_one<S, V, 16>(n_rows, row_offsets, col, vals, x, y, long_min, long_rows, n_long, stream); break;
+        case 32: csrvec_launch_one<S, V, 32>(n_rows, row_offsets, col, vals, x, y, long_min, long_rows, n_long, stream); break;
+        default: csrvec_launch_one<S, V, 64>(n_rows, row_offsets, col, vals, x, y, long_min, long_rows, n_long, stream); break;
     }
 }
 
-// Rows longer than this many nonzeros per lane go to the long-row kernel (when the caller has their list).
+// Rows longer than this many nonzeros per lane go to the long-row workgroups (when the caller has their list).
 constexpr uint32_t kCsrvecLongPerLane = 16;
 
-// CSR-vector SpMV; with a list of the matrix's rows longer than long_min (csrvec_long_rows), those rows
-// run one workgroup each in a second launch.
+// CSR-vector SpMV; with a list of the matrix's rows longer than long_min (ensure_long_rows), those rows run
+// one workgroup each at the start of the same launch.
 template <typename S>
 g1_status launch_spmv_csrvec(uint32_t n_rows, const uint32_t* row_offsets, const uint32_t* col,
                              const typename S::value_t* vals, const typename S::x_t* x, typename S::y_t* y,
                              uint32_t lanes, hipStream_t stream, const uint32_t* long_rows = nullptr,
                              uint32_t n_long = 0, uint32_t long_min = kInf) {
     if (n_rows == 0) return G1_OK;
-    if (!long_rows || n_long == 0) long_min = kInf;
-    if (vals) csrvec_dispatch<S, true>(lanes, n_rows, row_offsets, col, vals, x, y, long_min, stream);
-    else csrvec_dispatch<S, false>(lanes, n_rows, row_offsets, col, vals, x, y, long_min, stream);
-    G1_LAUNCHED();
-    if (long_min != kInf) {
-        if (vals) spmv_csrvec_long_kernel<S, true><<<dim3(n_long), dim3(256), 0, stream>>>(long_rows, row_offsets, col,
-                                                                                          vals, x, y);
-        else spmv_csrvec_long_kernel<S, false><<<dim3(n_long), dim3(256), 0, stream>>>(long_rows, row_offsets, col,
-                                                                                      vals, x, y);
-        G1_LAUNCHED();
+    if (!long_rows || n_long == 0) {
+        long_min = kInf;
+        n_long = 0;
     }
+    if (vals) csrvec_dispatch<S, true>(lanes, n_rows, row_offsets, col, vals, x, y, long_min, long_rows, n_long, stream);
+    else csrvec_dispatch<S, false>(lanes, n_rows, row_offsets, col, vals, x, y, long_min, long_rows, n_long, stream);
+    G1_LAUNCHED();
     return G1_OK;
 }
 

@@ -275,15 +275,39 @@ __global__ void __launch_bounds__(256) spmv_merge_fixup_kernel(uint32_t num_tile
 }
 
 // CSR-vector: L lanes per row (L a power of two dividing 64), lanes stride over the row, then a
-// shuffle tree folds lane results with lower lanes on the left. Rows longer than long_min are left to
-// spmv_csrvec_long_kernel (their y is not written here).
+// shuffle tree folds lane results with lower lanes on the left. Rows longer than long_min (a list,
+// long_rows[0 .. n_long), built once per matrix) are taken by the first n_long workgroups of the same
+// launch, one row each: 256 threads striding over it, each folding its entries left to right, then a
+// fixed-order tree across the workgroup (lower threads on the left). A hub row of a skewed graph then no
+// longer holds one group of L lanes for its whole length, and it runs beside the short rows.
 template <typename S, bool kHasValues, int L>
 __global__ void __launch_bounds__(256)
 spmv_csrvec_kernel(uint32_t n_rows, const uint32_t* __restrict__ row_offsets, const uint32_t* __restrict__ col,
                    const typename S::value_t* __restrict__ vals, const typename S::x_t* __restrict__ x,
-                   typename S::y_t* __restrict__ y, uint32_t long_min) {
+                   typename S::y_t* __restrict__ y, uint32_t long_min, const uint32_t* __restrict__ long_rows,
+                   uint32_t n_long) {
     using acc_t = typename S::acc_t;
-    const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (blockIdx.x < n_long) {
+        __shared__ acc_t s_acc[256];
+        const uint32_t row = long_rows[blockIdx.x];
+        const uint32_t beg = row_offsets[row];
+        const uint32_t end = row_offsets[row + 1];
+        acc_t acc = S::identity();
+        for (uint32_t e = beg + threadIdx.x; e < end; e += 256) {
+            const typename S::value_t w = kHasValues ? vals[e] : typename S::value_t(1);
+            S::add(acc, S::combine(w, x[col[e]]));
+        }
+        s_acc[threadIdx.x] = acc;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t d = 128; d >= 1; d >>= 1) {
+            if (threadIdx.x < d) s_acc[threadIdx.x] = S::merge(s_acc[threadIdx.x], s_acc[threadIdx.x + d]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) y[row] = S::finalize(s_acc[0]);
+        return;
+    }
+    const uint64_t gid = static_cast<uint64_t>(blockIdx.x - n_long) * 256 + threadIdx.x;
     const uint64_t row = gid / L;
     const uint32_t lane = static_cast<uint32_t>(gid % L);
     acc_t acc = S::identity();
@@ -305,34 +329,6 @@ spmv_csrvec_kernel(uint32_t n_rows, const uint32_t* __restrict__ row_offsets, co
         if (lane + off < static_cast<uint32_t>(L)) acc = S::merge(acc, other);
     }
     if (mine && lane == 0) y[row] = S::finalize(acc);
-}
-
-// CSR-vector's long rows (a list built once per matrix): one workgroup per row, 256 threads striding over
-// it, each folding its entries left to right, then a fixed-order tree across the workgroup (lower threads
-// on the left). A hub row of a skewed graph no longer holds one group of L lanes for its whole length.
-template <typename S, bool kHasValues>
-__global__ void __launch_bounds__(256)
-spmv_csrvec_long_kernel(const uint32_t* __restrict__ long_rows, const uint32_t* __restrict__ row_offsets,
-                        const uint32_t* __restrict__ col, const typename S::value_t* __restrict__ vals,
-                        const typename S::x_t* __restrict__ x, typename S::y_t* __restrict__ y) {
-    using acc_t = typename S::acc_t;
-    __shared__ acc_t s_acc[256];
-    const uint32_t row = long_rows[blockIdx.x];
-    const uint32_t beg = row_offsets[row];
-    const uint32_t end = row_offsets[row + 1];
-    acc_t acc = S::identity();
-    for (uint32_t e = beg + threadIdx.x; e < end; e += 256) {
-        const typename S::value_t w = kHasValues ? vals[e] : typename S::value_t(1);
-        S::add(acc, S::combine(w, x[col[e]]));
-    }
-    s_acc[threadIdx.x] = acc;
-    __syncthreads();
-#pragma unroll
-    for (uint32_t d = 128; d >= 1; d >>= 1) {
-        if (threadIdx.x < d) s_acc[threadIdx.x] = S::merge(s_acc[threadIdx.x], s_acc[threadIdx.x + d]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) y[row] = S::finalize(s_acc[0]);
 }
 
 }  // namespace g1
