@@ -152,6 +152,7 @@ def batched_queries(g1, g, host, count: int, seed: int) -> dict:
                for c, s in enumerate(srcs))
     out["sssp"] = {
         "k_autotuned": k_auto, "ms_by_k": {str(k): round(v, 3) for k, v in widths.items()},
+        "merge_tile_autotuned": res.stats["merge_tile"],
         "batch_ms": round(t_batch * 1e3, 3), "sequential_ms": round(t_seq * 1e3, 3),
         "batch_qps": round(count / t_batch, 1), "sequential_qps": round(count / t_seq, 1),
         "speedup": round(t_seq / t_batch, 3), "iterations_sum": res.stats["iterations_sum"],

@@ -329,6 +329,10 @@ class Graph:
              int(weighted), C.byref(best), stream)
         return best.value
 
+    def set_merge_tile(self, tile_items: int) -> None:
+        """Merge tile (rows + nonzeros: 512, 1024, 2048 or 4096) of the k = 64 min-plus batches."""
+        call("g1_batch_set_merge_tile", self._h, C.c_uint32(tile_items))
+
     def sssp_batch(self, sources, k: int = 0, weighted: bool = True, out: DeviceArray | None = None, stream=None,
                    to_host: bool = True):
         """Distances (weighted) or BFS depths of every source: values[c] is the run from sources[c]."""

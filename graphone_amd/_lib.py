@@ -88,10 +88,12 @@ class BatchStats(C.Structure):
         ("tiles", C.c_uint32),
         ("iterations_sum", C.c_uint64),
         ("elapsed_ms", C.c_double),
+        ("merge_tile", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 class ServerConfig(C.Structure):
@@ -202,6 +204,7 @@ _SIGNATURES = {
     "g1_pagerank": (_st, [_vp, C.c_float, _u32, C.c_double, _i32, _i32, _vp, C.POINTER(RunStats), _vp]),
     "g1_ppr": (_st, [_vp, _u32, C.c_float, _u32, C.c_double, _vp, C.POINTER(RunStats), _vp]),
     "g1_batch_autotune": (_st, [_vp, _i32, _pu32, _u32, _i32, _pu32, _vp]),
+    "g1_batch_set_merge_tile": (_st, [_vp, _u32]),
     "g1_road_batch": (_st, [_vp, _pu32, _pu32, _u32, _vp, _pu32, _vp]),
     "g1_sssp_batch": (_st, [_vp, _pu32, _u32, _u32, _i32, _vp, C.POINTER(BatchStats), _vp]),
     "g1_ppr_batch": (_st, [_vp, _pu32, _u32, C.c_float, _u32, C.c_double, _u32, _vp, C.POINTER(BatchStats), _vp]),

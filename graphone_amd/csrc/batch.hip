@@ -18,6 +18,9 @@
 #include "session.hpp"
 #include "spmv_launch.hpp"
 
+// Merge tiles the k = 64 min-plus kernel may run on (kWaveTileItems below).
+constexpr uint32_t kWaveTileCount = 4;
+
 struct BatchWork {
     uint32_t K = 0;
     uint32_t* D = nullptr;                                 // n x K distances
@@ -27,10 +30,10 @@ struct BatchWork {
     float* R[2] = {nullptr, nullptr};                      // PPR ranks, n x K
     float* X = nullptr;
     float* Y = nullptr;
-    hipGraphExec_t minplus_exec[2] = {nullptr, nullptr};   // two captured iterations, [weighted]
+    hipGraphExec_t minplus_exec[kWaveTileCount][2] = {};   // two captured iterations, [merge tile][weighted]
     int16_t* D16 = nullptr;                                // n x 64 distances, 16-bit storage (k = 64)
     uint32_t* overflow = nullptr;                          // set when a distance does not fit 16 bits
-    hipGraphExec_t minplus16_exec[2] = {nullptr, nullptr};
+    hipGraphExec_t minplus16_exec[kWaveTileCount][2] = {};
     uint32_t* act = nullptr;                               // k = 64: bitmap of last iteration's changed vertices
 };
 
@@ -116,6 +119,28 @@ g1_status ensure_row_starts(g1_graph* g, hipStream_t s) {
     return G1_OK;
 }
 
+// Merge tiles (rows + nonzeros per wavefront) the autotuner tries for the k = 64 min-plus kernel. Min is
+// order-free, so every partition gives the same bits; [kWaveTileOwn] is the graph's own partition (t_ws), shared
+// with the single-source SpMV and the plus-times kernels, whose fold order it fixes.
+constexpr uint32_t kWaveTileItems[kWaveTileCount] = {kMergeTile / 4, kMergeTile / 2, kMergeTile, kMergeTile * 2};
+constexpr uint32_t kWaveTileOwn = 2;  // kWaveTileItems[kWaveTileOwn] == kMergeTile
+static_assert(sizeof(g1_graph::wave_coords) / sizeof(void*) == kWaveTileCount, "one partition per merge tile");
+
+// The merge partition of the in-edge CSR at kWaveTileItems[idx]; built once per graph and size (caller
+// holds g->mu), finished before it returns.
+g1_status ensure_wave_coords(g1_graph* g, uint32_t idx, hipStream_t s) {
+    if (idx == kWaveTileOwn || g->wave_coords[idx] || g->n == 0) return G1_OK;
+    const uint32_t tiles = merge_num_tiles_for(g->n, g->m, kWaveTileItems[idx]);
+    MergeCoord* c = nullptr;
+    G1_TRY(alloc_n(g, &c, static_cast<size_t>(tiles) + 1));
+    merge_partition_kernel<<<(tiles + 1 + 255) / 256, 256, 0, s>>>(g->t_row_off, g->n, g->m, tiles, c,
+                                                                   kWaveTileItems[idx]);
+    G1_LAUNCHED();
+    G1_HIP(hipStreamSynchronize(s));
+    g->wave_coords[idx] = c;
+    return G1_OK;
+}
+
 g1_status ensure_batch(g1_graph* g, uint32_t K, bool ppr, BatchWork** out, hipStream_t s) {
     const int slot = k_slot(K);
     G1_CHECK(slot >= 0, G1_ERR_INVALID_ARG, "column tile k must be 8, 16, 32 or 64 (got %u)", K);
@@ -128,7 +153,9 @@ g1_status ensure_batch(g1_graph* g, uint32_t K, bool ppr, BatchWork** out, hipSt
     BatchWork* b = g->batch[slot];
     const size_t n = g->n ? g->n : 1;
     if (!b->recs) {
-        const size_t rb = std::max(spmm_records_bytes<MinPlusU32, 64>(g->n, g->m),
+        // (min-plus k = 64 may run on the smallest merge tile: four times the graph's tiles)
+        const size_t rb = std::max(static_cast<size_t>(merge_num_tiles_for(g->n, g->m, kWaveTileItems[0])) *
+                                       sizeof(SpmmTileRec<MinPlusU32, 64>),
                                    spmm_records_bytes<PlusTimesKahanF32, 64>(g->n, g->m));
         void* v = nullptr;
         G1_TRY(graph_alloc(g, &v, rb + 256));
@@ -273,14 +300,22 @@ g1_status launch_plustimes(g1_graph* g, const float* X, float* Y, void* recs_v, 
 template <int K, typename DT = uint32_t>
 g1_status launch_minplus(g1_graph* g, DT* D, unsigned long long* mask_prev, unsigned long long* mask_next,
                          void* recs_v, bool weighted, unsigned long long* flag, hipStream_t s, uint32_t* act,
-                         uint32_t* overflow = nullptr) {
-    const uint32_t tiles = merge_num_tiles(g->n, g->m);
+                         uint32_t* overflow = nullptr, uint32_t tile_idx = kWaveTileOwn) {
+    uint32_t tiles = merge_num_tiles(g->n, g->m);
     auto* recs = static_cast<SpmmTileRec<MinPlusU32, K>*>(recs_v);
     const uint32_t* vals = weighted ? g->t_w : nullptr;
     const MergeCoord* coords = merge_coords(g->t_ws);
-    const uint32_t fix_blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
+    uint32_t fix_blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
     if constexpr (K == 64) {
         if (sizeof(DT) == 2 || minplus_wave()) {
+            // the wavefront kernel runs on any merge partition (kWaveTileItems[tile_idx])
+            if (tile_idx != kWaveTileOwn) {
+                G1_CHECK(tile_idx < kWaveTileCount && g->wave_coords[tile_idx] != nullptr, G1_ERR_INTERNAL,
+                         "min-plus k = 64: merge partition %u missing", tile_idx);
+                tiles = merge_num_tiles_for(g->n, g->m, kWaveTileItems[tile_idx]);
+                coords = static_cast<const MergeCoord*>(g->wave_coords[tile_idx]);
+                fix_blocks = static_cast<uint32_t>((static_cast<uint64_t>(tiles) * K + 255) / 256);
+            }
             const uint32_t blocks = (tiles + 3) / 4;
             G1_CHECK(g->t_starts != nullptr && act != nullptr, G1_ERR_INTERNAL,
                      "min-plus k = 64: row-start or activity bitmap missing");
@@ -334,8 +369,8 @@ g1_status launch_minplus(g1_graph* g, DT* D, unsigned long long* mask_prev, unsi
 // nothing; the count includes that last, unchanged iteration. The pair is captured once per (graph, k,
 // storage, weighted) on a private stream; the pointers it holds never change.
 template <int K, typename DT>
-g1_status minplus_replay(g1_graph* g, BatchWork* b, DT* D, hipGraphExec_t& exec, bool weighted, uint32_t* iterations,
-                         hipStream_t s) {
+g1_status minplus_replay(g1_graph* g, BatchWork* b, DT* D, hipGraphExec_t& exec, bool weighted, uint32_t tile_idx,
+                         uint32_t* iterations, hipStream_t s) {
     uint32_t* ovf = sizeof(DT) == 2 ? b->overflow : nullptr;
     if (!exec) {
         hipStream_t cs = nullptr;
@@ -345,10 +380,11 @@ g1_status minplus_replay(g1_graph* g, BatchWork* b, DT* D, hipGraphExec_t& exec,
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             clear_flags_kernel<<<1, 64, 0, cs>>>(b->changed);
-            st = launch_minplus<K, DT>(g, D, b->mask[0], b->mask[1], b->recs, weighted, b->changed, cs, b->act, ovf);
+            st = launch_minplus<K, DT>(g, D, b->mask[0], b->mask[1], b->recs, weighted, b->changed, cs, b->act, ovf,
+                                       tile_idx);
             if (st == G1_OK)
                 st = launch_minplus<K, DT>(g, D, b->mask[1], b->mask[0], b->recs, weighted, b->changed + 1, cs, b->act,
-                                           ovf);
+                                           ovf, tile_idx);
             e = hipStreamEndCapture(cs, &graph);
         }
         (void)hipStreamDestroy(cs);
@@ -377,6 +413,9 @@ g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host,
                            uint32_t* out, uint32_t* iterations, hipStream_t s) {
     const uint32_t n = g->n;
     G1_HIP(hipMemcpyAsync(g->seeds_dev, srcs_host, cols * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    // the autotuned merge tile (k = 64 wavefront kernel only; the other kernels use the graph's partition)
+    const uint32_t tile = (K == 64 && g->minplus_tile < kWaveTileCount) ? g->minplus_tile : kWaveTileOwn;
+    G1_TRY(ensure_wave_coords(g, tile, s));
     if constexpr (K == 64) {
         if (b->D16 && (!weighted || g->max_w <= 0xFFFF0000u)) {
             // 16-bit storage first; a distance that does not fit voids the run, which is redone in 32 bits.
@@ -385,8 +424,8 @@ g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host,
             G1_LAUNCHED();
             minplus_batch_seed_kernel<64, int16_t><<<1, 64, 0, s>>>(b->D16, b->mask[0], g->seeds_dev, cols);
             G1_LAUNCHED();
-            G1_TRY((minplus_replay<64, int16_t>(g, b, b->D16, b->minplus16_exec[weighted ? 1 : 0], weighted,
-                                                 iterations, s)));
+            G1_TRY((minplus_replay<64, int16_t>(g, b, b->D16, b->minplus16_exec[tile][weighted ? 1 : 0], weighted,
+                                                 tile, iterations, s)));
             uint32_t* hov = reinterpret_cast<uint32_t*>(g->h_ctr + 14);
             G1_HIP(hipMemcpyAsync(hov, b->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             G1_HIP(hipStreamSynchronize(s));
@@ -402,7 +441,8 @@ g1_status minplus_tile_run(g1_graph* g, BatchWork* b, const uint32_t* srcs_host,
     G1_LAUNCHED();
     minplus_batch_seed_kernel<K, uint32_t><<<1, 64, 0, s>>>(b->D, b->mask[0], g->seeds_dev, cols);
     G1_LAUNCHED();
-    G1_TRY((minplus_replay<K, uint32_t>(g, b, b->D, b->minplus_exec[weighted ? 1 : 0], weighted, iterations, s)));
+    G1_TRY((minplus_replay<K, uint32_t>(g, b, b->D, b->minplus_exec[tile][weighted ? 1 : 0], weighted, tile,
+                                        iterations, s)));
     batch_columns_out_kernel<K, uint32_t><<<(n + 63) / 64, 256, 0, s>>>(n, b->D, cols, out);
     G1_LAUNCHED();
     return G1_OK;
@@ -527,7 +567,8 @@ g1_status ppr_batch_dispatch(g1_graph* g, const uint32_t* seeds, uint32_t count,
 }
 
 // Times one full column tile of every width on sources drawn from `srcs` (repeated as needed) and
-// returns the width with the lowest device time per source.
+// returns the width with the lowest device time per source. When min-plus picks k = 64, the merge tile of
+// its wavefront kernel (kWaveTileItems) is timed the same way and kept in g->minplus_tile.
 g1_status autotune(g1_graph* g, int kind, const uint32_t* srcs, uint32_t count, bool weighted, uint32_t* best,
                    hipStream_t s) {
     G1_CHECK(count > 0, G1_ERR_INVALID_ARG, "autotune needs at least one source");
@@ -539,10 +580,9 @@ g1_status autotune(g1_graph* g, int kind, const uint32_t* srcs, uint32_t count, 
     double best_per = 1e300;
     uint32_t best_k = 64;
     g1_status status = G1_OK;
-    for (uint32_t k : {8u, 16u, 32u, 64u}) {
+    // one tile of k columns: an untimed first pass (allocation, graph capture), then the timed pass
+    auto time_tile = [&](uint32_t k, double* ms) {
         g1_batch_stats st;
-        memset(&st, 0, sizeof(st));
-        // untimed first pass (allocation, graph capture), then the timed pass
         for (int pass = 0; pass < 2 && status == G1_OK; ++pass) {
             memset(&st, 0, sizeof(st));
             if (hipEventRecord(g->ev[0], s) != hipSuccess) { status = set_error(G1_ERR_HIP, "event"); break; }
@@ -551,13 +591,38 @@ g1_status autotune(g1_graph* g, int kind, const uint32_t* srcs, uint32_t count, 
                                                      static_cast<uint32_t*>(out), &st, s);
             if (status == G1_OK) status = timed(g, s, &st);
         }
+        *ms = st.elapsed_ms;
+    };
+    if (kind == 0) g->minplus_tile = kWaveTileOwn;  // widths are compared on the graph's own partition
+    double ms64 = 0.0;
+    for (uint32_t k : {8u, 16u, 32u, 64u}) {
+        double ms = 0.0;
+        time_tile(k, &ms);
         if (status != G1_OK) break;
-        const double per = st.elapsed_ms / k;
+        if (k == 64) ms64 = ms;
+        const double per = ms / k;
         if (per < best_per) {
             best_per = per;
             best_k = k;
         }
     }
+    if (status == G1_OK && kind == 0 && best_k == 64) {
+        uint32_t best_tile = kWaveTileOwn;
+        double best_ms = ms64;
+        for (uint32_t idx = 0; idx < kWaveTileCount; ++idx) {
+            if (idx == kWaveTileOwn) continue;  // timed above
+            g->minplus_tile = idx;
+            double ms = 0.0;
+            time_tile(64, &ms);
+            if (status != G1_OK) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_tile = idx;
+            }
+        }
+        g->minplus_tile = best_tile;
+    }
+    if (status != G1_OK && kind == 0) g->minplus_tile = kWaveTileOwn;
     (void)hipFree(out);
     G1_TRY(status);
     *best = best_k;
@@ -895,10 +960,12 @@ void session_release(Session* ss, uint32_t col) {
 void batch_release(g1_graph* g) {
     for (BatchWork*& b : g->batch) {
         if (!b) continue;
-        for (hipGraphExec_t& e : b->minplus_exec)
-            if (e) (void)hipGraphExecDestroy(e);
-        for (hipGraphExec_t& e : b->minplus16_exec)
-            if (e) (void)hipGraphExecDestroy(e);
+        for (auto& row : b->minplus_exec)
+            for (hipGraphExec_t& e : row)
+                if (e) (void)hipGraphExecDestroy(e);
+        for (auto& row : b->minplus16_exec)
+            for (hipGraphExec_t& e : row)
+                if (e) (void)hipGraphExecDestroy(e);
         delete b;  // device areas belong to the graph's allocation list
         b = nullptr;
     }
@@ -907,6 +974,20 @@ void batch_release(g1_graph* g) {
 }  // namespace g1
 
 extern "C" {
+
+g1_status g1_batch_set_merge_tile(g1_graph* g, uint32_t tile_items) {
+    using namespace g1;
+    G1_CHECK(g != nullptr, G1_ERR_INVALID_ARG, "g1_batch_set_merge_tile: null graph");
+    std::lock_guard<std::mutex> lock(g->mu);
+    for (uint32_t i = 0; i < kWaveTileCount; ++i) {
+        if (kWaveTileItems[i] == tile_items) {
+            g->minplus_tile = i;
+            return G1_OK;
+        }
+    }
+    return set_error(G1_ERR_INVALID_ARG, "g1_batch_set_merge_tile: tile must be 512, 1024, 2048 or 4096 (got %u)",
+                     tile_items);
+}
 
 g1_status g1_batch_autotune(g1_graph* g, int32_t kind, const uint32_t* sources, uint32_t count, int32_t weighted,
                             uint32_t* best_k, g1_stream_t stream) {
@@ -938,6 +1019,7 @@ g1_status g1_sssp_batch(g1_graph* g, const uint32_t* sources, uint32_t count, ui
         k = g->best_k[0];
     }
     stats->k = k;
+    stats->merge_tile = k == 64 ? kWaveTileItems[g->minplus_tile < kWaveTileCount ? g->minplus_tile : kWaveTileOwn] : kMergeTile;
     G1_HIP(hipEventRecord(g->ev[0], s));
     G1_TRY(sssp_batch_dispatch(g, sources, count, k, weighted != 0, dist_dev, stats, s));
     return timed(g, s, stats);
@@ -961,6 +1043,7 @@ g1_status g1_ppr_batch(g1_graph* g, const uint32_t* seeds, uint32_t count, float
         k = g->best_k[1];
     }
     stats->k = k;
+    stats->merge_tile = kMergeTile;
     G1_HIP(hipEventRecord(g->ev[0], s));
     G1_TRY(ppr_batch_dispatch(g, seeds, count, k, restart, max_iter, tol, rank_dev, stats, s));
     return timed(g, s, stats);

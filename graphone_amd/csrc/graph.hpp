@@ -80,6 +80,8 @@ struct g1_graph {
     uint32_t* seeds_dev = nullptr;  // seed vertex per column (kBatchCols)
     struct BatchWork* batch[4] = {nullptr, nullptr, nullptr, nullptr};  // per column count 8, 16, 32, 64
     uint32_t best_k[2] = {0, 0};                                        // autotuned k: [0] min-plus, [1] PPR
+    uint32_t minplus_tile = 2;  // merge tile of the k = 64 min-plus kernel: index into kWaveTileItems (2: 2048)
+    void* wave_coords[4] = {};  // merge partitions of the in-edge CSR per kWaveTileItems ([2]: t_ws)
     unsigned long long* h_ctr = nullptr;  // pinned host mirror of ctr (16) and a counter set (at [16])
     double* h_scalar = nullptr;           // pinned host scalar
     unsigned long long* h_hist = nullptr;  // pinned copy of the push engines' iteration history (lazy)

@@ -46,9 +46,12 @@ struct MergeTileRec {
 constexpr uint32_t kTileHasEnd = 1u;
 constexpr uint32_t kTileDeferred = 2u;
 
-__host__ __device__ inline uint32_t merge_num_tiles(uint32_t n_rows, uint32_t nnz) {
+__host__ __device__ inline uint32_t merge_num_tiles_for(uint32_t n_rows, uint32_t nnz, uint32_t tile_items) {
     const uint64_t items = static_cast<uint64_t>(n_rows) + nnz;
-    return static_cast<uint32_t>((items + kMergeTile - 1) / kMergeTile);
+    return static_cast<uint32_t>((items + tile_items - 1) / tile_items);
+}
+__host__ __device__ inline uint32_t merge_num_tiles(uint32_t n_rows, uint32_t nnz) {
+    return merge_num_tiles_for(n_rows, nnz, kMergeTile);
 }
 
 // Workspace: coords[num_tiles + 1] then records[num_tiles] (sized for the largest record type).
@@ -83,11 +86,12 @@ __device__ __forceinline__ MergeCoord merge_path_search(uint64_t diag, const uin
 static __global__ void __launch_bounds__(256) merge_partition_kernel(const uint32_t* __restrict__ row_offsets,
                                                               uint32_t n_rows, uint32_t nnz,
                                                               uint32_t num_tiles,
-                                                              MergeCoord* __restrict__ coords) {
+                                                              MergeCoord* __restrict__ coords,
+                                                              uint32_t tile_items = kMergeTile) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t > num_tiles) return;
     const uint64_t total = static_cast<uint64_t>(n_rows) + nnz;
-    uint64_t diag = static_cast<uint64_t>(t) * kMergeTile;
+    uint64_t diag = static_cast<uint64_t>(t) * tile_items;
     if (diag > total) diag = total;
     coords[t] = merge_path_search(diag, row_offsets + 1, n_rows, nnz);
 }

@@ -259,6 +259,8 @@ typedef struct g1_batch_stats {
     uint32_t tiles;           /* column tiles run                                     */
     uint64_t iterations_sum;  /* SpMM iterations summed over the tiles                */
     double elapsed_ms;        /* device time of the whole call (autotuning excluded)  */
+    uint32_t merge_tile;      /* rows + nonzeros per merge tile of the SpMM kernel      */
+    uint32_t reserved;
 } g1_batch_stats;
 
 #define G1_BATCH_MINPLUS 0
@@ -266,9 +268,16 @@ typedef struct g1_batch_stats {
 
 /* Times one tile of every width on up to 64 of `sources` (repeated when fewer) and stores the width with
  * the lowest time per source as the graph's default for `kind`. weighted selects the min-plus variant
- * timed (ignored for G1_BATCH_PPR). */
+ * timed (ignored for G1_BATCH_PPR). When min-plus picks k = 64, the merge tile of its kernel (512, 1024,
+ * 2048 or 4096 rows + nonzeros; every choice gives the same bits) is timed too and kept for the graph's later
+ * batches (g1_batch_stats.merge_tile reports it). */
 g1_status g1_batch_autotune(g1_graph* graph, int32_t kind, const uint32_t* sources, uint32_t count,
                             int32_t weighted, uint32_t* best_k, g1_stream_t stream);
+
+/* Sets the merge tile of the graph's k = 64 min-plus batches (512, 1024, 2048 or 4096 rows + nonzeros;
+ * the autotuner's choice until then, 2048 without one). No counterpart in the reference: it extends
+ * g1_batch_autotune, for callers that tune once and reuse the result. */
+g1_status g1_batch_set_merge_tile(g1_graph* graph, uint32_t tile_items);
 
 /* Multi-source SSSP (weighted != 0) or BFS depths (weighted == 0): dist_dev holds count x n u32,
  * source-major. sources are host memory. */

@@ -58,6 +58,41 @@ def test_bfs_batch_columns_bitexact(graphs, k):
             assert np.array_equal(got.values[c], want), (name, k, c, s)
 
 
+@pytest.mark.parametrize("tile", [512, 1024, 2048, 4096])
+def test_minplus_merge_tiles_bitexact(graphs, tile):
+    """The k = 64 min-plus kernel on each merge partition the autotuner may pick (rows + nonzeros per
+    tile): SSSP and BFS columns equal the oracle's, and so does the 32-bit rerun of a batch whose
+    distances outgrow 16 bits (a 400-vertex path: 3 tiles of 512, 1 of 4096)."""
+    import graphone_amd as g1
+    try:
+        for name, h, g in graphs:
+            g.set_merge_tile(tile)
+            srcs = _sources(h, 64, seed=tile)
+            got = g.sssp_batch(srcs, k=64)
+            assert got.stats["merge_tile"] == tile
+            bfs = g.sssp_batch(srcs, k=64, weighted=False)
+            for c, s in enumerate(srcs):
+                want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, s)
+                assert np.array_equal(got.values[c], want), (name, tile, c, s)
+                want = oracle.bfs_queue(h.row_offsets, h.col_indices, s)
+                assert np.array_equal(bfs.values[c], want), (name, tile, c, s)
+    finally:
+        for _, _, g in graphs:
+            g.set_merge_tile(2048)
+    n = 400
+    src = np.arange(n - 1, dtype=np.uint32)
+    h = g1.HostGraph.from_edges(n, src, src + 1, np.full(n - 1, 250, dtype=np.uint32), directed=False)
+    g = g1.Graph(h)
+    g.set_merge_tile(tile)
+    srcs = [int(s) for s in np.linspace(0, n - 1, 64).astype(np.uint32)]
+    got = g.sssp_batch(srcs, k=64)
+    assert int(got.values.max()) > 0xFFFF
+    for c, s in enumerate(srcs):
+        assert np.array_equal(got.values[c], oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, s)), (tile, c)
+    with pytest.raises(g1.G1Error):
+        g.set_merge_tile(3000)
+
+
 def test_sssp_batch_equals_single_source_engine(graphs):
     name, h, g = graphs[0]
     srcs = _sources(h, 7, seed=3)
@@ -101,6 +136,7 @@ def test_autotune_and_default_width(graphs):
     assert k in WIDTHS
     got = g.sssp_batch(srcs)  # k = 0: the tuned width
     assert got.stats["k"] == k
+    assert got.stats["merge_tile"] in (512, 1024, 2048, 4096)
     for c, s in enumerate(srcs):
         assert np.array_equal(got.values[c], oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, s))
     kp = g.autotune("ppr", srcs)
