@@ -396,7 +396,19 @@ constexpr uint32_t kRoadMaxDeg = 16;
 bool road_graph(const g1_graph* g) { return g->n > 0 && g->max_out_deg <= kRoadMaxDeg; }
 
 // near-far bucket width: sixteen heaviest edges (a wide bucket: few splits, few extra relaxations)
-uint32_t road_delta(const g1_graph* g) { return std::max<uint32_t>(16u * std::max<uint32_t>(g->max_w, 1u), 16u); }
+// Bucket width of the road engine's near-far split, a multiple of the largest weight: 16 for the LDS kernel
+// (a wider bucket saves iterations, which cost ~3 us each there), 8 for the global-memory kernel (whose
+// extra relaxations cost more). Measured from vertex 1 on p = 0.6 grids: 213 x 213, weights [1, 64], LDS
+// kernel 1.60 ms at 16 (1.67 at 8, 1.63 at 32); 2048 x 2048, weights [1, 255], global kernel 68.3 ms at 8
+// (76.3 at 16, 68.1 at 4). G1_ROAD_DELTA sets one multiple for both.
+uint32_t road_delta(const g1_graph* g, bool lds) {
+    static const int forced = [] {
+        const char* e = getenv("G1_ROAD_DELTA");
+        return e ? atoi(e) : 0;
+    }();
+    const uint32_t mult = forced > 0 ? static_cast<uint32_t>(forced) : (lds ? 16u : 8u);
+    return std::max<uint32_t>(mult * std::max<uint32_t>(g->max_w, 1u), 16u);
+}
 
 constexpr uint32_t kRoadSlots = 64;  // queries per batched launch
 
@@ -445,10 +457,10 @@ g1_status run_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats* st,
         tok = prof_begin(g, kSssp ? "road_sssp" : "road_bfs", s);
         RoadQuery* dq = static_cast<RoadQuery*>(g->road_q);
         if (pass == 0) {
-            road_lds_sssp_kernel<<<1, kRoadThreads, 0, s>>>(g->n, g->row_off, g->col, g->w, dq, road_delta(g),
+            road_lds_sssp_kernel<<<1, kRoadThreads, 0, s>>>(g->n, g->row_off, g->col, g->w, dq, road_delta(g, true),
                                                             g->road_far_cap, static_cast<uint32_t>(qcap), kHistIters);
         } else {
-            road_query_kernel<<<1, kRoadThreads, 0, s>>>(g->row_off, g->col, g->w, dq, road_delta(g), g->road_far_cap,
+            road_query_kernel<<<1, kRoadThreads, 0, s>>>(g->row_off, g->col, g->w, dq, road_delta(g, false), g->road_far_cap,
                                                          static_cast<uint32_t>(qcap), kHistIters);
         }
         G1_LAUNCHED();
@@ -529,7 +541,7 @@ g1_status road_batch_run(g1_graph* g, const uint32_t* kinds, const uint32_t* sou
         G1_HIP(hipMemcpyAsync(douts, hout.data(), c * sizeof(uint32_t*), hipMemcpyHostToDevice, s));
         road_init_kernel<<<dim3(std::min<uint32_t>(grid_for(g->n, 256, 1024), 1024), c), 256, 0, s>>>(g->n, dq, c);
         G1_LAUNCHED();
-        road_query_kernel<<<c, kRoadThreads, 0, s>>>(g->row_off, g->col, g->w, dq, road_delta(g), g->road_far_cap,
+        road_query_kernel<<<c, kRoadThreads, 0, s>>>(g->row_off, g->col, g->w, dq, road_delta(g, false), g->road_far_cap,
                                                      static_cast<uint32_t>(qcap), kHistIters);
         G1_LAUNCHED();
         road_extract_kernel<<<dim3(std::min<uint32_t>(grid_for(g->n, 256, 1024), 1024), c), 256, 0, s>>>(g->n, dq, douts, c);
