@@ -368,8 +368,9 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-sources", type=int, default=64)
-    ap.add_argument("--pagerank-scale", type=int, default=0,
-                    help="also run PageRank on an unweighted RMAT graph of this scale (24: the RMAT24 workload)")
+    ap.add_argument("--pagerank-scale", type=int, default=24,
+                    help="PageRank on an unweighted RMAT graph of this scale (default 24: the RMAT24 workload; "
+                         "0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 1 GiB device-to-device copy: the known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
