@@ -278,6 +278,35 @@ __global__ void __launch_bounds__(256) spmv_merge_fixup_kernel(uint32_t num_tile
     y[row] = S::finalize(acc);
 }
 
+// One lane's share of a CSR-vector row: entries e, e + STRIDE, ... below end, folded into acc in that
+// order. Four column indices are loaded before their four gathers, so a long share keeps four
+// independent gathers in flight instead of one (the fold order, and so the result, is unchanged).
+template <typename S, bool kHasValues, uint32_t STRIDE>
+__device__ __forceinline__ void csrvec_fold(const uint32_t* __restrict__ col,
+                                            const typename S::value_t* __restrict__ vals,
+                                            const typename S::x_t* __restrict__ x, uint32_t e, uint32_t end,
+                                            typename S::acc_t& acc) {
+    using value_t = typename S::value_t;
+    for (; e < end && end - e > 3 * STRIDE; e += 4 * STRIDE) {
+        uint32_t c[4];
+        value_t w[4];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) {
+            c[r] = col[e + r * STRIDE];
+            w[r] = kHasValues ? vals[e + r * STRIDE] : value_t(1);
+        }
+        typename S::x_t xv[4];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) xv[r] = x[c[r]];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) S::add(acc, S::combine(w[r], xv[r]));
+    }
+    for (; e < end; e += STRIDE) {
+        const value_t w = kHasValues ? vals[e] : value_t(1);
+        S::add(acc, S::combine(w, x[col[e]]));
+    }
+}
+
 // CSR-vector: L lanes per row (L a power of two dividing 64), lanes stride over the row, then a
 // shuffle tree folds lane results with lower lanes on the left. Rows longer than long_min (a list,
 // long_rows[0 .. n_long), built once per matrix) are taken by the first n_long workgroups of the same
@@ -297,10 +326,7 @@ spmv_csrvec_kernel(uint32_t n_rows, const uint32_t* __restrict__ row_offsets, co
         const uint32_t beg = row_offsets[row];
         const uint32_t end = row_offsets[row + 1];
         acc_t acc = S::identity();
-        for (uint32_t e = beg + threadIdx.x; e < end; e += 256) {
-            const typename S::value_t w = kHasValues ? vals[e] : typename S::value_t(1);
-            S::add(acc, S::combine(w, x[col[e]]));
-        }
+        csrvec_fold<S, kHasValues, 256>(col, vals, x, beg + threadIdx.x, end, acc);
         s_acc[threadIdx.x] = acc;
         __syncthreads();
 #pragma unroll
@@ -320,12 +346,7 @@ spmv_csrvec_kernel(uint32_t n_rows, const uint32_t* __restrict__ row_offsets, co
         const uint32_t beg = row_offsets[row];
         const uint32_t end = row_offsets[row + 1];
         mine = end - beg <= long_min;
-        if (mine) {
-            for (uint32_t e = beg + lane; e < end; e += L) {
-                const typename S::value_t w = kHasValues ? vals[e] : typename S::value_t(1);
-                S::add(acc, S::combine(w, x[col[e]]));
-            }
-        }
+        if (mine) csrvec_fold<S, kHasValues, L>(col, vals, x, beg + lane, end, acc);
     }
 #pragma unroll
     for (int off = L / 2; off >= 1; off >>= 1) {
