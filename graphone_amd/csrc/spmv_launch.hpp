@@ -1,6 +1,8 @@
 // Host-side launchers of the SpMV kernels, shared by the C-ABI exports and the engines.
 #pragma once
 
+#include <stdlib.h>
+
 #include <hip/hip_runtime.h>
 
 #include "g1/spmv_merge.hpp"
@@ -9,10 +11,21 @@
 namespace g1 {
 
 inline uint32_t csrvec_lanes(uint32_t n_rows, uint32_t nnz) {
-    // Average row length rounded up to a power of two, at most one wave per row.
+    // Half the average row length, rounded up to a power of two, at most one wave per row: on skewed graphs
+    // most rows are far shorter than the average, and rows past 16 lanes' worth run one workgroup each.
+    // Measured on RMAT22 (CSR-vector BFS / SSSP) and RMAT24 (PageRank per iteration), average 15 nonzeros:
+    // 16 lanes 5.1 / 12.5 / 2.34 ms, 8 lanes 3.8 / 9.1 / 2.01 ms, 4 lanes 3.6 / 9.2 / 2.06 ms. G1_CSRVEC_LANES
+    // (a power of two up to 64) overrides it.
+    static const uint32_t forced = [] {
+        const char* e = getenv("G1_CSRVEC_LANES");
+        const int v = e ? atoi(e) : 0;
+        return (v > 0 && v <= 64 && (v & (v - 1)) == 0) ? static_cast<uint32_t>(v) : 0u;
+    }();
+    if (forced) return forced;
     const uint32_t avg = n_rows ? nnz / n_rows : 0;
+    const uint32_t half = (avg + 1) / 2;
     uint32_t lanes = 1;
-    while (lanes < avg && lanes < 64) lanes <<= 1;
+    while (lanes < half && lanes < 64) lanes <<= 1;
     return lanes;
 }
 

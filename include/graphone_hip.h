@@ -98,8 +98,8 @@ g1_status g1_spmv_merge_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_t
                                       const float* values_dev, const float* x_dev, float* y_dev,
                                       void* workspace_dev, size_t workspace_bytes, g1_stream_t stream);
 
-/* CSR-vector: a vector of L lanes (L in 1..64, a power of two) per row inside a wave64, L chosen from
- * nnz / n_rows unless lanes > 0 forces it. Replaces SpmvDispatcher<SpmvCudaCsrVector>
+/* CSR-vector: a vector of L lanes (L in 1..64, a power of two) per row inside a wave64, L = half of
+ * nnz / n_rows rounded up to a power of two unless lanes > 0 forces it. Replaces SpmvDispatcher<SpmvCudaCsrVector>
  * (include/GraphOne/archi/blas/spmv/cuda/cuda.cuh:142-193, csr_vector.cuh:20-68), written from its
  * definition (the reference kernel does not compile and is never instantiated). */
 uint32_t g1_spmv_csrvec_lanes(uint32_t n_rows, uint32_t nnz);

@@ -54,8 +54,8 @@ def test_merge_workspace_size_formula():
     a = L.g1_spmv_merge_workspace_bytes(4194304, 65000000)
     tiles = (4194304 + 65000000 + 2047) // 2048
     assert a >= (tiles + 1) * 8 + tiles * 32
-    assert L.g1_spmv_csrvec_lanes(4194304, 65000000) == 16
-    assert L.g1_spmv_csrvec_lanes(45369, 108374) == 2  # integer average 2
+    assert L.g1_spmv_csrvec_lanes(4194304, 65000000) == 8  # average 15: half, rounded up to 2^k
+    assert L.g1_spmv_csrvec_lanes(45369, 108374) == 1  # integer average 2
     assert L.g1_spmv_csrvec_lanes(10, 5000) == 64
 
 
