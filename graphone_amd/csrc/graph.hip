@@ -120,7 +120,7 @@ g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s) {
     const uint32_t* ro_dev = relabelled ? g->pr_row_off : g->t_row_off;
     G1_CHECK(ro_dev != nullptr, G1_ERR_INTERNAL, "long rows of a missing matrix");
     const uint32_t n = g->n;
-    g->long_min = kCsrvecLongPerLane * csrvec_lanes(n, g->m);
+    g->long_min = csrvec_long_per_lane() * csrvec_lanes(n, g->m);
     G1_HIP(hipStreamSynchronize(s));
     std::vector<uint32_t> ro(static_cast<size_t>(n) + 1), rows;
     G1_HIP(hipMemcpy(ro.data(), ro_dev, sizeof(uint32_t) * (n + 1ull), hipMemcpyDeviceToHost));

@@ -88,8 +88,17 @@ inline void csrvec_dispatch(uint32_t lanes, uint32_t n_rows, const uint32_t* row
     }
 }
 
-// Rows longer than this many nonzeros per lane go to the long-row workgroups (when the caller has their list).
-constexpr uint32_t kCsrvecLongPerLane = 16;
+// Rows longer than this many nonzeros per lane go to the long-row workgroups (when the caller has their
+// list): 16, or G1_CSRVEC_LONG (1..1024). At 8 lanes, CSR-vector SSSP on RMAT22 takes 9.0 ms with 16 (9.1
+// with 8, 12.5 with 32, 18.5 with 64); RMAT24 PageRank 2.02 ms per iteration (2.18, 2.00, 1.99).
+inline uint32_t csrvec_long_per_lane() {
+    static const uint32_t v = [] {
+        const char* e = getenv("G1_CSRVEC_LONG");
+        const int x = e ? atoi(e) : 0;
+        return (x > 0 && x <= 1024) ? static_cast<uint32_t>(x) : 16u;
+    }();
+    return v;
+}
 
 // CSR-vector SpMV; with a list of the matrix's rows longer than long_min (ensure_long_rows), those rows run
 // one workgroup each at the start of the same launch.
