@@ -102,6 +102,19 @@ double sssp_noreturn_fraction() {
     return f;
 }
 
+// Workgroups of a no-return advance launch: 1024 (four per CU), or G1_SSSP_NR_BLOCKS (1..kMaxPushBlocks;
+// the per-workgroup stat slots are sized for kMaxPushBlocks). RMAT22 step, 10 steps each: 2.44 ms at 1024
+// and at 768, 2.48 at 1536 and 2048, 2.55 at 896, 2.66 at 512 (measured; counts that are not a multiple
+// of the 256 CUs did worse).
+uint32_t sssp_noreturn_blocks() {
+    static const uint32_t b = [] {
+        const char* e = getenv("G1_SSSP_NR_BLOCKS");
+        const int v = e ? atoi(e) : 0;
+        return (v > 0 && v <= static_cast<int>(kMaxPushBlocks)) ? static_cast<uint32_t>(v) : 1024u;
+    }();
+    return b;
+}
+
 // The push SSSP engine switches an iteration to a pull step when its frontier has at least
 // G1_SSSP_PULL (default 0.8) times the graph's edges; G1_SSSP_PULL=0 disables the switch.
 double sssp_pull_fraction() {
@@ -616,8 +629,8 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
             ++dense;
             SsspPushNoReturn pol{g->dstamp, it + 1};
             tok = prof_begin(g, "sssp_advance_nr", s);
-            push_advance_kernel<SsspPushNoReturn><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w,
-                                                                                         pq, pol, it, g->hist);
+            push_advance_kernel<SsspPushNoReturn><<<sssp_noreturn_blocks(), kPushThreads, 0, s>>>(
+                g->row_off, g->col, g->w, pq, pol, it, g->hist);
             G1_LAUNCHED();
             prof_end(g, tok, s, 0.0);
             tok_c = prof_begin(g, "sssp_collect", s);
