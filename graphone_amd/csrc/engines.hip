@@ -115,6 +115,18 @@ uint32_t sssp_noreturn_blocks() {
     return b;
 }
 
+// Workgroups of a returning SSSP advance launch: kMaxPushBlocks, or G1_SSSP_PUSH_BLOCKS
+// (1..kMaxPushBlocks: segment capacities and stat slots are sized for at most kMaxPushBlocks producers).
+// On the RMAT22 step 1024 and 2048 are within the run-to-run spread (2.43-2.47 ms); 512 is no better.
+uint32_t sssp_push_blocks() {
+    static const uint32_t b = [] {
+        const char* e = getenv("G1_SSSP_PUSH_BLOCKS");
+        const int v = e ? atoi(e) : 0;
+        return (v > 0 && v <= static_cast<int>(kMaxPushBlocks)) ? static_cast<uint32_t>(v) : kMaxPushBlocks;
+    }();
+    return b;
+}
+
 // The push SSSP engine switches an iteration to a pull step when its frontier has at least
 // G1_SSSP_PULL (default 0.8) times the graph's edges; G1_SSSP_PULL=0 disables the switch.
 double sssp_pull_fraction() {
@@ -640,8 +652,8 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
         } else {
             SsspPush pol{g->dstamp, it + 1};
             tok = prof_begin(g, "sssp_advance", s);
-            push_advance_kernel<SsspPush><<<kMaxPushBlocks, kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq, pol,
-                                                                                 it, g->hist);
+            push_advance_kernel<SsspPush><<<sssp_push_blocks(), kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq,
+                                                                                     pol, it, g->hist);
             G1_LAUNCHED();
             prof_end(g, tok, s, 0.0);
         }
