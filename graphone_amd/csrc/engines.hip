@@ -127,6 +127,18 @@ uint32_t sssp_push_blocks() {
     return b;
 }
 
+// Workgroups of a BFS step launch: 1024 (four per CU), or G1_BFS_BLOCKS (1..kMaxPushBlocks: segment
+// capacities and stat slots are sized for at most kMaxPushBlocks producers). RMAT22 BFS from the hub,
+// 10 steps each, twice: 0.228 / 0.226 ms at 1024, 0.230 / 0.232 at 2048, 0.255 at 512.
+uint32_t bfs_step_blocks() {
+    static const uint32_t b = [] {
+        const char* e = getenv("G1_BFS_BLOCKS");
+        const int v = e ? atoi(e) : 0;
+        return (v > 0 && v <= static_cast<int>(kMaxPushBlocks)) ? static_cast<uint32_t>(v) : 1024u;
+    }();
+    return b;
+}
+
 // The push SSSP engine switches an iteration to a pull step when its frontier has at least
 // G1_SSSP_PULL (default 0.8) times the graph's edges; G1_SSSP_PULL=0 disables the switch.
 double sssp_pull_fraction() {
@@ -356,10 +368,10 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
         PushQueues pq = make_queues(g, it);
         tok = prof_begin(g, diropt ? "bfs_step_diropt" : "bfs_advance", s);
         if (diropt) {
-            bfs_step_kernel<true><<<kMaxPushBlocks, kPushThreads, 0, s>>>(
+            bfs_step_kernel<true><<<bfs_step_blocks(), kPushThreads, 0, s>>>(
                 n, g->m, g->row_off, g->col, g->t_row_off, g->t_col, depth, g->fb3, words, pq, it, g->hist);
         } else {
-            bfs_step_kernel<false><<<kMaxPushBlocks, kPushThreads, 0, s>>>(
+            bfs_step_kernel<false><<<bfs_step_blocks(), kPushThreads, 0, s>>>(
                 n, g->m, g->row_off, g->col, g->t_row_off, g->t_col, depth, g->fb3, words, pq, it, g->hist);
         }
         G1_LAUNCHED();
