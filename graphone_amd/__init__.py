@@ -38,6 +38,19 @@ def _engine(e) -> int:
     return int(e)
 
 
+def _check_out(out, count: int, dtype, what: str) -> None:
+    """Rejects a caller-supplied output array that is too small or of the wrong element type before any
+    device work: the library writes `count` elements through the raw pointer."""
+    if out is None:
+        return
+    if not isinstance(out, DeviceArray):
+        raise TypeError(f"{what}: out must be a DeviceArray, got {type(out).__name__}")
+    if out.dtype != np.dtype(dtype):
+        raise ValueError(f"{what}: out has dtype {out.dtype}, {np.dtype(dtype)} needed")
+    if out.count < count:
+        raise ValueError(f"{what}: out holds {out.count} elements, {count} needed")
+
+
 # ----------------------------------------------------------------------------------------- device memory
 
 class DeviceArray:
@@ -282,6 +295,7 @@ class Graph:
         return out
 
     def _run_u32(self, fn: str, src: int, engine, out: DeviceArray | None, stream):
+        _check_out(out, self.n, np.uint32, fn)
         d = out if out is not None else DeviceArray(self.n, np.uint32)
         st = RunStats()
         call(fn, self._h, C.c_uint32(src), C.c_void_p(d.ptr), _engine(engine), C.byref(st), stream)
@@ -299,6 +313,7 @@ class Graph:
 
     def pagerank(self, alpha: float = 0.85, max_iter: int = 50, tol: float = 1e-7, mode: str = "l1",
                  engine="merge", out: DeviceArray | None = None, stream=None, to_host: bool = True):
+        _check_out(out, self.n, np.float32, "g1_pagerank")
         r = out if out is not None else DeviceArray(self.n, np.float32)
         st = RunStats()
         cm = G1_PR_L1_F64 if mode == "l1" else G1_PR_MAX_ABS
@@ -308,6 +323,7 @@ class Graph:
 
     def ppr(self, seed: int, restart: float = 0.15, max_iter: int = 50, tol: float = 1e-7,
             out: DeviceArray | None = None, stream=None, to_host: bool = True):
+        _check_out(out, self.n, np.float32, "g1_ppr")
         r = out if out is not None else DeviceArray(self.n, np.float32)
         st = RunStats()
         call("g1_ppr", self._h, C.c_uint32(seed), C.c_float(restart), C.c_uint32(max_iter), C.c_double(tol),
@@ -337,6 +353,7 @@ class Graph:
                    to_host: bool = True):
         """Distances (weighted) or BFS depths of every source: values[c] is the run from sources[c]."""
         src = self._sources(sources)
+        _check_out(out, src.size * self.n, np.uint32, "g1_sssp_batch")
         d = out if out is not None else DeviceArray(max(1, src.size * self.n), np.uint32)
         st = BatchStats()
         call("g1_sssp_batch", self._h, src.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_uint32(src.size),
@@ -350,7 +367,11 @@ class Graph:
         src = self._sources(sources)
         kk = None
         if kinds is not None:
+            kinds = list(kinds)
+            if len(kinds) != src.size:
+                raise ValueError(f"g1_road_batch: {len(kinds)} kinds for {src.size} sources")
             kk = np.ascontiguousarray([1 if k in ("sssp", 1) else 0 for k in kinds], dtype=np.uint32)
+        _check_out(out, src.size * self.n, np.uint32, "g1_road_batch")
         d = out if out is not None else DeviceArray(max(1, src.size * self.n), np.uint32)
         its = np.zeros(max(1, src.size), np.uint32)
         call("g1_road_batch", self._h, None if kk is None else kk.ctypes.data_as(C.POINTER(C.c_uint32)),
@@ -363,6 +384,7 @@ class Graph:
                   out: DeviceArray | None = None, stream=None, to_host: bool = True):
         """Personalized PageRank of every seed: values[c] equals ppr(seeds[c]) bit for bit."""
         src = self._sources(seeds)
+        _check_out(out, src.size * self.n, np.float32, "g1_ppr_batch")
         r = out if out is not None else DeviceArray(max(1, src.size * self.n), np.float32)
         st = BatchStats()
         call("g1_ppr_batch", self._h, src.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_uint32(src.size),

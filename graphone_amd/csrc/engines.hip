@@ -102,40 +102,37 @@ double sssp_noreturn_fraction() {
     return f;
 }
 
-// Workgroups of a no-return advance launch: 1024 (four per CU), or G1_SSSP_NR_BLOCKS (1..kMaxPushBlocks;
-// the per-workgroup stat slots are sized for kMaxPushBlocks). RMAT22 step, 10 steps each: 2.44 ms at 1024
-// and at 768, 2.48 at 1536 and 2048, 2.55 at 896, 2.66 at 512 (measured; counts that are not a multiple
-// of the 256 CUs did worse).
+// Workgroup count of a push launch from the environment variable `name`, or `dflt`. Valid counts are the
+// multiples of kSegments from kSegments to kMaxPushBlocks: push_segment_capacity() bounds a segment's
+// entries by items / kSegments plus slack, which holds when every segment has the same number of
+// producers (G / kSegments) each taking 1/G of the items. Anything else falls back to `dflt`.
+uint32_t push_blocks_from_env(const char* name, uint32_t dflt) {
+    const char* e = getenv(name);
+    const long v = e ? atol(e) : 0;
+    if (v >= static_cast<long>(kSegments) && v <= static_cast<long>(kMaxPushBlocks) && v % kSegments == 0)
+        return static_cast<uint32_t>(v);
+    return dflt;
+}
+
+// Workgroups of a no-return advance launch: 1024 (four per CU), or G1_SSSP_NR_BLOCKS. RMAT22 step, 10
+// steps each: 2.44 ms at 1024 and at 768, 2.48 at 1536 and 2048, 2.55 at 896, 2.66 at 512 (measured;
+// counts that are not a multiple of the 256 CUs did worse).
 uint32_t sssp_noreturn_blocks() {
-    static const uint32_t b = [] {
-        const char* e = getenv("G1_SSSP_NR_BLOCKS");
-        const int v = e ? atoi(e) : 0;
-        return (v > 0 && v <= static_cast<int>(kMaxPushBlocks)) ? static_cast<uint32_t>(v) : 1024u;
-    }();
+    static const uint32_t b = push_blocks_from_env("G1_SSSP_NR_BLOCKS", 1024u);
     return b;
 }
 
-// Workgroups of a returning SSSP advance launch: kMaxPushBlocks, or G1_SSSP_PUSH_BLOCKS
-// (1..kMaxPushBlocks: segment capacities and stat slots are sized for at most kMaxPushBlocks producers).
-// On the RMAT22 step 1024 and 2048 are within the run-to-run spread (2.43-2.47 ms); 512 is no better.
+// Workgroups of a returning SSSP advance launch: kMaxPushBlocks, or G1_SSSP_PUSH_BLOCKS. On the RMAT22
+// step 1024 and 2048 are within the run-to-run spread (2.43-2.47 ms); 512 is no better.
 uint32_t sssp_push_blocks() {
-    static const uint32_t b = [] {
-        const char* e = getenv("G1_SSSP_PUSH_BLOCKS");
-        const int v = e ? atoi(e) : 0;
-        return (v > 0 && v <= static_cast<int>(kMaxPushBlocks)) ? static_cast<uint32_t>(v) : kMaxPushBlocks;
-    }();
+    static const uint32_t b = push_blocks_from_env("G1_SSSP_PUSH_BLOCKS", kMaxPushBlocks);
     return b;
 }
 
-// Workgroups of a BFS step launch: 1024 (four per CU), or G1_BFS_BLOCKS (1..kMaxPushBlocks: segment
-// capacities and stat slots are sized for at most kMaxPushBlocks producers). RMAT22 BFS from the hub,
-// 10 steps each, twice: 0.228 / 0.226 ms at 1024, 0.230 / 0.232 at 2048, 0.255 at 512.
+// Workgroups of a BFS step launch: 1024 (four per CU), or G1_BFS_BLOCKS. RMAT22 BFS from the hub, 10
+// steps each, twice: 0.228 / 0.226 ms at 1024, 0.230 / 0.232 at 2048, 0.255 at 512.
 uint32_t bfs_step_blocks() {
-    static const uint32_t b = [] {
-        const char* e = getenv("G1_BFS_BLOCKS");
-        const int v = e ? atoi(e) : 0;
-        return (v > 0 && v <= static_cast<int>(kMaxPushBlocks)) ? static_cast<uint32_t>(v) : 1024u;
-    }();
+    static const uint32_t b = push_blocks_from_env("G1_BFS_BLOCKS", 1024u);
     return b;
 }
 

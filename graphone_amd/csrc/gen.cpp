@@ -78,7 +78,8 @@ void parallel_for(size_t n, int threads, F&& f) {
     for (auto& th : pool) th.join();
 }
 
-// Stable LSD radix sort of 64-bit keys that fit in key_bits bits. Result ends in `a`.
+// Stable LSD radix sort of 64-bit keys by their low key_bits bits (higher bits travel with the key and do
+// not take part in the order). Result ends in `a`.
 void radix_sort_u64(std::vector<uint64_t>& a, int key_bits, int threads) {
     const size_t n = a.size();
     if (n < 2) return;
@@ -93,13 +94,14 @@ void radix_sort_u64(std::vector<uint64_t>& a, int key_bits, int threads) {
     uint64_t* dst = tmp.data();
     for (int p = 0; p < passes; ++p) {
         const int shift = p * kBits;
+        const uint64_t dmask = (size_t(1) << std::min(kBits, key_bits - shift)) - 1;
         std::fill(hist.begin(), hist.end(), 0);
         std::vector<std::thread> pool;
         for (int t = 0; t < T; ++t) {
             pool.emplace_back([&, t] {
                 const size_t lo = t * chunk, hi = std::min(n, lo + chunk);
                 size_t* h = &hist[t * kR];
-                for (size_t i = lo; i < hi; ++i) ++h[(src[i] >> shift) & (kR - 1)];
+                for (size_t i = lo; i < hi; ++i) ++h[(src[i] >> shift) & dmask];
             });
         }
         for (auto& th : pool) th.join();
@@ -118,7 +120,7 @@ void radix_sort_u64(std::vector<uint64_t>& a, int key_bits, int threads) {
                 size_t* h = &hist[t * kR];
                 for (size_t i = lo; i < hi; ++i) {
                     const uint64_t k = src[i];
-                    dst[h[(k >> shift) & (kR - 1)]++] = k;
+                    dst[h[(k >> shift) & dmask]++] = k;
                 }
             });
         }
@@ -352,43 +354,35 @@ g1_status g1_host_csr_transpose(const g1_host_csr* in, g1_host_csr* out) {
     const uint32_t m = in->m;
     G1_TRY(alloc_csr(out, n, m, in->weights != nullptr));
     const int threads = host_threads();
-    uint32_t* cnt = out->row_offsets;
-    std::fill(cnt, cnt + static_cast<size_t>(n) + 1, 0u);
-    parallel_for(m, threads, [&](int, size_t lo, size_t hi) {
-        for (size_t e = lo; e < hi; ++e) __atomic_fetch_add(&cnt[in->col_indices[e] + 1], 1u, __ATOMIC_RELAXED);
-    });
-    for (uint32_t r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
-    std::vector<uint32_t> cursor(cnt, cnt + n);
+    // A stable sort of the edges by destination, the edges listed by ascending source: every in-edge row
+    // ends up with its sources ascending and the weights follow their edges (the device transpose of
+    // convert.hip gives the same bytes). Keys: (edge index << vb) | destination, sorted on the low vb bits.
+    const int vb = std::max(1, bits_for(n > 0 ? n - 1 : 0));
+    G1_CHECK(m == 0 || bits_for(m - 1) + vb <= 64, G1_ERR_INVALID_ARG, "g1_host_csr_transpose: graph too large");
+    std::vector<uint64_t> keys(m);
+    std::vector<uint32_t> rowof(m);
     parallel_for(n, threads, [&](int, size_t lo, size_t hi) {
-        for (size_t u = lo; u < hi; ++u) {
+        for (size_t u = lo; u < hi; ++u)
             for (uint32_t e = in->row_offsets[u]; e < in->row_offsets[u + 1]; ++e) {
-                const uint32_t v = in->col_indices[e];
-                const uint32_t p = __atomic_fetch_add(&cursor[v], 1u, __ATOMIC_RELAXED);
-                out->col_indices[p] = static_cast<uint32_t>(u);
-                if (out->weights) out->weights[p] = in->weights[e];
+                rowof[e] = static_cast<uint32_t>(u);
+                keys[e] = (static_cast<uint64_t>(e) << vb) | in->col_indices[e];
             }
+    });
+    radix_sort_u64(keys, vb, threads);
+    const uint64_t vmask = (1ull << vb) - 1;
+    parallel_for(m, threads, [&](int, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            const uint64_t e = keys[i] >> vb;
+            out->col_indices[i] = rowof[e];
+            if (out->weights) out->weights[i] = in->weights[e];
+            // row starts: every vertex in (previous destination, this destination] starts at i
+            const uint32_t v = static_cast<uint32_t>(keys[i] & vmask);
+            const int64_t prev = i == 0 ? -1 : static_cast<int64_t>(keys[i - 1] & vmask);
+            for (int64_t x = prev + 1; x <= static_cast<int64_t>(v); ++x) out->row_offsets[x] = static_cast<uint32_t>(i);
         }
     });
-    // Rows were filled in thread order: sort each row by source id (weights follow).
-    parallel_for(n, threads, [&](int, size_t lo, size_t hi) {
-        std::vector<uint64_t> buf;
-        for (size_t v = lo; v < hi; ++v) {
-            const uint32_t b = out->row_offsets[v], e = out->row_offsets[v + 1];
-            if (e - b < 2) continue;
-            if (!out->weights) {
-                std::sort(out->col_indices + b, out->col_indices + e);
-                continue;
-            }
-            buf.resize(e - b);
-            for (uint32_t i = b; i < e; ++i)
-                buf[i - b] = (static_cast<uint64_t>(out->col_indices[i]) << 32) | out->weights[i];
-            std::sort(buf.begin(), buf.end());
-            for (uint32_t i = b; i < e; ++i) {
-                out->col_indices[i] = static_cast<uint32_t>(buf[i - b] >> 32);
-                out->weights[i] = static_cast<uint32_t>(buf[i - b]);
-            }
-        }
-    });
+    const int64_t last = m == 0 ? -1 : static_cast<int64_t>(keys[m - 1] & vmask);
+    for (int64_t x = last + 1; x <= static_cast<int64_t>(n); ++x) out->row_offsets[x] = m;
     return G1_OK;
 }
 

@@ -1,6 +1,6 @@
 // g1_graph: device-resident CSR (+ transpose) and its work areas. Replaces graph::build +
 // Graph::ToArch (include/GraphOne/graph/builder.h:9-46, graph.h:94-113): the host CSR is uploaded once
-// and stays resident in HBM; the in-edge CSR is built on the host from the same edges.
+// and stays resident in HBM; the in-edge CSR is built from it on the device (convert.hip).
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -61,56 +61,6 @@ g1_status ensure_push_work(g1_graph* g, hipStream_t s) {
     return G1_OK;
 }
 
-// The relabelled copy of the in-edge CSR for plain PageRank, built once on the host: vertices ordered by
-// out-degree, largest first (ties by id), so the sources gathered most often -- the hubs -- share cache
-// lines; a row keeps its in-edges in their original order. Needs the out-weight sums (ensure_pull_work).
-g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s) {
-    if (g->pr_perm || !g->has_t) return G1_OK;
-    const uint32_t n = g->n, m = g->m;
-    G1_HIP(hipStreamSynchronize(s));
-    std::vector<uint32_t> ro(static_cast<size_t>(n) + 1), tro(static_cast<size_t>(n) + 1), tcol(m);
-    std::vector<float> osum(n), twf(g->t_wf ? m : 0);
-    G1_HIP(hipMemcpy(ro.data(), g->row_off, sizeof(uint32_t) * (n + 1ull), hipMemcpyDeviceToHost));
-    G1_HIP(hipMemcpy(tro.data(), g->t_row_off, sizeof(uint32_t) * (n + 1ull), hipMemcpyDeviceToHost));
-    if (m) G1_HIP(hipMemcpy(tcol.data(), g->t_col, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
-    if (m && g->t_wf) G1_HIP(hipMemcpy(twf.data(), g->t_wf, sizeof(float) * m, hipMemcpyDeviceToHost));
-    if (n) G1_HIP(hipMemcpy(osum.data(), g->osum, sizeof(float) * n, hipMemcpyDeviceToHost));
-    // counting sort by out-degree, descending, stable
-    uint32_t maxd = 0;
-    for (uint32_t v = 0; v < n; ++v) maxd = std::max(maxd, ro[v + 1] - ro[v]);
-    std::vector<uint32_t> start(static_cast<size_t>(maxd) + 2, 0), perm(n), inv(n);
-    for (uint32_t v = 0; v < n; ++v) ++start[maxd - (ro[v + 1] - ro[v]) + 1];
-    for (size_t d = 1; d < start.size(); ++d) start[d] += start[d - 1];
-    for (uint32_t v = 0; v < n; ++v) perm[start[maxd - (ro[v + 1] - ro[v])]++] = v;
-    for (uint32_t i = 0; i < n; ++i) inv[perm[i]] = i;
-    std::vector<uint32_t> nro(static_cast<size_t>(n) + 1), ncol(static_cast<size_t>(m) + kCsrPad, 0);
-    std::vector<float> nwf(g->t_wf ? static_cast<size_t>(m) + kCsrPad : 0, 0.0f), nosum(n);
-    nro[0] = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t v = perm[i], b = tro[v], e = tro[v + 1];
-        nro[i + 1] = nro[i] + (e - b);
-        for (uint32_t k = b; k < e; ++k) {
-            ncol[nro[i] + (k - b)] = inv[tcol[k]];
-            if (g->t_wf) nwf[nro[i] + (k - b)] = twf[k];
-        }
-        nosum[i] = osum[v];
-    }
-    G1_TRY(alloc_n(g, &g->pr_perm, n ? n : 1));
-    G1_TRY(alloc_n(g, &g->pr_row_off, static_cast<size_t>(n) + 1));
-    G1_TRY(alloc_n(g, &g->pr_col, static_cast<size_t>(m) + kCsrPad));
-    G1_TRY(alloc_n(g, &g->pr_osum, n ? n : 1));
-    if (g->t_wf) G1_TRY(alloc_n(g, &g->pr_wf, static_cast<size_t>(m) + kCsrPad));
-    if (n) G1_HIP(hipMemcpy(g->pr_perm, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-    G1_HIP(hipMemcpy(g->pr_row_off, nro.data(), sizeof(uint32_t) * (n + 1ull), hipMemcpyHostToDevice));
-    G1_HIP(hipMemcpy(g->pr_col, ncol.data(), sizeof(uint32_t) * ncol.size(), hipMemcpyHostToDevice));
-    if (n) G1_HIP(hipMemcpy(g->pr_osum, nosum.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    if (g->t_wf) G1_HIP(hipMemcpy(g->pr_wf, nwf.data(), sizeof(float) * nwf.size(), hipMemcpyHostToDevice));
-    G1_TRY(graph_alloc(g, &g->pr_ws, merge_workspace_bytes(n, m)));
-    G1_TRY(launch_merge_partition(n, m, g->pr_row_off, g->pr_ws, s));
-    G1_HIP(hipStreamSynchronize(s));
-    return G1_OK;
-}
-
 // The rows of the in-edge CSR (or of PageRank's relabelled copy) longer than long_min nonzeros, for the
 // CSR-vector engine's long-row kernel; built once on the host.
 g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s) {
@@ -156,14 +106,10 @@ g1_status ensure_pull_work(g1_graph* g, bool pagerank) {
         }
     }
     if (pagerank && g->t_w && !g->t_wf) {
-        G1_TRY(alloc_n(g, &g->t_wf, g->m ? g->m : 1));
-        std::vector<uint32_t> hw(g->m);
-        std::vector<float> hf(g->m);
-        if (g->m) {
-            G1_HIP(hipMemcpy(hw.data(), g->t_w, sizeof(uint32_t) * g->m, hipMemcpyDeviceToHost));
-            for (uint32_t i = 0; i < g->m; ++i) hf[i] = static_cast<float>(hw[i]);
-            G1_HIP(hipMemcpy(g->t_wf, hf.data(), sizeof(float) * g->m, hipMemcpyHostToDevice));
-        }
+        // with the padding of the integer weights (zeros), converted on the device
+        G1_TRY(alloc_n(g, &g->t_wf, static_cast<size_t>(g->m) + kCsrPad));
+        G1_TRY(device_u32_to_f32(g->t_w, g->t_wf, static_cast<size_t>(g->m) + kCsrPad, nullptr));
+        G1_HIP(hipDeviceSynchronize());
     }
     return G1_OK;
 }
@@ -297,16 +243,24 @@ g1_status g1_graph_create(g1_graph** out, const g1_host_csr* csr, uint32_t flags
         g->t_col = g->col;
         g->t_w = g->w;
     } else if (flags & G1_GRAPH_TRANSPOSE) {
-        g1_host_csr t;
-        if ((s = g1_host_csr_transpose(csr, &t)) != G1_OK) return fail(s);
+        // the in-edge CSR is built on the device (convert.hip) into padded arrays; its work areas are freed
+        // before the graph is returned
         g->has_t = true;
-        s = alloc_n(g, &g->t_row_off, n1);
-        if (s == G1_OK && hipMemcpy(g->t_row_off, t.row_offsets, n1 * 4, hipMemcpyHostToDevice) != hipSuccess)
-            s = set_error(G1_ERR_HIP, "g1_graph_create: transpose upload failed");
-        if (s == G1_OK) s = upload_padded(&g->t_col, t.col_indices);
-        if (s == G1_OK && t.weights) s = upload_padded(&g->t_w, t.weights);
-        g1_host_csr_free(&t);
+        const size_t mp = static_cast<size_t>(csr->m) + kCsrPad;
+        if ((s = alloc_n(g, &g->t_row_off, n1)) != G1_OK || (s = alloc_n(g, &g->t_col, mp)) != G1_OK ||
+            (csr->weights && (s = alloc_n(g, &g->t_w, mp)) != G1_OK))
+            return fail(s);
+        if ((s = g1_fill_u32(g->t_col + csr->m, 0u, kCsrPad, nullptr)) != G1_OK ||
+            (g->t_w && (s = g1_fill_u32(g->t_w + csr->m, 0u, kCsrPad, nullptr)) != G1_OK))
+            return fail(s);
+        void* tws = nullptr;
+        if (hipMalloc(&tws, transpose_workspace_bytes(csr->n, csr->m, csr->weights != nullptr)) != hipSuccess)
+            return fail(set_error(G1_ERR_OUT_OF_MEMORY, "g1_graph_create: transpose work areas"));
+        s = device_transpose(csr->n, csr->m, g->row_off, g->col, g->w, g->t_row_off, g->t_col, g->t_w, tws, nullptr);
+        const hipError_t sync = hipDeviceSynchronize();
+        (void)hipFree(tws);
         if (s != G1_OK) return fail(s);
+        if (sync != hipSuccess) return fail(set_error(G1_ERR_HIP, "g1_graph_create: transpose failed"));
     }
     if (g->has_t) {
         g->t_ws_bytes = merge_workspace_bytes(g->n, g->m);

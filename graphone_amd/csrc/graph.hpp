@@ -133,6 +133,14 @@ g1_status road_queries(g1_graph* g, const uint32_t* kinds, const uint32_t* sourc
 g1_status ensure_push_work(g1_graph* g, hipStream_t s);  // work areas cleared on stream s
 g1_status ensure_pull_work(g1_graph* g, bool pagerank);
 g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s);  // after ensure_pull_work(g, true)
+
+// On-device CSR conversions (convert.hip): the transpose (workspace of transpose_workspace_bytes), an
+// exclusive scan of u32 (partials: one entry per 4096 items), u32 -> f32.
+size_t transpose_workspace_bytes(uint32_t n, uint32_t m, bool weighted);
+g1_status device_transpose(uint32_t n, uint32_t m, const uint32_t* row_off, const uint32_t* col, const uint32_t* w,
+                           uint32_t* t_row_off, uint32_t* t_col, uint32_t* t_w, void* ws, hipStream_t s);
+g1_status exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* partials, hipStream_t s);
+g1_status device_u32_to_f32(const uint32_t* in, float* out, size_t count, hipStream_t s);
 g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s);
 
 // Profiling: prof_begin records a start event (returns -1 when profiling is off), prof_end the stop

@@ -592,11 +592,18 @@ g1_status g1_server_add_graph(g1_server* sv, g1_graph* graph, uint32_t* graph_id
     uint64_t bytes = 0;
     G1_TRY(g1_graph_info(graph, &n, &m, &bytes));
     G1_CHECK(graph->has_t, G1_ERR_INVALID_ARG, "g1_server_add_graph: the graph needs its transpose");
-    std::lock_guard<std::mutex> lock(sv->mu);
-    G1_CHECK(sv->hbm_used + bytes <= sv->cfg.hbm_budget_bytes, G1_ERR_BUDGET,
-             "g1_server_add_graph: graph (%llu bytes) exceeds the HBM budget", static_cast<unsigned long long>(bytes));
+    {
+        // the budget is checked and the graph's bytes reserved under the lock ...
+        std::lock_guard<std::mutex> lock(sv->mu);
+        G1_CHECK(!sv->stop, G1_ERR_CANCELLED, "g1_server_add_graph: server is shutting down");
+        G1_CHECK(sv->hbm_used + bytes <= sv->cfg.hbm_budget_bytes, G1_ERR_BUDGET,
+                 "g1_server_add_graph: graph (%llu bytes) exceeds the HBM budget",
+                 static_cast<unsigned long long>(bytes));
+        sv->reserve(bytes);
+    }
+    // ... the views are created without it (they allocate, copy and synchronize the device, which must not
+    // stall submit/poll/wait or the workers of the graphs already resident) ...
     auto gs = std::make_unique<GraphSlot>();
-    gs->id = static_cast<uint32_t>(sv->graphs.size());
     gs->g = graph;
     gs->n = n;
     for (uint32_t i = 0; i < sv->cfg.single_workers; ++i) {
@@ -604,11 +611,15 @@ g1_status g1_server_add_graph(g1_server* sv, g1_graph* graph, uint32_t* graph_id
         const g1_status st = g1_graph_create_view(graph, &v);
         if (st != G1_OK) {
             for (g1_graph* x : gs->views) g1_graph_destroy(x);
+            std::lock_guard<std::mutex> lock(sv->mu);
+            sv->unreserve(bytes);
             return st;
         }
         gs->views.push_back(v);
     }
-    sv->reserve(bytes);
+    // ... and the slot is published under the lock again.
+    std::lock_guard<std::mutex> lock(sv->mu);
+    gs->id = static_cast<uint32_t>(sv->graphs.size());
     GraphSlot* raw = gs.get();
     sv->graphs.push_back(std::move(gs));
     raw->worker = std::thread([sv, raw] {

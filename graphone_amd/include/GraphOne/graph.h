@@ -1,6 +1,6 @@
 // Graph containers of the graph_one C++ API: views, the resident Graph and its device view
 // GraphArchRef, and graph::build.
-// Reference: include/GraphOne/graph/graph_view.hpp:10-120, graph.h:13-140, graph_arch_ref.h:19-116,
+// Reference: include/GraphOne/graph/graph_view.hpp:10-89, graph.h:13-140, graph_arch_ref.h:19-116,
 // builder.h:9-46 (same template parameters, accessors and view rules: the "normal" view is the
 // out-edge CSR, "transpose" the in-edge CSR; with both views the transpose is kept beside it).
 #pragma once

@@ -1,6 +1,6 @@
 // Semiring SpMV dispatch of the graph_one C++ API (namespace graph_one::blas).
 // Reference: include/GraphOne/archi/blas/spmv/def.hpp:8-73 (SpmvCsrParams, MakeCsrSpMV, SpmvDispatcher),
-// cpu/naive.hpp:5-20 + cpu/cpu.hpp:8-32 (the host fold), cuda/cuda.cuh:10-114 (CSR-vector and merge-based
+// cpu/naive.hpp:5-20 + cpu/cpu.hpp:8-32 (the host fold), archi/blas/spmv/cuda/cuda.cuh:10-114 (CSR-vector and merge-based
 // dispatchers). The functor contract is the same: static initialize(), combine(nonzero, x), reduce(a, b).
 // The device kinds are SpmvHipMergeBased and SpmvHipCsrVector; their kernels are the gfx950 templates of
 // g1/spmv_merge.hpp instantiated with the caller's functor, and the once-per-matrix merge-path partition

@@ -7,7 +7,7 @@
 // it closes row i. The n_rows + nnz steps are cut into tiles of kMergeTile steps (one workgroup of
 // kMergeThreads threads, kMergeItems consecutive steps per thread). Tile start coordinates are found
 // once per matrix (merge_partition_kernel, the reference's SegmentSearch run once in the dispatcher
-// constructor, cuda/cuda.cuh:89-91) and reused by every multiply.
+// constructor, include/GraphOne/archi/blas/spmv/cuda/cuda.cuh:89-91) and reused by every multiply.
 //
 // Fold order (what makes plus-times results reproducible): inside a thread a row's products are
 // added left to right; a row that crosses thread boundaries is finished by the thread that closes it

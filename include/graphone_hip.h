@@ -83,7 +83,7 @@ g1_status g1_event_elapsed_ms(float* ms, g1_event_t start, g1_event_t stop);
  * Results are deterministic: the plus-times fold order is a function of the matrix only.
  *
  * Merge-path (Merrill & Garland): replaces SpmvDispatcher<SpmvCudaCsrMergeBased>
- * (include/GraphOne/archi/blas/spmv/cuda/cuda.cuh:196-246): GetBufferSize -> workspace_bytes,
+ * (include/GraphOne/archi/blas/spmv/cuda/cuda.cuh:64-114): GetBufferSize -> workspace_bytes,
  * SegmentSearch (run once per matrix) -> partition, SpmvMerge -> spmv_merge_*.
  * ------------------------------------------------------------------------------------------- */
 size_t g1_spmv_merge_workspace_bytes(uint32_t n_rows, uint32_t nnz);
@@ -100,7 +100,7 @@ g1_status g1_spmv_merge_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_t
 
 /* CSR-vector: a vector of L lanes (L in 1..64, a power of two) per row inside a wave64, L = half of
  * nnz / n_rows rounded up to a power of two unless lanes > 0 forces it. Replaces SpmvDispatcher<SpmvCudaCsrVector>
- * (include/GraphOne/archi/blas/spmv/cuda/cuda.cuh:142-193, csr_vector.cuh:20-68), written from its
+ * (include/GraphOne/archi/blas/spmv/cuda/cuda.cuh:10-61, csr_vector.cuh:20-68), written from its
  * definition (the reference kernel does not compile and is never instantiated). */
 uint32_t g1_spmv_csrvec_lanes(uint32_t n_rows, uint32_t nnz);
 g1_status g1_spmv_csrvec_minplus_u32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz,

@@ -97,6 +97,23 @@ def sssp_advance(rp, ci, w, src: int):
     return d, {"iterations": int(st[0]), "frontier_sum": int(st[1]), "edges_sum": int(st[2])}
 
 
+def sssp_advance_omp(rp, ci, w, src: int):
+    """orc_sssp_advance_omp: the advance loop on every host thread (see set_threads)."""
+    rp, ci, w = _u32(rp), _u32(ci), _u32(w)
+    n = rp.size - 1
+    d = np.empty(n, dtype=np.uint32)
+    st = np.zeros(3, dtype=np.uint64)
+    rc = orc().orc_sssp_advance_omp(C.c_uint32(n), _p(rp), _p(ci), _p(w), C.c_uint32(src), _p(d), _p(st))
+    assert rc == 0
+    return d, {"iterations": int(st[0]), "frontier_sum": int(st[1]), "edges_sum": int(st[2])}
+
+
+def set_threads(threads: int = 0) -> int:
+    """Host threads of the oracle's OpenMP loops (0: every thread of the machine); returns the count."""
+    orc().orc_set_threads(C.c_int(int(threads)))
+    return int(orc().orc_get_threads())
+
+
 def minplus_blas(tp, tj, tw, src: int):
     tp, tj, tw = _u32(tp), _u32(tj), _u32(tw)
     n = tp.size - 1

@@ -13,11 +13,17 @@
  * INF = 0xFFFFFFFF for distances, the invalid vertex of include/GraphOne/utils/limits.hpp:54-73.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
 #define ORC_INF 0xFFFFFFFFu
+
+/* Host threads of the OpenMP loops (orc_pagerank_f64 and orc_sssp_advance_omp): every thread of the
+ * machine by default. The results do not depend on it (see each function). */
+void orc_set_threads(int threads) { omp_set_num_threads(threads > 0 ? threads : omp_get_num_procs()); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 /* ---------------------------------------------------------------------------------------------
  * Semiring SpMV, the naive row fold of include/GraphOne/archi/blas/spmv/cpu/naive.hpp:5-20:
@@ -110,6 +116,58 @@ int orc_sssp_advance(uint32_t n, const uint32_t* rp, const uint32_t* ci, const u
                 }
             }
         }
+        uint32_t* t = fin; fin = fout; fout = t;
+        nin = nout;
+    }
+    if (stats) { stats[0] = iter; stats[1] = fsum; stats[2] = esum; }
+    free(fin); free(fout); free(visited);
+    return 0;
+}
+
+/* The same advance/filter loop on every host thread (this build's own variant, timed as the all-core CPU
+ * baseline; the reference has no CPU engine): the frontier's vertices are split among the threads, a
+ * relaxation is an atomic min on d[v] (compare-and-swap loop) that reports an improvement when it lowered
+ * the value, and the visited filter claims v for this iteration with an atomic exchange of its stamp.
+ * Frontier order varies with the threads; the final distances are the unique shortest-path distances.
+ * stats as orc_sssp_advance. */
+int orc_sssp_advance_omp(uint32_t n, const uint32_t* rp, const uint32_t* ci, const uint32_t* w, uint32_t src,
+                         uint32_t* d, uint64_t* stats) {
+    uint32_t* fin = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t* fout = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t* visited = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
+    if (!fin || !fout || !visited) { free(fin); free(fout); free(visited); return -1; }
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < (int64_t)n; ++v) d[v] = ORC_INF;
+    d[src] = 0;
+    uint32_t nin = 1, iter = 0;
+    uint64_t fsum = 0, esum = 0;
+    fin[0] = src;
+    while (nin > 0) {
+        ++iter;
+        uint32_t nout = 0;
+        fsum += nin;
+        uint64_t e_it = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : e_it)
+        for (int64_t i = 0; i < (int64_t)nin; ++i) {
+            const uint32_t u = fin[i];
+            const uint32_t du = __atomic_load_n(&d[u], __ATOMIC_RELAXED);
+            e_it += rp[u + 1] - rp[u];
+            for (uint32_t k = rp[u]; k < rp[u + 1]; ++k) {
+                const uint32_t v = ci[k];
+                const uint32_t nd = du + (w ? w[k] : 1u);
+                uint32_t old = __atomic_load_n(&d[v], __ATOMIC_RELAXED);
+                int improved = 0;
+                while (nd < old) {
+                    if (__atomic_compare_exchange_n(&d[v], &old, nd, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                        improved = 1;
+                        break;
+                    }
+                }
+                if (improved && __atomic_exchange_n(&visited[v], iter, __ATOMIC_RELAXED) != iter)
+                    fout[__atomic_fetch_add(&nout, 1u, __ATOMIC_RELAXED)] = v;
+            }
+        }
+        esum += e_it;
         uint32_t* t = fin; fin = fout; fout = t;
         nin = nout;
     }
@@ -327,20 +385,38 @@ int orc_pagerank_f64(uint32_t n, const uint32_t* rp, const uint32_t* w, const ui
         free(y);
         cur = idx;
     } else {
+        /* Rows are independent and each row sums in storage order, so the iterates do not depend on the
+         * thread count; the L1 norm is summed per block of 65536 vertices, then over the blocks in order. */
+        const uint32_t kBlock = 65536;
+        const uint32_t nblk = (n + kBlock - 1) / kBlock;
+        double* part = (double*)malloc(sizeof(double) * (nblk ? nblk : 1));
+        if (!part) { free(iw); free(r0); free(r1); free(x); return -1; }
         while (it < max_iter) {
-            for (uint32_t u = 0; u < n; ++u) x[u] = ranks[cur][u] * iw[u];
-            delta = 0.0;
-            for (uint32_t v = 0; v < n; ++v) {
-                double s = 0.0;
-                for (uint32_t k = tp[v]; k < tp[v + 1]; ++k) s += x[tj[k]] * (tw ? (double)tw[k] : 1.0);
-                const double rn = base + s + ((personalized && v == seed) ? restart : 0.0);
-                delta += fabs(rn - ranks[cur][v]);
-                ranks[cur ^ 1][v] = rn;
+            const double* rc = ranks[cur];
+            double* rn_out = ranks[cur ^ 1];
+#pragma omp parallel for schedule(static)
+            for (int64_t u = 0; u < (int64_t)n; ++u) x[u] = rc[u] * iw[u];
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int64_t b = 0; b < (int64_t)nblk; ++b) {
+                const uint32_t v0 = (uint32_t)b * kBlock;
+                const uint32_t v1 = v0 + kBlock < n ? v0 + kBlock : n;
+                double dsum = 0.0;
+                for (uint32_t v = v0; v < v1; ++v) {
+                    double s = 0.0;
+                    for (uint32_t k = tp[v]; k < tp[v + 1]; ++k) s += x[tj[k]] * (tw ? (double)tw[k] : 1.0);
+                    const double rn = base + s + ((personalized && v == seed) ? restart : 0.0);
+                    dsum += fabs(rn - rc[v]);
+                    rn_out[v] = rn;
+                }
+                part[b] = dsum;
             }
+            delta = 0.0;
+            for (uint32_t b = 0; b < nblk; ++b) delta += part[b];
             cur ^= 1;
             ++it;
             if (delta < tol) break;
         }
+        free(part);
     }
     memcpy(out, ranks[cur], sizeof(double) * n);
     if (last_delta) *last_delta = delta;

@@ -83,3 +83,45 @@ def test_server_argument_errors_without_a_gpu():
     L = _lib.lib()
     assert L.g1_server_destroy(None) == _lib.G1_OK
     assert L.g1_server_get_stats(None, None) == _lib.G1_ERR_INVALID_ARG
+
+
+def _fake_graph(n):
+    """A Graph object with no device handle: the Python layer's argument checks run before any C call."""
+    g = g1.Graph.__new__(g1.Graph)
+    g._h = C.c_void_p()
+    g.n, g.m, g.weighted = n, 0, True
+    return g
+
+
+def _fake_array(count, dtype):
+    a = g1.DeviceArray.__new__(g1.DeviceArray)
+    a.count, a.dtype, a._ptr = count, np.dtype(dtype), 0
+    a.nbytes = count * a.dtype.itemsize
+    return a
+
+
+def test_output_buffers_checked_before_device_work():
+    """An undersized or mistyped output array is refused on the host: the library would write n (or
+    sources x n) elements through its pointer."""
+    g = _fake_graph(100)
+    for fn, args in ((g.bfs, (0,)), (g.sssp, (0,))):
+        with pytest.raises(ValueError, match="99 elements, 100 needed"):
+            fn(*args, out=_fake_array(99, np.uint32))
+        with pytest.raises(ValueError, match="dtype"):
+            fn(*args, out=_fake_array(100, np.float32))
+    with pytest.raises(ValueError, match="needed"):
+        g.pagerank(out=_fake_array(50, np.float32))
+    with pytest.raises(ValueError, match="dtype"):
+        g.pagerank(out=_fake_array(100, np.uint32))
+    with pytest.raises(ValueError, match="needed"):
+        g.ppr(0, out=_fake_array(10, np.float32))
+    with pytest.raises(ValueError, match="300 needed"):
+        g.sssp_batch([0, 1, 2], out=_fake_array(299, np.uint32))
+    with pytest.raises(ValueError, match="200 needed"):
+        g.ppr_batch([0, 1], out=_fake_array(100, np.float32))
+    with pytest.raises(ValueError, match="200 needed"):
+        g.road_batch([0, 1], out=_fake_array(199, np.uint32))
+    with pytest.raises(ValueError, match="1 kinds for 2 sources"):
+        g.road_batch([0, 1], kinds=["bfs"])
+    with pytest.raises(TypeError):
+        g.bfs(0, out=np.zeros(100, np.uint32))

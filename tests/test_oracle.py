@@ -174,6 +174,49 @@ def test_restated_engines_equal_dijkstra():
             assert np.array_equal(oracle.dijkstra(rp, ci, np.ones_like(w), s), bfs)
 
 
+def test_openmp_advance_equals_dijkstra_at_every_thread_count():
+    """The all-core restated advance loop (orc_sssp_advance_omp) reaches Dijkstra's distances whatever
+    the thread count, with the serial loop's edge total on a graph whose frontiers it cannot reorder
+    (a path)."""
+    rng = np.random.default_rng(5)
+    try:
+        for threads in (1, 3, 8):
+            oracle.set_threads(threads)
+            for n, avg in ((40, 2), (800, 5), (20000, 8)):
+                rp, ci, w = random_csr(rng, n, avg)
+                for s in (0, n // 3):
+                    want = oracle.dijkstra(rp, ci, w, s)
+                    got, st = oracle.sssp_advance_omp(rp, ci, w, s)
+                    assert np.array_equal(got, want), (threads, n, s)
+                    assert np.array_equal(oracle.sssp_advance_omp(rp, ci, None, s)[0], oracle.bfs_queue(rp, ci, s))
+        n = 50
+        rp = np.arange(n + 1, dtype=np.uint32)
+        rp[-1] = n - 1
+        ci = np.arange(1, n, dtype=np.uint32)
+        d, st = oracle.sssp_advance_omp(rp, ci, None, 0)
+        assert np.array_equal(d, np.arange(n, dtype=np.uint32))
+        assert st == oracle.sssp_advance(rp, ci, None, 0)[1]
+    finally:
+        oracle.set_threads(0)
+
+
+def test_pagerank_iterates_do_not_depend_on_threads():
+    """orc_pagerank_f64 splits its rows over the host threads: one thread and many give the same bits and
+    the same iteration count."""
+    rng = np.random.default_rng(6)
+    rp, ci, w = random_csr(rng, 150000, 6)
+    tp, tj, tw = transpose(rp, ci, w)
+    try:
+        oracle.set_threads(1)
+        a, sa = oracle.pagerank_f64(rp, w, tp, tj, tw, 0.85, max_iter=30, tol=1e-9)
+        oracle.set_threads(7)
+        b, sb = oracle.pagerank_f64(rp, w, tp, tj, tw, 0.85, max_iter=30, tol=1e-9)
+    finally:
+        oracle.set_threads(0)
+    assert sa == sb
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
 def test_blas_engine_iteration_count_is_hop_depth():
     """Synchronous rounds: the BLAS loop runs (max hops of a shortest-path tree) + 1 iterations; on a
     path graph with unit weights that is the path length + 1."""
