@@ -72,11 +72,36 @@ def roofline_from_profile(prof: list[dict], scale: int = 22) -> dict | None:
     }
 
 
-def cpu_baseline(host, src: int, budget_s: float) -> dict:
-    """The oracle's restated push engine (oracle/g1_oracle.c orc_sssp_advance) on this host's cores, one
-    thread: the BFS (unit weights) and the SSSP of one step on the same graph, repeated until about 10 s of
-    CPU work (at most `budget_s`) have run. The textbook BFS queue and binary-heap Dijkstra from the same
-    source are timed once beside it."""
+def host_cpu() -> dict:
+    """The host's logical core count (os.cpu_count), the cores this process may run on, and the CPU model
+    (/proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"logical_cores": os.cpu_count(), "usable_cores": usable, "model": model}
+
+
+def cpu_baseline(host, src: int, budget_s: float, gpu_depth: np.ndarray, gpu_dist: np.ndarray) -> dict:
+    """The oracle's restated push engine (oracle/g1_oracle.c orc_sssp_advance, the loop of
+    apps/sssp_adv.cu:48-72 over engine/advance_gc.h) on this host: the BFS (unit weights) and the SSSP of one
+    step on the same graph and source.
+    - one thread (the reference's single-threaded CPU semantics): repeated until about 10 s of CPU work
+      (at most `budget_s`) have run; this is `value`;
+    - every usable core (orc_sssp_advance_omp: the same loop with an atomic-min relaxation, OpenMP), repeated
+      for about 5 s;
+    - the textbook queue BFS and binary-heap Dijkstra from the same source, timed once.
+    The first single-thread run's depths and distances, and the textbook ones, are compared with the GPU
+    step's results bit for bit (`bit_exact`)."""
     import oracle
 
     rp, ci, w = host.row_offsets, host.col_indices, host.weights
@@ -84,14 +109,19 @@ def cpu_baseline(host, src: int, budget_s: float) -> dict:
     target = min(10.0, budget_s)
     edges = 0
     reps = 0
+    d_bfs = d_sssp = None
     t0 = time.perf_counter()
     while True:
-        d_bfs, _ = oracle.sssp_advance(rp, ci, None, src)
-        m_reach = int(deg[d_bfs != oracle.INF].sum())
+        b, _ = oracle.sssp_advance(rp, ci, None, src)
+        if d_bfs is None:
+            d_bfs = b
+        m_reach = int(deg[b != oracle.INF].sum())
         edges += m_reach
         if time.perf_counter() - t0 >= budget_s:
             break
-        oracle.sssp_advance(rp, ci, w, src)
+        s, _ = oracle.sssp_advance(rp, ci, w, src)
+        if d_sssp is None:
+            d_sssp = s
         edges += m_reach
         reps += 1
         if time.perf_counter() - t0 >= target:
@@ -102,13 +132,40 @@ def cpu_baseline(host, src: int, budget_s: float) -> dict:
     else:
         sample = "one BFS from the bench source on the full graph (restated advance engine)"
     out = {"value": round(edges / secs / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
-           "sample": sample, "seconds": round(secs, 3)}
+           "sample": sample, "seconds": round(secs, 3), "host": host_cpu()}
+
+    threads = oracle.set_threads(0)
+    edges = 0
+    reps = 0
+    t0 = time.perf_counter()
+    omp_same = True
+    while True:
+        b, _ = oracle.sssp_advance_omp(rp, ci, None, src)
+        s, _ = oracle.sssp_advance_omp(rp, ci, w, src)
+        if reps == 0:
+            omp_same = bool(np.array_equal(b, d_bfs)) and (d_sssp is None or bool(np.array_equal(s, d_sssp)))
+        edges += 2 * int(deg[b != oracle.INF].sum())
+        reps += 1
+        if time.perf_counter() - t0 >= 5.0:
+            break
+    secs = time.perf_counter() - t0
+    out["all_cores"] = {"value": round(edges / secs / 1e9, 5), "unit": "GTEPS", "threads": threads,
+                        "sample": f"{reps} x (one BFS + one SSSP), the advance loop on every usable core "
+                                  "(this build's OpenMP variant, not the reference's)",
+                        "seconds": round(secs, 3), "same_result_as_one_thread": omp_same}
+
     t1 = time.perf_counter()
-    oracle.bfs_queue(rp, ci, src)
+    q_bfs = oracle.bfs_queue(rp, ci, src)
     t2 = time.perf_counter()
-    oracle.dijkstra(rp, ci, w, src)
+    q_sssp = oracle.dijkstra(rp, ci, w, src)
     t3 = time.perf_counter()
     out["textbook_seconds"] = {"bfs_queue": round(t2 - t1, 3), "dijkstra_binary_heap": round(t3 - t2, 3)}
+    out["bit_exact"] = {
+        "bfs_vs_restated_engine": bool(np.array_equal(gpu_depth, d_bfs)),
+        "bfs_vs_queue_bfs": bool(np.array_equal(gpu_depth, q_bfs)),
+        "sssp_vs_restated_engine": None if d_sssp is None else bool(np.array_equal(gpu_dist, d_sssp)),
+        "sssp_vs_dijkstra": bool(np.array_equal(gpu_dist, q_sssp)),
+    }
     return out
 
 
@@ -190,30 +247,52 @@ def batched_queries(g1, g, host, count: int, seed: int) -> dict:
     return out
 
 
-def grid213_sssp(g1) -> dict:
+def grid213_sssp(g1, cpu: bool) -> dict:
     """The sssp_adv workload on a 213 x 213 four-neighbour grid (each undirected edge kept with p = 0.6, both
     directions with the same u32 weight in [1, 64]), source vertex 1: a latency-bound run (the graph is
-    about 1 MB), reported as time and iterations per engine. Bit-exactness is checked by the GPU tests."""
+    about 1 MB), reported as time and iterations per engine. With `cpu`, the oracle's restated advance
+    loop (one thread) is timed on the same graph and source, and every engine's distances are compared
+    with its result and with Dijkstra's."""
     h = g1.HostGraph.grid(213, 213, 0.6, seed=7).assign_weights(1, 64, seed=8, symmetric=True)
     g = g1.Graph(h, symmetric=True)
     d = g1.DeviceArray(h.n, np.uint32)
     out = {"n": h.n, "m": h.m, "source": 1}
+    got = {}
     for eng, label in (("advance", "advance (road engine: one workgroup)"), ("merge", "blas merge-path")):
         g.sssp(1, eng, out=d, to_host=False)
         runs = [g.sssp(1, eng, out=d, to_host=False) for _ in range(5)]
         ms = sorted(r.stats["elapsed_ms"] for r in runs)[2]
+        got[eng] = d.numpy()
         out[eng] = {"engine": label, "ms": round(ms, 4), "iterations": runs[0].stats["iterations"],
                     "reached": runs[0].stats.get("reached")}
+    if cpu:
+        import oracle
+
+        rp, ci, w = h.row_offsets, h.col_indices, h.weights
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            ref, st = oracle.sssp_advance(rp, ci, w, 1)
+            reps += 1
+            if time.perf_counter() - t0 >= 2.0:
+                break
+        secs = (time.perf_counter() - t0) / reps
+        dj = oracle.dijkstra(rp, ci, w, 1)
+        out["cpu_baseline"] = {"ms": round(secs * 1e3, 3), "cores": 1, "kind": "port",
+                               "iterations": st["iterations"],
+                               "sample": f"{reps} runs of the restated advance loop (apps/sssp_adv.cu), one thread"}
+        out["bit_exact"] = {eng: bool(np.array_equal(v, ref)) and bool(np.array_equal(v, dj))
+                            for eng, v in got.items()}
     return out
 
 
-def pagerank_benchmark(g1, scale: int, seed: int) -> dict:
+def pagerank_benchmark(g1, scale: int, seed: int, cpu: bool) -> dict:
     """PageRank (apps/pagerank_blas.cu formula) on an unweighted RMAT graph (edge factor 16, the main a/b/c/d):
     damping 0.85, FP32 ranks with Kahan row sums, FP64 L1 stop at 1e-7 or 50 iterations, both SpMV engines.
     Per iteration the algorithmic bytes are 4(n+1) + 4m + 4m (x gather) + 4n (y) for the SpMV and 20n for
-    the apply. Size-independent checks on the device results: the two engines' ranks agree (L1, max-abs)
-    and the ranks sum to 1 up to the mass lost at dangling vertices (reported); the FP64 comparison lives in
-    the GPU tests."""
+    the apply. The two engines' ranks are compared with each other, and (with `cpu`) with the oracle's FP64
+    power iteration under the same stop rule on the same graph: L1 and max-abs per engine. The FP64 run is
+    timed on every usable host core (the oracle's OpenMP loop) and reported as the CPU baseline."""
     t0 = time.perf_counter()
     h = g1.HostGraph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=seed + 10)
     gr = g1.Graph(h)
@@ -238,9 +317,31 @@ def pagerank_benchmark(g1, scale: int, seed: int) -> dict:
                     "spmv_frac": round(sp_b / max(sp_ms, 1e-9) / 1e6 / HBM_PEAK_GBS, 4),
                     "last_l1_delta": res.stats["last_delta"]}
         ranks[eng] = r.numpy().astype(np.float64)
+    del gr, r
     diff = np.abs(ranks["merge"] - ranks["csrvec"])
     out["merge_vs_csrvec"] = {"l1": float(diff.sum()), "max_abs": float(diff.max())}
     out["rank_sum"] = float(ranks["merge"].sum())
+    if cpu:
+        import oracle
+
+        t0 = time.perf_counter()
+        t = h.transpose()
+        log(f"host transpose {time.perf_counter() - t0:.1f}s")
+        threads = oracle.set_threads(0)
+        t0 = time.perf_counter()
+        want, st = oracle.pagerank_f64(h.row_offsets, None, t.row_offsets, t.col_indices, None, 0.85,
+                                       max_iter=50, tol=1e-7)
+        secs = time.perf_counter() - t0
+        del t
+        out["vs_f64"] = {}
+        for eng, v in ranks.items():
+            d = np.abs(v - want)
+            out["vs_f64"][eng] = {"l1": float(d.sum()), "max_abs": float(d.max()),
+                                  "within_1e-5": bool(d.sum() <= 1e-5)}
+        out["cpu_baseline"] = {"ms_per_iteration": round(secs * 1e3 / max(st["iterations"], 1), 2),
+                               "iterations": st["iterations"], "threads": threads, "kind": "port",
+                               "sample": "the oracle's FP64 power iteration (apps/pagerank_blas.cu formula, "
+                                         "L1 stop), whole run on every usable core"}
     return out
 
 
@@ -425,6 +526,7 @@ def main() -> int:
     bfs_ms = float(np.mean([b["elapsed_ms"] for b in bstats]))
     sssp_ms = float(np.mean([s["elapsed_ms"] for s in sstats]))
     log(f"step {ms_per_step:.3f} ms (bfs {bfs_ms:.3f} ms, sssp {sssp_ms:.3f} ms), {value:.3f} GTEPS")
+    gpu_depth, gpu_dist = depth.numpy(), dist.numpy()  # the last timed step's results (checked below)
 
     # live per-kernel timing (HIP events on the run's stream) for the roofline of the dominant kernel
     g.profile(True)
@@ -453,7 +555,7 @@ def main() -> int:
         rb = g.bfs(src, "advance", out=depth, to_host=False)
         extras["bfs_push_only"] = {"ms": round(rb.stats["elapsed_ms"], 3),
                                    "gteps": round(m_reach / rb.stats["elapsed_ms"] / 1e6, 3)}
-        extras["grid213_sssp"] = grid213_sssp(g1)
+        extras["grid213_sssp"] = grid213_sssp(g1, not args.no_cpu)
         log(json.dumps(extras))
 
     if args.copy_probe:
@@ -474,7 +576,7 @@ def main() -> int:
     if args.pagerank_scale:
         del g
         g = None
-        pagerank = pagerank_benchmark(g1, args.pagerank_scale, args.seed)
+        pagerank = pagerank_benchmark(g1, args.pagerank_scale, args.seed, not args.no_cpu)
         log(json.dumps(pagerank))
 
     serving = None
@@ -486,7 +588,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(host, src, args.cpu_budget)
+        cpu = cpu_baseline(host, src, args.cpu_budget, gpu_depth, gpu_dist)
         log(json.dumps(cpu))
 
     line = {
@@ -518,6 +620,7 @@ def main() -> int:
         "m_reach": m_reach,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "bit_exact": None if cpu is None else all(v for v in cpu["bit_exact"].values() if v is not None),
         "extras": extras,
         "batch": batch,
         "serving": serving,
