@@ -183,6 +183,8 @@ _SIGNATURES = {
     "g1_spmv_merge_partition": (_st, [_u32, _u32, _vp, _vp, C.c_size_t, _vp]),
     "g1_spmv_merge_minplus_u32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "g1_spmv_merge_plustimes_f32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_spmv_merge_noalloc_max_workspace_bytes": (C.c_size_t, []),
+    "g1_spmv_merge_minplus_u32_noalloc": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "g1_spmv_csrvec_lanes": (_u32, [_u32, _u32]),
     "g1_spmv_csrvec_minplus_u32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "g1_spmv_csrvec_plustimes_f32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),

@@ -98,6 +98,21 @@ g1_status g1_spmv_merge_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_t
                                       const float* values_dev, const float* x_dev, float* y_dev,
                                       void* workspace_dev, size_t workspace_bytes, g1_stream_t stream);
 
+/* Merge-path min-plus SpMV that allocates nothing and takes no workspace. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Each call partitions the matrix and multiplies with
+ * the library's own per-GPU scratch array of g1_spmv_merge_noalloc_max_workspace_bytes() bytes (8 MiB,
+ * reserved when the library loads and held until the process ends); a matrix whose
+ * g1_spmv_merge_workspace_bytes() exceeds it is rejected with G1_ERR_INVALID_ARG (up to about 4.2e8
+ * n_rows + nnz fit). Results equal g1_spmv_merge_minplus_u32's. Calls on one GPU share the scratch:
+ * calls that overlap in time on one GPU (other streams, other threads) race and must be serialized by
+ * the caller. */
+size_t g1_spmv_merge_noalloc_max_workspace_bytes(void);
+g1_status g1_spmv_merge_minplus_u32_noalloc(uint32_t n_rows, uint32_t n_cols, uint32_t nnz,
+                                            const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                                            const uint32_t* values_dev, const uint32_t* x_dev, uint32_t* y_dev,
+                                            g1_stream_t stream);
+
 /* CSR-vector: a vector of L lanes (L in 1..64, a power of two) per row inside a wave64, L = half of
  * nnz / n_rows rounded up to a power of two unless lanes > 0 forces it. Replaces SpmvDispatcher<SpmvCudaCsrVector>
  * (include/GraphOne/archi/blas/spmv/cuda/cuda.cuh:10-61, csr_vector.cuh:20-68), written from its

@@ -125,3 +125,16 @@ def test_output_buffers_checked_before_device_work():
         g.road_batch([0, 1], kinds=["bfs"])
     with pytest.raises(TypeError):
         g.bfs(0, out=np.zeros(100, np.uint32))
+
+
+def test_noalloc_spmv_rejects_oversize_before_device_work():
+    """g1_spmv_merge_minplus_u32_noalloc refuses a matrix whose workspace exceeds its static scratch with a
+    status code and a message, before touching any pointer or the device."""
+    L = _lib.lib()
+    cap = L.g1_spmv_merge_noalloc_max_workspace_bytes()
+    assert cap == 8 << 20
+    n_rows, nnz = 1 << 26, 400_000_000
+    assert L.g1_spmv_merge_workspace_bytes(n_rows, nnz) > cap
+    st = L.g1_spmv_merge_minplus_u32_noalloc(n_rows, n_rows, nnz, None, None, None, None, None, None)
+    assert st == _lib.G1_ERR_INVALID_ARG
+    assert b"scratch" in L.g1_last_error_message()
