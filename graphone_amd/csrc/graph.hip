@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "g1/push.hpp"
+#include "g1/sweep.hpp"
 #include "g1/spmv_merge.hpp"
 #include "g1/apply.hpp"
 #include "graph.hpp"
@@ -81,6 +82,52 @@ g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s) {
         G1_HIP(hipMemcpy(*list, rows.data(), sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice));
     *count = static_cast<uint32_t>(rows.size());
     return G1_OK;
+}
+
+// The pieces of the out-edge rows longer than kSweepLong edges, for the dense SSSP sweep: counted per row,
+// scanned and written on the device; the count is read back once (a stream synchronization, at the first
+// dense iteration on this handle).
+g1_status ensure_sweep_pieces(g1_graph* g, hipStream_t s) {
+    if (g->sweep_pieces) return G1_OK;
+    const uint32_t n = g->n;
+    uint32_t *cnt = nullptr, *pos = nullptr, *part = nullptr;
+    const size_t nn = n ? n : 1;
+    G1_HIP(hipMalloc(reinterpret_cast<void**>(&cnt), nn * sizeof(uint32_t)));
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&pos), nn * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&part), (nn / 4096 + 2) * sizeof(uint32_t));
+    g1_status st = G1_OK;
+    uint32_t last[2] = {0, 0};
+    if (e != hipSuccess) {
+        st = set_error(G1_ERR_OUT_OF_MEMORY, "ensure_sweep_pieces: %s", hipGetErrorString(e));
+    } else if (n) {
+        const uint32_t blocks = (n + 255) / 256;
+        sweep_piece_count_kernel<<<blocks, 256, 0, s>>>(n, g->row_off, cnt);
+        st = exclusive_scan_u32(cnt, pos, n, part, s);
+        if (st == G1_OK) {
+            e = hipMemcpyAsync(&last[0], pos + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(&last[1], cnt + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) st = set_error(G1_ERR_HIP, "ensure_sweep_pieces: %s", hipGetErrorString(e));
+        }
+        const uint32_t total = last[0] + last[1];
+        if (st == G1_OK) {
+            void* p = nullptr;
+            st = graph_alloc(g, &p, static_cast<size_t>(total ? total : 1) * sizeof(SweepPiece));
+            if (st == G1_OK && total) {
+                sweep_piece_write_kernel<<<blocks, 256, 0, s>>>(n, g->row_off, pos, static_cast<SweepPiece*>(p));
+                e = hipStreamSynchronize(s);
+                if (e != hipSuccess) st = set_error(G1_ERR_HIP, "ensure_sweep_pieces: %s", hipGetErrorString(e));
+            }
+            if (st == G1_OK) {
+                g->sweep_pieces = p;
+                g->n_sweep_pieces = total;
+            }
+        }
+    }
+    (void)hipFree(cnt);
+    (void)hipFree(pos);
+    (void)hipFree(part);
+    return st;
 }
 
 g1_status ensure_pull_work(g1_graph* g, bool pagerank) {

@@ -59,6 +59,8 @@ struct g1_graph {
     uint32_t* fb[2] = {nullptr, nullptr};
     uint32_t* fb3 = nullptr;                    // three rotating frontier bitmaps (device-driven BFS)
     unsigned long long* dstamp = nullptr;       // SSSP (dist << 32 | stamp) words
+    void* sweep_pieces = nullptr;               // SweepPiece records of the long out-edge rows (lazy)
+    uint32_t n_sweep_pieces = 0;
     unsigned long long* hist = nullptr;         // per-iteration history of the push engines
     uint32_t* road_far[2] = {nullptr, nullptr};  // far lists of the road-graph kernel (road_far_cap each)
     uint32_t road_far_cap = 0;
@@ -132,6 +134,7 @@ g1_status road_queries(g1_graph* g, const uint32_t* kinds, const uint32_t* sourc
                        uint8_t* ok, uint32_t* iters, hipStream_t s);
 g1_status ensure_push_work(g1_graph* g, hipStream_t s);  // work areas cleared on stream s
 g1_status ensure_pull_work(g1_graph* g, bool pagerank);
+g1_status ensure_sweep_pieces(g1_graph* g, hipStream_t s);  // pieces of the long rows of the out-edge CSR
 g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s);  // after ensure_pull_work(g, true)
 
 // On-device CSR conversions (convert.hip): the transpose (workspace of transpose_workspace_bytes), an
