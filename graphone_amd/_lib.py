@@ -186,6 +186,21 @@ _SIGNATURES = {
     "g1_spmv_merge_noalloc_max_workspace_bytes": (C.c_size_t, []),
     "g1_spmv_merge_minplus_u32_noalloc": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "g1_spmv_csrvec_lanes": (_u32, [_u32, _u32]),
+    "g1_csr_transpose_workspace_bytes": (C.c_size_t, [_u32, _u32, C.c_int32]),
+    "g1_csr_transpose": (_st, [_u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_step_workspace_bytes": (C.c_size_t, [_u32, _u32]),
+    "g1_frontier_density": (_st, [_u32, _vp, _vp, _vp, _u32, _vp, _vp, C.c_size_t, _vp]),
+    "g1_advance_sssp_u32": (_st, [_u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _vp, C.c_size_t,
+                                  _vp]),
+    "g1_advance_bfs": (_st, [_u32, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_filter_compact": (_st, [_vp, _vp, _u32, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_frontier_queue_to_bitmap": (_st, [_u32, _vp, _vp, _u32, _vp, _vp]),
+    "g1_frontier_bitmap_to_queue": (_st, [_u32, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_apply_minplus_u32": (_st, [_u32, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_pr_apply_f32": (_st, [_u32, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_l1_delta_f64": (_st, [_u32, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_spmm_minplus_u32": (_st, [_u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "g1_spmm_plustimes_f32": (_st, [_u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "g1_spmv_csrvec_minplus_u32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "g1_spmv_csrvec_plustimes_f32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "g1_gen_rmat": (_st, [_u32, _u32, C.c_double, C.c_double, C.c_double, _u64, _i32, C.POINTER(HostCSR)]),

@@ -13,13 +13,18 @@
 // Apply fused with the next Construct where the reference also fuses it (apply_construct_kernel).
 #pragma once
 
+#include <stdio.h>
+#include <stdlib.h>
+
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "GraphOne/frontier.h"
 #include "GraphOne/graph.h"
 #include "GraphOne/spmv.h"
 #include "g1/push.hpp"
+#include "g1/step.hpp"
 
 namespace graph_one {
 
@@ -76,32 +81,19 @@ protected:
 
 // ------------------------------------------------------------------------------------------ kernels
 
-// qoff[i] = out-degree of list[i] (0 for an invalid vertex); one pass, then an exclusive scan in place.
-template <vstart_t vstart, typename vertex_t>
-__global__ void list_degrees_kernel(const vertex_t* __restrict__ list, uint32_t n, const uint32_t* __restrict__ row_off,
-                                    uint32_t* __restrict__ out) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const vertex_t v = list[i];
-        out[i] = utils::is_vertex_valid<vstart>(v) ? row_off[v + 1] - row_off[v] : 0u;
-    }
-}
-
-static __global__ void nonzero_flags_kernel(const uint32_t* __restrict__ deg, uint32_t n, uint32_t* __restrict__ flags) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) flags[i] = deg[i] ? 1u : 0u;
-}
-
+// deg[i] = out-degree of list[i] for i < n (0 for an invalid vertex) and deg[n] = 0, so that an exclusive
+// scan of n + 1 entries leaves the total in deg[n].
 template <typename vertex_t>
-__global__ void compact_nonzero_kernel(const vertex_t* __restrict__ list, const uint32_t* __restrict__ deg,
-                                       const uint32_t* __restrict__ pos, uint32_t n, uint32_t* __restrict__ out,
-                                       uint32_t* __restrict__ out_deg) {
+__global__ void list_degrees_kernel(const vertex_t* __restrict__ list, uint32_t n, uint32_t n_vertices,
+                                    const uint32_t* __restrict__ row_off, uint32_t* __restrict__ out) {
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        if (deg[i]) {
-            out[pos[i]] = static_cast<uint32_t>(list[i]);
-            out_deg[pos[i]] = deg[i];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += stride) {
+        uint32_t d = 0;
+        if (i < n) {
+            const uint32_t v = static_cast<uint32_t>(list[i]);
+            if (v < n_vertices) d = row_off[v + 1] - row_off[v];
         }
+        out[i] = d;
     }
 }
 
@@ -154,77 +146,162 @@ static __global__ void scan_apply_kernel(uint32_t* __restrict__ data, uint32_t n
     }
 }
 
-// Exclusive scan of data[0..n) in place; returns the total. scratch holds >= n/kScanBlock + 2 entries.
-inline uint32_t exclusive_scan_u32(uint32_t* data, uint32_t n, uint32_t* scratch) {
-    if (n == 0) return 0;
+// Exclusive scan of data[0..n) in place, on the device only; the total is left in scratch[blocks].
+// scratch holds >= n/kScanBlock + 2 entries.
+inline void exclusive_scan_u32_dev(uint32_t* data, uint32_t n, uint32_t* scratch) {
+    if (n == 0) return;
     const uint32_t blocks = (n + kScanBlock - 1) / kScanBlock;
     scan_block_sums_kernel<<<blocks, 256>>>(data, n, scratch);
     scan_sums_kernel<<<1, 256>>>(scratch, blocks);
     scan_apply_kernel<<<blocks, 256>>>(data, n, scratch);
     archi::launch_check("exclusive scan");
+}
+
+// The same, returning the total to the host (one device-to-host read).
+inline uint32_t exclusive_scan_u32(uint32_t* data, uint32_t n, uint32_t* scratch) {
+    if (n == 0) return 0;
+    exclusive_scan_u32_dev(data, n, scratch);
     uint32_t total = 0;
-    archi::memcpy<arch_t::cpu, arch_t::hip, uint32_t>(&total, scratch + blocks, 1);
+    archi::memcpy<arch_t::cpu, arch_t::hip, uint32_t>(&total, scratch + (n + kScanBlock - 1) / kScanBlock, 1);
     return total;
 }
 
-// Gathers the kSegments queue segments into one contiguous list with global edge offsets.
-template <typename vertex_t>
-__global__ void gather_segments_kernel(const uint32_t* __restrict__ q, const uint32_t* __restrict__ qoff,
-                                       const unsigned long long* __restrict__ cnt, uint32_t cap,
-                                       vertex_t* __restrict__ out, uint32_t* __restrict__ out_off) {
-    __shared__ uint32_t vb[g1::kSegments + 1], eb[g1::kSegments + 1];
-    if (threadIdx.x < 64) g1::load_segment_bases(cnt, vb, eb);
-    __syncthreads();
-    const uint32_t nf = vb[g1::kSegments];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += stride) {
-        const uint32_t s = g1::search_last_le(vb, g1::kSegments, i);
-        const size_t at = static_cast<size_t>(s) * cap + (i - vb[s]);
-        out[i] = static_cast<vertex_t>(q[at]);
-        out_off[i] = eb[s] + qoff[at];
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out_off[nf] = eb[g1::kSegments];
+// Clears the advance engine's packed output counter ((edges << 32) | count) and overflow flag.
+static __global__ void clear_counters_kernel(unsigned long long* packed, uint32_t* overflow) {
+    *packed = 0ull;
+    *overflow = 0u;
 }
 
-// Advance policy around a user AdvanceFunctor: the source value is the source id itself; the functor's
-// advance() decides the edge and its filter() (when present) drops the vertex.
+// Appends each lane's kept vertices (bit u of `keep`: slot u's vertex v[u], out-degree dg[u]) to the output
+// list with one reservation per wave on the packed counter (count in the low word, out-edges in the high
+// word); entries past `capacity` are dropped and flagged.
+template <int N>
+__device__ __forceinline__ void append_kept(uint32_t keep, const uint32_t (&v)[N], const uint32_t (&dg)[N],
+                                            uint32_t* out, uint32_t capacity, unsigned long long* packed,
+                                            uint32_t* overflow) {
+    unsigned long long mine = 0ull;
+#pragma unroll
+    for (int u = 0; u < N; ++u)
+        if (keep & (1u << u)) mine += (static_cast<unsigned long long>(dg[u]) << 32) | 1ull;
+    const unsigned long long incl = g1::wave_inclusive_sum(mine);
+    const unsigned long long total = __shfl(incl, g1::kWave - 1);
+    if (total == 0ull) return;
+    unsigned long long base = 0ull;
+    if (g1::lane_id() == 0) base = atomicAdd(packed, total);
+    base = __shfl(base, 0);
+    uint32_t at = static_cast<uint32_t>(base + incl - mine);
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+        if (keep & (1u << u)) {
+            if (at < capacity) out[at] = v[u];
+            else atomicExch(overflow, 1u);
+            ++at;
+        }
+    }
+}
+
+// Push relaxation of a user AdvanceFunctor for g1::step_advance_kernel: the source value is the source id;
+// the functor's advance() decides each edge and its filter() (when present) drops the kept vertex. Kept
+// vertices are appended with their out-degrees (the next iteration's edge count).
 template <typename F, typename GRef, typename D, bool kHasFilter>
-struct FunctorAdvance {
-    using src_t = uint32_t;
-    static constexpr bool kWeighted = false;
-    static constexpr bool kQueueLeaves = true;  // the output frontier lists every kept vertex
-    static constexpr bool kEnqueue = true;
+struct FunctorPush {
+    using vertex_t = typename GRef::vertex_type;
+    using edge_t = typename GRef::edge_type;
     GRef g;
     D d;
-    __device__ __forceinline__ uint32_t source_value(uint32_t u) const { return u; }
-    template <int N>
-    __device__ __forceinline__ void relax_many(const uint32_t (&)[N], const uint32_t (&u)[N], const uint32_t (&v)[N],
-                                               const uint32_t (&)[N], const uint32_t (&e)[N], bool (&won)[N]) const {
+    const uint32_t* row_off;
+    const uint32_t* col;
+    uint32_t* out;
+    uint32_t capacity;
+    unsigned long long* packed;
+    uint32_t* overflow;
+    __device__ __forceinline__ uint32_t source(uint32_t u) const { return u; }
+    __device__ __forceinline__ void operator()(const uint32_t (&e)[g1::kSweepUnroll], const uint32_t (&src)[g1::kSweepUnroll],
+                                               const bool (&ok)[g1::kSweepUnroll]) const {
         D dl = d;
+        uint32_t v[g1::kSweepUnroll], dg[g1::kSweepUnroll];
+        uint32_t keep = 0;
 #pragma unroll
-        for (int r = 0; r < N; ++r) {
-            won[r] = false;
-            if (v[r] == g1::kInf) continue;
-            using vertex_t = typename GRef::vertex_type;
-            using edge_t = typename GRef::edge_type;
-            bool keep = F::advance(static_cast<vertex_t>(u[r]), static_cast<vertex_t>(v[r]), static_cast<edge_t>(e[r]),
-                                   g.get_out_edge_weight(static_cast<edge_t>(e[r])), dl);
+        for (int u = 0; u < g1::kSweepUnroll; ++u) v[u] = ok[u] ? col[e[u]] : 0u;
+#pragma unroll
+        for (int u = 0; u < g1::kSweepUnroll; ++u) {
+            dg[u] = 0;
+            if (!ok[u]) continue;
+            bool k = F::advance(static_cast<vertex_t>(src[u]), static_cast<vertex_t>(v[u]), static_cast<edge_t>(e[u]),
+                                g.get_out_edge_weight(static_cast<edge_t>(e[u])), dl);
             if constexpr (kHasFilter) {
-                if (keep) keep = !F::filter(static_cast<vertex_t>(v[r]), dl);
+                if (k) k = !F::filter(static_cast<vertex_t>(v[u]), dl);
             }
-            won[r] = keep;
+            if (k) {
+                keep |= 1u << u;
+                dg[u] = row_off[v[u] + 1] - row_off[v[u]];
+            }
         }
+        append_kept(keep, v, dg, out, capacity, packed, overflow);
     }
 };
 
-template <typename P>
-__global__ void __launch_bounds__(g1::kPushThreads)
-functor_advance_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col, g1::PushQueues pq,
-                       P pol) {
-    __shared__ uint32_t s_vb[g1::kSegments + 1];
-    __shared__ uint32_t s_eb[g1::kSegments + 1];
-    if (!g1::push_prologue(pq, s_vb, s_eb)) return;
-    g1::advance_tiles(row_off, col, nullptr, pq, pol, s_vb, s_eb);
+// Pull step of the adaptive frontier (DblBufFrontier): groups of 64 vertices per wave; every in-edge
+// (u, v) of the group whose source is in the input frontier (bitmap fb) calls the functor's advance(u, v,
+// in-edge index, weight) -- the relaxations the push step would make, grouped by target -- and v joins the
+// output once if any of them kept it and the filter (when present) does not drop it. The in-edges are
+// dealt out 64 per step (g1::wave_expand); a 64-flag LDS array per wave collects the keeps.
+template <typename F, typename GRef, typename D, bool kHasFilter, typename weight_t>
+__global__ void __launch_bounds__(256)
+functor_pull_kernel(uint32_t n, const uint32_t* __restrict__ t_row_off, const uint32_t* __restrict__ t_col,
+                    const weight_t* __restrict__ t_w, const uint32_t* __restrict__ fb, GRef g, D d,
+                    const uint32_t* __restrict__ row_off, uint32_t* out, uint32_t capacity,
+                    unsigned long long* packed, uint32_t* overflow) {
+    using vertex_t = typename GRef::vertex_type;
+    using edge_t = typename GRef::edge_type;
+    __shared__ uint32_t s_mark[256];
+    __shared__ uint32_t s_keep[256];
+    const uint32_t lane = g1::lane_id();
+    const uint32_t wave = threadIdx.x / g1::kWave;
+    uint32_t* keepf = s_keep + wave * g1::kWave;
+    const uint32_t groups = (n + g1::kWave - 1) / g1::kWave;
+    const uint32_t waves = gridDim.x * (blockDim.x / g1::kWave);
+    D dl = d;
+    for (uint32_t grp = blockIdx.x * (blockDim.x / g1::kWave) + wave; grp < groups; grp += waves) {  // wave-uniform
+        const uint32_t v0 = grp * g1::kWave;
+        const uint32_t v = v0 + lane;
+        uint32_t rs = 0, deg = 0;
+        if (v < n) {
+            rs = t_row_off[v];
+            deg = t_row_off[v + 1] - rs;
+        }
+        keepf[lane] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        auto relax = [&](const uint32_t (&e)[g1::kSweepUnroll], const uint32_t (&tv)[g1::kSweepUnroll],
+                         const bool (&ok)[g1::kSweepUnroll]) {
+            uint32_t u[g1::kSweepUnroll];
+            bool in[g1::kSweepUnroll];
+#pragma unroll
+            for (int k = 0; k < g1::kSweepUnroll; ++k) u[k] = ok[k] ? t_col[e[k]] : 0u;
+#pragma unroll
+            for (int k = 0; k < g1::kSweepUnroll; ++k) in[k] = ok[k] && ((fb[u[k] >> 5] >> (u[k] & 31u)) & 1u);
+#pragma unroll
+            for (int k = 0; k < g1::kSweepUnroll; ++k) {
+                if (!in[k]) continue;
+                weight_t w{};
+                if constexpr (!std::is_same_v<weight_t, empty_t>) w = t_w ? t_w[e[k]] : weight_t(1);
+                if (F::advance(static_cast<vertex_t>(u[k]), static_cast<vertex_t>(tv[k]), static_cast<edge_t>(e[k]), w, dl))
+                    keepf[tv[k] - v0] = 1u;
+            }
+        };
+        g1::wave_expand(rs, deg, v, s_mark + wave * g1::kWave, relax);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        bool k = v < n && keepf[lane] != 0u;
+        if constexpr (kHasFilter) {
+            if (k) k = !F::filter(static_cast<vertex_t>(v), dl);
+        }
+        uint32_t vv[1] = {v}, dg[1] = {k ? row_off[v + 1] - row_off[v] : 0u};
+        append_kept<1>(k ? 1u : 0u, vv, dg, out, capacity, packed, overflow);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
 }
 
 // x[v] = construct_each(v) for the listed vertices (the sparse Construct's transform + scatter).
@@ -292,6 +369,13 @@ __global__ void apply_construct_kernel(const gather_t* __restrict__ y, uint32_t 
 
 // ------------------------------------------------------------------------------------------ AdvanceGC
 
+// AdvanceGC::Forward (advance_gc.h:180-244): ResizeOutput, Advance and Filter as one device-driven step.
+// The input list's degrees are scanned on the device into edge offsets; its edge count E comes from the
+// previous step's read-back (or, for a list the engine did not write, one read after the scan) and sizes
+// the output; the functor runs inside g1::step_advance_kernel (edge windows of 4096 slots, each dealt 64
+// per step over the entries that overlap it) and appends each kept vertex with its out-degree. The one
+// host read per iteration is the output's (count, out-edges) pair, which Run's convergence test needs.
+// On a DblBufFrontier an iteration whose E reaches pull_fraction() x m runs as a pull over the in-edges.
 template <typename functor_t, typename comp_t, typename frontier_t>
 class AdvanceGC : public GcEngine<comp_t, frontier_t> {
 public:
@@ -300,7 +384,9 @@ public:
     using dstatus_t = typename base_t::dstatus_type;
     using vertex_t = typename graph_t::vertex_type;
     using edge_t = typename graph_t::edge_type;
+    using weight_t = typename graph_t::weight_type;
     using index_t = typename frontier_t::index_type;
+    using gref_t = decltype(std::declval<const graph_t&>().ToArch());
     constexpr static arch_t arch = graph_t::arch_value;
     static_assert(frontier_t::kind == SPARSE_BASED, "AdvanceGC runs on a sparse frontier");
     static_assert(arch == arch_t::hip, "AdvanceGC runs on the device");
@@ -314,93 +400,139 @@ public:
         constexpr static bool value = decltype(test<functor_t>(0))::value;
     };
 
-    AdvanceGC(comp_t& comp, frontier_t& frontier)
-        : base_t(comp, frontier), scratch_(4096), cnt_(3 * g1::kSegments), stats_(2 * g1::kMaxPushBlocks), overflow_(4) {}
+    AdvanceGC(comp_t& comp, frontier_t& frontier) : base_t(comp, frontier), scratch_(64), ctr_(2) {}
+
+    ~AdvanceGC() override {
+        if constexpr (frontier_t::adaptive) {
+            if (getenv("G1_DBL_TRACE"))
+                fprintf(stderr, "[graph_one] DblBufFrontier: %u push steps, %u pull steps\n", pushes_, pulls_);
+        }
+    }
 
     void Forward() override {
         auto& fr = this->frontier_;
         const auto gref = this->graph_.ToArch();
         const uint32_t nin = fr.input_size();
-        // Prepare: the input entries that own edges, in order, with their edge offsets.
-        uint32_t k = 0, E = 0;
-        if (nin > 0) {
-            grow(deg_, nin + 1);
-            grow(pos_, nin + 1);
-            grow(list_, nin + 1);
-            grow(off_, nin + 1);
-            ensure_scratch(nin);
-            list_degrees_kernel<graph_t::vstart_value, vertex_t><<<archi::grid_blocks(nin), 256>>>(
-                fr.input().data(), nin, reinterpret_cast<const uint32_t*>(gref.row_offsets), deg_.data());
-            nonzero_flags_kernel<<<archi::grid_blocks(nin), 256>>>(deg_.data(), nin, pos_.data());
-            archi::launch_check("prepare");
-            k = exclusive_scan_u32(pos_.data(), nin, scratch_.data());
-            compact_nonzero_kernel<vertex_t><<<archi::grid_blocks(nin), 256>>>(fr.input().data(), deg_.data(), pos_.data(),
-                                                                               nin, list_.data(), off_.data());
-            archi::launch_check("prepare compact");
-            E = exclusive_scan_u32(off_.data(), k, scratch_.data());
+        const uint32_t nv = static_cast<uint32_t>(this->graph_.num_vertices());
+        const uint32_t m = static_cast<uint32_t>(this->graph_.num_edges());
+        const uint32_t* row_off = reinterpret_cast<const uint32_t*>(gref.row_offsets);
+        if (nin == 0) {
+            fr.reset_output(0);
+            fr.set_output_edges(0);
+            return;
         }
-        // Every edge emits at most one vertex: size the output segments for E edges.
-        const uint64_t cap = g1::push_segment_capacity(~0ull, E > 0 ? E : 1);
-        grow(segq_, static_cast<uint32_t>(cap * g1::kSegments));
-        grow(segoff_, static_cast<uint32_t>(cap * g1::kSegments));
-        // The prepared list is segment 0 of the input: counter (E << 32) | k, every other segment empty.
-        std::vector<unsigned long long> c(3 * g1::kSegments, 0ull);
-        c[0] = (static_cast<unsigned long long>(E) << 32) | k;
-        archi::memcpy<arch_t::hip, arch_t::cpu, unsigned long long>(cnt_.data(), c.data(), c.size());
-        if (E > 0) {
-            g1::PushQueues pq;
-            pq.q_in = list_.data();
-            pq.qoff_in = off_.data();
-            pq.cnt_in = cnt_.data();
-            pq.q_out = segq_.data();
-            pq.qoff_out = segoff_.data();
-            pq.cnt_out = cnt_.data() + g1::kSegments;
-            pq.cnt_clear = cnt_.data() + 2 * g1::kSegments;
-            pq.stats = stats_.data();
-            pq.overflow = overflow_.data();
-            pq.cap = static_cast<uint32_t>(cap);  // only input segment 0 is used, so the stride is the output's
-            const uint32_t blocks = std::min<uint32_t>(g1::kMaxPushBlocks, (E + g1::kPushTile - 1) / g1::kPushTile);
-            FunctorAdvance<functor_t, decltype(gref), dstatus_t, HasFilter::value> pol{gref, this->d_status_};
-            functor_advance_kernel<decltype(pol)><<<blocks, g1::kPushThreads>>>(
-                reinterpret_cast<const uint32_t*>(gref.row_offsets), reinterpret_cast<const uint32_t*>(gref.col_indices),
-                pq, pol);
+        // edge offsets of the input list: off_[0..nin], off_[nin] = E
+        grow(off_, nin + 1);
+        grow(scratch_, (nin + 1) / kScanBlock + 4);
+        list_degrees_kernel<vertex_t><<<archi::grid_blocks(nin + 1), 256>>>(fr.input().data(), nin, nv, row_off,
+                                                                          off_.data());
+        archi::launch_check("advance degrees");
+        exclusive_scan_u32_dev(off_.data(), nin + 1, scratch_.data());
+        uint32_t E;
+        if (fr.input_edges_valid()) {
+            E = fr.input_edges();
+        } else {
+            archi::memcpy<arch_t::cpu, arch_t::hip, uint32_t>(&E, off_.data() + nin, 1);
+        }
+        bool pull = false;
+        if constexpr (frontier_t::adaptive) {
+            const double f = frontier_t::pull_fraction();
+            pull = f > 0.0 && m > 0 && static_cast<double>(E) >= f * static_cast<double>(m);
+        }
+        // each kept edge (push) or vertex (pull) appends one entry
+        const uint32_t cap = std::max<uint32_t>(pull ? nv : E, 1u);
+        fr.reserve_output_capacity(cap);
+        clear_counters_kernel<<<1, 1>>>(packed(), overflow());
+        if (pull) {
+            run_pull(gref, nin, nv, m, fr.output().data(), cap);
+            ++pulls_;
+        } else if (E > 0) {
+            FunctorPush<functor_t, gref_t, dstatus_t, HasFilter::value> rel{
+                gref, this->d_status_, row_off, reinterpret_cast<const uint32_t*>(gref.col_indices),
+                reinterpret_cast<uint32_t*>(fr.output().data()), cap, packed(), overflow()};
+            const uint32_t windows = (E + g1::kStepWindow - 1) / g1::kStepWindow;
+            const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>(2048u, (windows + 3) / 4));
+            g1::step_advance_kernel<decltype(rel)><<<blocks, 256>>>(row_off, reinterpret_cast<const uint32_t*>(fr.input().data()),
+                                                                    nullptr, off_.data(), nin, rel);
             archi::launch_check("advance");
+            ++pushes_;
         }
-        // Gather the output segments into the frontier's output list.
-        std::vector<unsigned long long> oc(g1::kSegments);
-        archi::memcpy<arch_t::cpu, arch_t::hip, unsigned long long>(oc.data(), cnt_.data() + g1::kSegments, g1::kSegments);
-        uint64_t nout = 0;
-        for (auto v : oc) nout += static_cast<uint32_t>(v);
-        uint32_t of = 0;
-        archi::memcpy<arch_t::cpu, arch_t::hip, uint32_t>(&of, overflow_.data(), 1);
-        if (of) LOG_ERROR("advance: frontier segment overflow");
-        fr.reset_output(static_cast<index_t>(nout));
-        if (nout > 0) {
-            grow(off_, static_cast<uint32_t>(nout) + 1);
-            gather_segments_kernel<vertex_t><<<archi::grid_blocks(nout), 256>>>(
-                segq_.data(), segoff_.data(), cnt_.data() + g1::kSegments, static_cast<uint32_t>(cap),
-                fr.output().data(), off_.data());
-            archi::launch_check("gather segments");
-        }
-        archi::LaunchSync<arch>();
+        // the one read of the iteration: (count, out-edges) of the output and the overflow flag
+        unsigned long long h[2] = {0ull, 0ull};
+        archi::memcpy<arch_t::cpu, arch_t::hip, unsigned long long>(h, ctr_.data(), 2);
+        const unsigned long long pk = h[0];
+        if (static_cast<uint32_t>(h[1])) LOG_ERROR("advance: output list overflow");
+        fr.reset_output(static_cast<index_t>(static_cast<uint32_t>(pk)));
+        fr.set_output_edges(static_cast<uint32_t>(pk >> 32));
     }
 
 private:
-    static void grow(Buffer<arch_t::hip, uint32_t, uint32_t>& b, uint32_t n) {
+    // The pull step: the input as a bitmap, then functor_pull_kernel over the in-edge CSR (the graph's
+    // transpose view, or one built here on the device at the first pull).
+    void run_pull(const gref_t& gref, uint32_t nin, uint32_t nv, uint32_t m, vertex_t* out, uint32_t cap) {
+        auto& fr = this->frontier_;
+        const uint32_t words = (nv + 31) / 32;
+        grow(fb_, words + 1);
+        archi::fill<arch_t::hip>(fb_.begin(), fb_.end(), 0u);
+        g1::queue_to_bitmap_kernel<<<archi::grid_blocks(nin), 256>>>(reinterpret_cast<const uint32_t*>(fr.input().data()),
+                                                                    nullptr, nin, nv, fb_.data());
+        archi::launch_check("pull bitmap");
+        const uint32_t* t_ro;
+        const uint32_t* t_ci;
+        const weight_t* t_w;
+        if constexpr (graph_t::both_mat) {
+            t_ro = reinterpret_cast<const uint32_t*>(gref.trans_row_offsets);
+            t_ci = reinterpret_cast<const uint32_t*>(gref.trans_col_indices);
+            t_w = gref.trans_values;
+        } else {
+            ensure_transpose(gref, nv, m);
+            t_ro = t_ro_.data();
+            t_ci = t_ci_.data();
+            t_w = has_w_ ? reinterpret_cast<const weight_t*>(t_w_.data()) : nullptr;
+        }
+        const uint32_t groups = (nv + 63) / 64;
+        const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>(2048u, (groups + 3) / 4));
+        functor_pull_kernel<functor_t, gref_t, dstatus_t, HasFilter::value, weight_t><<<blocks, 256>>>(
+            nv, t_ro, t_ci, t_w, fb_.data(), gref, this->d_status_, reinterpret_cast<const uint32_t*>(gref.row_offsets),
+            reinterpret_cast<uint32_t*>(out), cap, packed(), overflow());
+        archi::launch_check("pull");
+    }
+
+    // The in-edge CSR of a graph kept in its out-edge view only, through the library's device transpose
+    // (4-byte weights travel as their bit patterns).
+    void ensure_transpose(const gref_t& gref, uint32_t nv, uint32_t m) {
+        if (t_ro_.size()) return;
+        static_assert(sizeof(weight_t) == 4 || std::is_same_v<weight_t, empty_t>, "4-byte weights");
+        has_w_ = !std::is_same_v<weight_t, empty_t> && gref.values != nullptr;
+        t_ro_ = Buffer<arch_t::hip, uint32_t, uint32_t>(nv + 1);
+        t_ci_ = Buffer<arch_t::hip, uint32_t, uint32_t>(std::max<uint32_t>(m, 1));
+        if (has_w_) t_w_ = Buffer<arch_t::hip, uint32_t, uint32_t>(std::max<uint32_t>(m, 1));
+        const size_t bytes = g1_csr_transpose_workspace_bytes(nv, m, has_w_ ? 1 : 0);
+        Buffer<arch_t::hip, uint8_t, size_t> ws(std::max<size_t>(bytes, 4));
+        archi::check(g1_csr_transpose(nv, m, reinterpret_cast<const uint32_t*>(gref.row_offsets),
+                                             reinterpret_cast<const uint32_t*>(gref.col_indices),
+                                             has_w_ ? reinterpret_cast<const uint32_t*>(gref.values) : nullptr,
+                                             t_ro_.data(), t_ci_.data(), has_w_ ? t_w_.data() : nullptr, ws.data(),
+                                             ws.size(), nullptr),
+                            "transpose");
+        archi::LaunchSync<arch_t::hip>();
+    }
+
+    template <typename T>
+    static void grow(Buffer<arch_t::hip, T, uint32_t>& b, uint32_t n) {
         if (b.size() < n) b.resize(n);
     }
-    void ensure_scratch(uint32_t n) { grow(scratch_, n / kScanBlock + 4); }
 
-    Buffer<arch_t::hip, uint32_t, uint32_t> deg_;
-    Buffer<arch_t::hip, uint32_t, uint32_t> pos_;
-    Buffer<arch_t::hip, uint32_t, uint32_t> list_;
     Buffer<arch_t::hip, uint32_t, uint32_t> off_;
     Buffer<arch_t::hip, uint32_t, uint32_t> scratch_;
-    Buffer<arch_t::hip, unsigned long long, uint32_t> cnt_;
-    Buffer<arch_t::hip, unsigned long long, uint32_t> stats_;
-    Buffer<arch_t::hip, uint32_t, uint32_t> overflow_;
-    Buffer<arch_t::hip, uint32_t, uint32_t> segq_;
-    Buffer<arch_t::hip, uint32_t, uint32_t> segoff_;
+    // [0]: the output's packed counter (edges << 32 | count); [1]: the overflow flag (low word)
+    Buffer<arch_t::hip, unsigned long long, uint32_t> ctr_;
+    unsigned long long* packed() { return ctr_.data(); }
+    uint32_t* overflow() { return reinterpret_cast<uint32_t*>(ctr_.data() + 1); }
+    Buffer<arch_t::hip, uint32_t, uint32_t> fb_;
+    Buffer<arch_t::hip, uint32_t, uint32_t> t_ro_, t_ci_, t_w_;
+    bool has_w_ = false;
+    uint32_t pushes_ = 0, pulls_ = 0;
 };
 
 // ------------------------------------------------------------------------------------------ BLAS engines
@@ -472,11 +604,17 @@ public:
         : base_t(comp, frontier), flags_(comp.graph.num_vertices() + 1), pos_(comp.graph.num_vertices() + 1),
           scratch_(comp.graph.num_vertices() / kScanBlock + 4) {}
 
+    // One host read per iteration: the output's size (the scan total), which Run's convergence test needs;
+    // every other step stays on the stream.
     void Forward() override {
         auto& fr = this->frontier_;
         const uint32_t n = this->ipt_vec_.size();
         // Construct: x = default_info; x[F] = construct_each(F)
-        archi::fill<base_t::arch>(this->ipt_vec_.begin(), this->ipt_vec_.end(), functor_t::default_info());
+        if (n) {
+            archi::fill_value_kernel<info_t><<<archi::grid_blocks(n), 256>>>(this->ipt_vec_.data(),
+                                                                         functor_t::default_info(), n);
+            archi::launch_check("construct fill");
+        }
         const uint32_t nin = fr.input_size();
         if (nin) {
             construct_list_kernel<functor_t><<<archi::grid_blocks(nin), 256>>>(fr.input().data(), nin, this->d_status_,
@@ -489,7 +627,7 @@ public:
         apply_flags_kernel<functor_t><<<archi::grid_blocks(n), 256>>>(this->opt_vec_.data(), n, this->d_status_,
                                                                       flags_.data());
         archi::launch_check("apply");
-        archi::memcpy<arch_t::hip, arch_t::hip, uint32_t>(pos_.data(), flags_.data(), n);
+        archi::check(g1_memcpy_d2d(pos_.data(), flags_.data(), sizeof(uint32_t) * n, nullptr), "flags copy");
         const uint32_t nout = exclusive_scan_u32(pos_.data(), n, scratch_.data());
         fr.reset_output(nout);
         if (nout) {
@@ -497,7 +635,6 @@ public:
                                                                            fr.output().data());
             archi::launch_check("compact");
         }
-        archi::LaunchSync<base_t::arch>();
     }
 
 private:

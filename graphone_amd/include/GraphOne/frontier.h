@@ -1,13 +1,17 @@
 // Frontiers of the graph_one C++ API.
 // Reference: include/GraphOne/frontier/base.h:7-35 (FrontierKind, BaseFrontier), sparse_dbl.h:15-181
 // (SpDblFrontier), dense_dbl.h:14-67 (DenDblFrontier), all_active.h:12-49 (AllActiveFrontier); the
-// README's DblBufFrontier (README.md:102) is the sparse double buffer here as well.
+// README's DblBufFrontier (README.md:102) is the adaptive double buffer: a sparse frontier whose advance
+// steps switch to a pull over the in-edges when the frontier's out-edges pass a share of the graph's.
 //
 // SpDblFrontier keeps the reference's interface (double-buffered vertex lists, input/output sizes on the
-// host, swap after every engine step, convergence when the input is empty). Each list also carries the
-// exclusive prefix sum of its vertices' out-degrees (the merge-path row ends the advance engine
-// partitions edges with); `offsets_valid` says whether that companion array matches the list.
+// host, swap after every engine step, convergence when the input is empty). Each list also carries, when
+// the engine that wrote it knows it, the sum of its vertices' out-degrees (`edges`, valid flag beside it):
+// the advance engine sizes its output with it, so an iteration reads back only the output's size and
+// edge count, together.
 #pragma once
+
+#include <stdlib.h>
 
 #include <initializer_list>
 #include <string>
@@ -38,6 +42,7 @@ class SpDblFrontier : public BaseFrontier<SPARSE_BASED, true> {
 public:
     using vertex_type = vertex_t;
     using index_type = index_t;
+    constexpr static bool adaptive = false;  // advance steps always push
 
     // Device view (reference: sparse_dbl.h:28-64): get(i) reads the input list, set(i, v) writes the output.
     struct arch_ref_t {
@@ -69,12 +74,12 @@ public:
     void reset_input(index_t size) {
         grow(sel_, size, true);
         in_size_ = size;
-        offsets_valid_[sel_] = false;
+        edges_valid_[sel_] = false;
     }
     void reset_output(index_t size) {
         grow(sel_ ^ 1, size, true);
         out_size_ = size;
-        offsets_valid_[sel_ ^ 1] = false;
+        edges_valid_[sel_ ^ 1] = false;
     }
     index_t input_size() const { return in_size_; }
     index_t output_size() const { return out_size_; }
@@ -86,13 +91,16 @@ public:
     Buffer<arch, vertex_t, index_t>& input_mut() { return lists_[sel_]; }
     Buffer<arch, vertex_t, index_t>& output() { return lists_[sel_ ^ 1]; }
 
-    // Edge-offset companions (used by the advance engine).
-    Buffer<arch, uint32_t, index_t>& input_offsets() { return offsets_[sel_]; }
-    Buffer<arch, uint32_t, index_t>& output_offsets() { return offsets_[sel_ ^ 1]; }
-    bool& input_offsets_valid() { return offsets_valid_[sel_]; }
-    bool& output_offsets_valid() { return offsets_valid_[sel_ ^ 1]; }
-    uint32_t& input_edges() { return edges_[sel_]; }
-    uint32_t& output_edges() { return edges_[sel_ ^ 1]; }
+    // Out-edge sums of the lists (the advance engine's output sizing).
+    bool input_edges_valid() const { return edges_valid_[sel_]; }
+    uint32_t input_edges() const { return edges_[sel_]; }
+    void set_output_edges(uint32_t e) {
+        edges_[sel_ ^ 1] = e;
+        edges_valid_[sel_ ^ 1] = true;
+    }
+    // A list of at least `size` entries that outlives the swap (the output, grown without changing its size).
+    void reserve_output_capacity(index_t size) { grow(sel_ ^ 1, size, true); }
+    index_t output_capacity() const { return lists_[sel_ ^ 1].size(); }
 
     arch_ref_t ToArch() {
         arch_ref_t r;
@@ -123,27 +131,36 @@ protected:
         if (size <= lists_[b].size()) return;
         const index_t cap = doubling ? size * 2 : size;
         Buffer<arch, vertex_t, index_t> nl(cap);
-        Buffer<arch, uint32_t, index_t> no(cap + 1);
         if (lists_[b].size()) nl.copy_from(lists_[b]);
-        if (offsets_[b].size()) no.copy_from(offsets_[b]);
         lists_[b] = std::move(nl);
-        offsets_[b] = std::move(no);
     }
 
     int sel_;
     index_t in_size_;
     index_t out_size_;
     Buffer<arch, vertex_t, index_t> lists_[2];
-    Buffer<arch, uint32_t, index_t> offsets_[2];
-    bool offsets_valid_[2] = {false, false};
+    bool edges_valid_[2] = {false, false};
     uint32_t edges_[2] = {0, 0};
 };
 
-// The frontier name of the reference README: a double-buffered sparse frontier.
+// The frontier name of the reference README (README.md:102): the sparse double buffer, adaptive. The
+// advance engine runs an iteration as a pull over the in-edges (every vertex with an in-edge from the
+// frontier calls the functor's advance for those edges and joins the output once) when the frontier's
+// out-edges reach pull_fraction() of the graph's edges, and as a push otherwise. The in-edge CSR is the
+// graph's transpose view when it has one, otherwise built on the device at the first pull.
 template <arch_t arch, typename vertex_t, typename index_t = vertex_t>
 class DblBufFrontier : public SpDblFrontier<arch, vertex_t, index_t> {
 public:
     using SpDblFrontier<arch, vertex_t, index_t>::SpDblFrontier;
+    constexpr static bool adaptive = true;
+    // G1_DBL_PULL (a fraction in (0, 1]; 0 disables pulling), default 0.25.
+    static double pull_fraction() {
+        static const double f = [] {
+            const char* e = getenv("G1_DBL_PULL");
+            return e ? atof(e) : 0.25;
+        }();
+        return f;
+    }
 };
 
 template <arch_t arch, typename vertex_t, typename index_t = vertex_t>

@@ -128,6 +128,97 @@ g1_status g1_spmv_csrvec_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_
                                        uint32_t lanes, g1_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------------
+ * Step-level primitives, for a caller that keeps the reference's engine loop and its own buffers and
+ * routes each step through the library: AdvanceGC::Forward (include/GraphOne/engine/advance_gc.h:180-244)
+ * and the BlasGcEngine Construct / Gather / Apply steps (include/GraphOne/engine/blas_gc.h:21-76,
+ * 171-435). A queue frontier is an array of vertex ids with a capacity the caller allocated and a length
+ * in device memory (*count_dev), so steps chain on a stream with no host read; G1_INF entries are
+ * skipped (the reference's INVALID slots). A bitmap frontier holds bit v of word v / 32 (ceil(n / 32)
+ * words). Every step that needs scratch takes a caller-owned workspace of at least
+ * g1_step_workspace_bytes(n, capacity) bytes (n: vertices; capacity: the queue capacity the step reads),
+ * reusable across steps on one stream.
+ * ------------------------------------------------------------------------------------------- */
+size_t g1_step_workspace_bytes(uint32_t n, uint32_t capacity);
+
+/* *edges_dev = sum of the out-degrees of the valid entries of frontier[0..*count_dev): the E of
+ * AdvanceGC::ResizeOuput (include/GraphOne/engine/advance_gc.h:187-210), the quantity a caller compares
+ * with m to choose push or pull. */
+g1_status g1_frontier_density(uint32_t n, const uint32_t* row_offsets_dev, const uint32_t* frontier_dev,
+                              const uint32_t* count_dev, uint32_t capacity, uint32_t* edges_dev,
+                              void* workspace_dev, size_t workspace_bytes, g1_stream_t stream);
+
+/* One advance + filter of the SSSP functor (include/GraphOne/engine/advance_gc.h:180-244 with
+ * apps/sssp_adv.cu:49-71): for every valid u of frontier_in[0..*in_count_dev) and every out-edge
+ * (u, v, w): nd = dist[u] + w, old = atomicMin(&dist[v], nd); v joins frontier_out when nd < old and
+ * visited[v] != iter, and visited[v] = iter marks it (the functor's filter). dist[u] is read live, as the
+ * reference functor reads it. frontier_out (n entries suffice: a vertex joins once) gets the winners in
+ * no particular order and no G1_INF entries; *out_count_dev is set by the call. weights_dev NULL: unit
+ * weights. frontier_out must not overlap frontier_in. */
+g1_status g1_advance_sssp_u32(uint32_t n, const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                              const uint32_t* weights_dev, const uint32_t* frontier_in_dev,
+                              const uint32_t* in_count_dev, uint32_t in_capacity, uint32_t* dist_dev,
+                              uint32_t* visited_dev, uint32_t iter, uint32_t* frontier_out_dev,
+                              uint32_t* out_count_dev, void* workspace_dev, size_t workspace_bytes,
+                              g1_stream_t stream);
+
+/* One BFS advance: every out-neighbour v of a valid frontier entry with depth[v] == G1_INF gets
+ * depth[v] = level (compare-and-swap: its first visitor only) and joins frontier_out. */
+g1_status g1_advance_bfs(uint32_t n, const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                         const uint32_t* frontier_in_dev, const uint32_t* in_count_dev, uint32_t in_capacity,
+                         uint32_t* depth_dev, uint32_t level, uint32_t* frontier_out_dev, uint32_t* out_count_dev,
+                         void* workspace_dev, size_t workspace_bytes, g1_stream_t stream);
+
+/* Stable compaction: the valid (non-G1_INF) entries of in[0..*in_count_dev), in order, to out;
+ * *out_count_dev their number. Turns a reference-style output with INVALID slots
+ * (include/GraphOne/engine/advance_gc.h:212-244) into a dense queue. */
+g1_status g1_filter_compact(const uint32_t* in_dev, const uint32_t* in_count_dev, uint32_t capacity,
+                            uint32_t* out_dev, uint32_t* out_count_dev, void* workspace_dev, size_t workspace_bytes,
+                            g1_stream_t stream);
+
+/* Queue -> bitmap (the bitmap is cleared first); bitmap -> queue in ascending vertex order, the order of
+ * the sparse engine's stable copy_if (include/GraphOne/engine/blas_gc.h:239-243). */
+g1_status g1_frontier_queue_to_bitmap(uint32_t n, const uint32_t* frontier_dev, const uint32_t* count_dev,
+                                      uint32_t capacity, uint32_t* bitmap_dev, g1_stream_t stream);
+g1_status g1_frontier_bitmap_to_queue(uint32_t n, const uint32_t* bitmap_dev, uint32_t* frontier_dev,
+                                      uint32_t* count_dev, void* workspace_dev, size_t workspace_bytes,
+                                      g1_stream_t stream);
+
+/* Min-plus apply of the BLAS engines (apps/sssp_blas.cu:75-82 inside include/GraphOne/engine/blas_gc.h:43-55):
+ * for every v, pred = y[v] != G1_INF && y[v] < d[v]; then d[v] = y[v] and x[v] = y[v] (the next
+ * Construct), else x[v] = G1_INF; *changed_dev = the number of such v (the next frontier's size). */
+g1_status g1_apply_minplus_u32(uint32_t n, const uint32_t* y_dev, uint32_t* d_dev, uint32_t* x_dev,
+                               uint32_t* changed_dev, void* workspace_dev, size_t workspace_bytes,
+                               g1_stream_t stream);
+
+/* PageRank apply of the all-active engine (apps/pagerank_blas.cu:98-123 inside
+ * include/GraphOne/engine/blas_gc.h:353-435): r_new[v] = base + y[v], x[v] = r_new[v] * iw[v]. With
+ * l1_dev, also *l1_dev = sum |r_new - r_old| in FP64, in a fixed order (blocks of 256 vertices, four
+ * sequential quarter sums each, blocks folded in order): the same bits on every run. */
+g1_status g1_pr_apply_f32(uint32_t n, const float* y_dev, float base, const float* iw_dev, const float* r_old_dev,
+                          float* r_new_dev, float* x_dev, double* l1_dev, void* workspace_dev,
+                          size_t workspace_bytes, g1_stream_t stream);
+/* *out_dev = sum |a - b| in FP64, in the fixed order of g1_pr_apply_f32. */
+g1_status g1_l1_delta_f64(uint32_t n, const float* a_dev, const float* b_dev, double* out_dev, void* workspace_dev,
+                          size_t workspace_bytes, g1_stream_t stream);
+
+/* Semiring SpMM Y = A (x) X with k dense columns, both row-major (X: n_cols x k, Y: n_rows x k), replacing
+ * the reference's SpMM boundary (include/GraphOne/archi/blas/spmm/cuda/cuda.cuh:65-116, the GNN aggregation
+ * of include/GraphOne/engine/blas_gnn.h:18-122) and serving batched multi-source steps:
+ *   min-plus u32 : Y[r][j] = min over the nonzeros (r, c) of (X[c][j] == G1_INF ? G1_INF : A[r,c] + X[c][j])
+ *   plus-times f32: Y[r][j] = sum over the nonzeros of A[r,c] * X[c][j], left to right per row with the
+ *                  compensated sum of the SpMV kernels
+ * col_mask_dev (device, ceil(k / 64) words; NULL = every column): only the columns whose bit is set are
+ * computed, the others keep their Y values. values_dev NULL: unit values. */
+g1_status g1_spmm_minplus_u32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz, uint32_t k,
+                              const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                              const uint32_t* values_dev, const uint32_t* x_dev, uint32_t* y_dev,
+                              const uint64_t* col_mask_dev, g1_stream_t stream);
+g1_status g1_spmm_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz, uint32_t k,
+                                const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                                const float* values_dev, const float* x_dev, float* y_dev,
+                                const uint64_t* col_mask_dev, g1_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------------
  * Host-side synthetic graphs (seeded, splitmix64; the random streams are spelled out at the top of
  * graphone_amd/csrc/gen.cpp).  They produce the CSR invariants of GraphLoader + EdgeCache::ToCsr
  * (include/GraphOne/loader/graph_loader.h:254-296, include/GraphOne/loader/edge_cache.h:50-105):
@@ -161,6 +252,17 @@ g1_status g1_assign_weights_u32(g1_host_csr* csr, uint32_t lo, uint32_t hi, uint
 /* Transpose (in-edge CSR): rows are destinations, columns ascending source ids; weights follow. */
 g1_status g1_host_csr_transpose(const g1_host_csr* in, g1_host_csr* out);
 void g1_host_csr_free(g1_host_csr* csr);
+
+/* Transpose of a device CSR, on the device (replaces ToCsc, include/GraphOne/mat/convert.h:62-88: expand the
+ * row ids, sort by column keeping the original order among equal columns, count the new row starts): a
+ * stable LSD radix sort, so each in-edge row lists its sources in ascending order with their weights.
+ * t_row_offsets holds n + 1 entries, t_col_indices and t_weights m. weights and t_weights are both NULL or
+ * both given (any 4-byte payload travels as its bits). The workspace is
+ * g1_csr_transpose_workspace_bytes(n, m, weighted) bytes of device memory. */
+size_t g1_csr_transpose_workspace_bytes(uint32_t n, uint32_t m, int32_t weighted);
+g1_status g1_csr_transpose(uint32_t n, uint32_t m, const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
+                           const uint32_t* weights_dev, uint32_t* t_row_offsets_dev, uint32_t* t_col_indices_dev,
+                           uint32_t* t_weights_dev, void* workspace_dev, size_t workspace_bytes, g1_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Device-resident graph.  Replaces graph::build<arch, views> + Graph/GraphArchRef

@@ -87,6 +87,32 @@ def test_reference_sssp_apps_bitexact(gpu, tmp_path, app):
         assert np.array_equal(got, want), (app, s)
 
 
+def test_reference_sssp_app_on_adaptive_frontier(gpu, tmp_path):
+    """sssp_adv with its frontier declared as the README's DblBufFrontier: iterations whose frontier holds
+    at least G1_DBL_PULL of the edges pull over the in-edges (built on the device: the app keeps only the
+    out-edge view), the others push; the distances stay bit-exact and both directions are taken."""
+    import graphone_amd as g1
+    exe = _need(os.path.join(BIN, "sssp_adv_dbl"))
+    src, dst, w = _graph(7)
+    adj = str(tmp_path / "g.adj")
+    _write_adj(adj, src, dst, w)
+    n = int(max(src.max(), dst.max())) + 1
+    host = g1.HostGraph.from_edges(n, src, dst, w, directed=False)
+    env = dict(os.environ, G1_DBL_TRACE="1", G1_DBL_PULL="0.05")
+    for s in (int(np.argmax(host.out_degrees())), 1):
+        out = str(tmp_path / "out.txt")
+        r = subprocess.run([exe, "-i", adj, "--src", str(s), "-o", out], capture_output=True, text=True,
+                           timeout=120, env=env)
+        assert r.returncode == 0, r.stdout + r.stderr
+        got = _read_out(out)
+        want = oracle.dijkstra(host.row_offsets, host.col_indices, host.weights, s).astype(np.float64)
+        want[want == oracle.INF] = FLT_MAX
+        assert np.array_equal(got, want), s
+        line = [x for x in r.stderr.splitlines() if "DblBufFrontier" in x][-1]
+        pushes, pulls = int(line.split()[2]), int(line.split()[5])
+        assert pushes >= 1 and pulls >= 1, line
+
+
 def test_reference_pagerank_app(gpu, tmp_path):
     import graphone_amd as g1
     exe = _need(os.path.join(BIN, "pagerank_blas"))
