@@ -265,6 +265,9 @@ def grid213_sssp(g1, cpu: bool) -> dict:
         got[eng] = d.numpy()
         out[eng] = {"engine": label, "ms": round(ms, 4), "iterations": runs[0].stats["iterations"],
                     "reached": runs[0].stats.get("reached")}
+    dropin = dropin_sssp_adv(h, 1)
+    if dropin is not None:
+        out["dropin_sssp_adv"] = dropin
     if cpu:
         import oracle
 
@@ -286,6 +289,73 @@ def grid213_sssp(g1, cpu: bool) -> dict:
     return out
 
 
+def spmm_benchmark(g1, g, host, k: int) -> dict:
+    """The step-level SpMM (g1_spmm_plustimes_f32, the GNN aggregation boundary): Y = A^T X over the graph's
+    in-edge CSR with its u32 weights as floats, X n x k row-major f32. Algorithmic bytes per call: 4(n+1)
+    row offsets + 8m (column + value) + 4k per nonzero (the gathered X row) + 4kn (Y); median of 5 calls
+    after one warm call, timed around the calls with a stream synchronization."""
+    import ctypes as C
+
+    n, m = host.n, host.m
+    ro, ci, _ = g.csr_dev(transpose=True)
+    t = host.transpose()
+    wf = g1.DeviceArray.from_numpy(np.concatenate([t.weights.astype(np.float32), np.zeros(1, np.float32)]))
+    del t
+    X = g1.DeviceArray(n * k, np.float32)
+    X.fill(1.0 / n)
+    Y = g1.DeviceArray(n * k, np.float32)
+    args_ = (C.c_uint32(n), C.c_uint32(n), C.c_uint32(m), C.c_uint32(k), C.c_void_p(ro), C.c_void_p(ci),
+             C.c_void_p(wf.ptr), C.c_void_p(X.ptr), C.c_void_p(Y.ptr), None, None)
+    g1._lib.call("g1_spmm_plustimes_f32", *args_)
+    times = []
+    for _ in range(5):
+        times.append(_wall(lambda: g1._lib.call("g1_spmm_plustimes_f32", *args_)))
+    ms = sorted(times)[2] * 1e3
+    b = 4.0 * (n + 1) + 8.0 * m + 4.0 * k * m + 4.0 * k * n
+    return {"k": k, "ms": round(ms, 3), "algorithmic_gb": round(b / 1e9, 3), "gbs": round(b / ms / 1e6, 1),
+            "frac": round(b / ms / 1e6 / HBM_PEAK_GBS, 4)}
+
+
+def dropin_sssp_adv(h, src: int) -> dict | None:
+    """The reference's own apps/sssp_adv.cu recompiled against this build's graph_one API
+    (tests/dropin/_bin/sssp_adv; absent where the reference tree was not there to build it), run on the
+    same graph written as an undirected .adj file: its own timer around Run (apps/sssp_adv.cu:104-109,
+    whole milliseconds), median of 5 runs; its distances compared with this build's advance engine."""
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "tests", "dropin", "_bin", "sssp_adv")
+    if not os.path.exists(exe):
+        return None
+    rp, ci, w = h.row_offsets, h.col_indices, h.weights
+    srcs = np.repeat(np.arange(h.n, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    keep = srcs < ci
+    with tempfile.TemporaryDirectory() as d:
+        adj = os.path.join(d, "g.adj")
+        lines = [f"{a}\t{b}\t{c}" for a, b, c in zip(srcs[keep].tolist(), ci[keep].tolist(), w[keep].tolist())]
+        with open(adj, "w") as f:
+            f.write("% the grid213 workload\n" + "\n".join(lines) + "\n")
+        ms, dist = [], None
+        for _ in range(5):
+            outp = os.path.join(d, "out.txt")
+            r = subprocess.run([exe, "-i", adj, "--src", str(src), "-o", outp], capture_output=True, text=True,
+                               timeout=300)
+            if r.returncode != 0:
+                return {"error": (r.stdout + r.stderr)[-300:]}
+            with open(outp) as f:
+                rows = f.read().splitlines()
+            ms.append(float(rows[0].split(":")[1].split()[0]))
+            dist = np.array([float(x.split("-", 1)[1]) for x in rows[1:]])
+    import graphone_amd as g1
+    ref = g1.Graph(h, symmetric=True).sssp(src, "advance").values.astype(np.float64)
+    ref[ref == 0xFFFFFFFF] = float(np.finfo(np.float32).max)
+    return {"app": "apps/sssp_adv.cu recompiled (4-rule port), float distances", "ms": sorted(ms)[2],
+            "ms_runs": ms, "timer": "the app's own, around Run, whole ms",
+            # the .adj file ends at the last vertex with an edge; vertices past it are unreached
+            "bit_exact_vs_advance_engine": bool(dist.size <= ref.size and np.array_equal(dist, ref[:dist.size])
+                                                and np.all(ref[dist.size:] == np.finfo(np.float32).max))}
+
+
 def pagerank_benchmark(g1, scale: int, seed: int, cpu: bool) -> dict:
     """PageRank (apps/pagerank_blas.cu formula) on an unweighted RMAT graph (edge factor 16, the main a/b/c/d):
     damping 0.85, FP32 ranks with Kahan row sums, FP64 L1 stop at 1e-7 or 50 iterations, both SpMV engines.
@@ -295,11 +365,16 @@ def pagerank_benchmark(g1, scale: int, seed: int, cpu: bool) -> dict:
     timed on every usable host core (the oracle's OpenMP loop) and reported as the CPU baseline."""
     t0 = time.perf_counter()
     h = g1.HostGraph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=seed + 10)
+    t1 = time.perf_counter()
     gr = g1.Graph(h)
-    log(f"pagerank graph RMAT{scale}: n={h.n} m={h.m} ready in {time.perf_counter() - t0:.1f}s")
+    g1.synchronize()
+    t2 = time.perf_counter()
+    log(f"pagerank graph RMAT{scale}: n={h.n} m={h.m} generated in {t1 - t0:.1f}s, device graph in {t2 - t1:.2f}s")
     n, m = h.n, h.m
     r = g1.DeviceArray(n, np.float32)
-    out = {"workload": f"rmat{scale}_pagerank", "n": n, "m": m, "alpha": 0.85, "tol_l1": 1e-7, "max_iter": 50}
+    out = {"workload": f"rmat{scale}_pagerank", "n": n, "m": m, "alpha": 0.85, "tol_l1": 1e-7, "max_iter": 50,
+           "device_graph_build_s": round(t2 - t1, 3),
+           "device_graph_build": "upload of the out-edge CSR and its transpose built on the device (radix sort)"}
     ranks = {}
     for eng in ("merge", "csrvec"):
         gr.pagerank(0.85, 50, 1e-7, "l1", eng, out=r, to_host=False)
@@ -451,9 +526,59 @@ def serving_benchmark(g1, args) -> dict:
         results.append(r)
         log(json.dumps(r))
     best = max(r["achieved_qps"] for r in results)
-    return {"graphs": sizes, "mix": {"bfs": 0.4, "sssp": 0.4, "ppr": 0.2}, "ppr": {"restart": 0.15, "max_iter": 50,
+    budget_point = serving_with_binding_budget(g1, graphs, make_queries, args)
+    log(json.dumps(budget_point))
+    return {"budget_point": budget_point, "graphs": sizes, "mix": {"bfs": 0.4, "sssp": 0.4, "ppr": 0.2}, "ppr": {"restart": 0.15, "max_iter": 50,
             "tol_l1": 1e-7}, "sequential_qps": round(seq_qps, 1), "sequential_sample": len(sample),
             "rates": results, "best_achieved_qps": best, "speedup_vs_sequential": round(best / seq_qps, 3)}
+
+
+def serving_with_binding_budget(g1, graphs, make_queries, args) -> dict:
+    """One rate point (1000 q/s offered, the same mix) under an HBM budget that binds: the resident graphs
+    plus 300 MB for results and sessions (about nine RMAT23 result vectors). A consumer thread fetches (and
+    so frees) each result as soon as its query is done, in submission order, so admission waits are
+    released by fetches; every query must complete."""
+    import threading
+
+    probe = g1.Server()
+    for gr in graphs:
+        probe.add_graph(gr)
+    resident = probe.stats()["hbm_used"]
+    probe.close()
+    budget = resident + 300 * 2**20
+    rate, count = 1000.0, int(min(args.serve_max_queries, 1000 * args.serve_seconds))
+    qs = make_queries(count, 4000)
+    arrivals = np.cumsum(np.random.default_rng(4001).exponential(1.0 / rate, size=count))
+    srv = g1.Server(hbm_budget_bytes=budget, batch_k=64)
+    ids = [srv.add_graph(gr) for gr in graphs]
+    qids, sts = [None] * count, [None] * count
+    ready = threading.Semaphore(0)
+
+    def consume():
+        for i in range(count):
+            ready.acquire()
+            sts[i] = srv.wait(qids[i], 600000)
+            srv.cancel(qids[i])  # the result is dropped: its bytes return to the budget
+
+    th = threading.Thread(target=consume)
+    th.start()
+    start = time.perf_counter()
+    for i, ((gi, kind, src), at) in enumerate(zip(qs, arrivals)):
+        while time.perf_counter() - start < at:
+            pass
+        qids[i] = srv.submit(ids[gi], kind, src, 50, 1e-7)
+        ready.release()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    lat = np.array([q["t_done_ms"] - q["t_submit_ms"] for q in sts])
+    span = (max(q["t_done_ms"] for q in sts) - min(q["t_submit_ms"] for q in sts)) / 1e3
+    ok = sum(1 for q in sts if q["state_name"] == "done")
+    return {"rate_qps": rate, "queries": count, "completed": ok, "hbm_budget_gb": round(budget / 1e9, 3),
+            "resident_gb": round(resident / 1e9, 3), "hbm_peak_gb": round(st["hbm_peak"] / 1e9, 3),
+            "admission_waits": st["admission_waits"], "achieved_qps": round(ok / span, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "batched": sum(1 for q in sts if q["path"] == 1)}
 
 
 def main() -> int:
@@ -479,6 +604,9 @@ def main() -> int:
     ap.add_argument("--serve-rates", default="100,1000,3000,10000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-max-queries", type=int, default=600)
+    ap.add_argument("--only", choices=["grid213", "serving"], default=None,
+                    help="run one secondary workload alone and print its JSON (for profiling runs); the "
+                         "benchmark line is not printed")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -488,6 +616,12 @@ def main() -> int:
 
     import graphone_amd as g1
 
+    if args.only == "grid213":
+        print(json.dumps(grid213_sssp(g1, not args.no_cpu)), flush=True)
+        return 0
+    if args.only == "serving":
+        print(json.dumps(serving_benchmark(g1, args)), flush=True)
+        return 0
     info = g1.device_info()
     t0 = time.perf_counter()
     host = g1.HostGraph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed)
@@ -556,6 +690,7 @@ def main() -> int:
         extras["bfs_push_only"] = {"ms": round(rb.stats["elapsed_ms"], 3),
                                    "gteps": round(m_reach / rb.stats["elapsed_ms"] / 1e6, 3)}
         extras["grid213_sssp"] = grid213_sssp(g1, not args.no_cpu)
+        extras["spmm_plustimes_k64"] = spmm_benchmark(g1, g, host, 64)
         log(json.dumps(extras))
 
     if args.copy_probe:

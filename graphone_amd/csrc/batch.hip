@@ -40,6 +40,14 @@ struct BatchWork {
 namespace g1 {
 namespace {
 
+// state[0] = frozen, state[1] = 0 (the batched PPR's device-side stop-rule state)
+__global__ void set_state_kernel(unsigned long long* state, unsigned long long frozen) {
+    if (threadIdx.x == 0) {
+        state[0] = frozen;
+        state[1] = 0ull;
+    }
+}
+
 // Min-plus SpMM kernel: at k = 64 the wavefront-per-tile kernel (unless G1_MINPLUS_WAVE=0); otherwise
 // the chunked, barrier-free walk (default) or the round-based one of the plus-times path
 // (G1_MINPLUS_ROUNDS=1). All give the same bits.
@@ -462,34 +470,81 @@ g1_status ppr_tile_run(g1_graph* g, BatchWork* b, const uint32_t* seeds_host, ui
     ppr_batch_init_kernel<K><<<fill_blocks(static_cast<size_t>(n) * K), 256, 0, s>>>(n, g->iw, g->seeds_dev, b->R[0],
                                                                                       b->X);
     G1_LAUNCHED();
-    unsigned long long frozen = 0;
-    for (uint32_t j = cols; j < K; ++j) frozen |= 1ull << j;  // padding columns never run
-    std::vector<uint32_t> col_iters(K, 0);
-    std::vector<double> deltas(K);
-    int cur = 0;
-    uint32_t it = 0;
-    const unsigned long long all = (K == 64) ? ~0ull : ((1ull << K) - 1ull);
-    while (frozen != all && it < max_iter) {
-        G1_TRY(launch_plustimes<K>(g, b->X, b->Y, b->recs, s));
-        rank_apply_batch_kernel<K><<<(nb + 256 / K - 1) / (256 / K), 256, 0, s>>>(
-            n, b->Y, 0.0f, g->seeds_dev, restart, g->iw, b->R[cur], b->R[cur ^ 1], b->X, g->rpart, nb, frozen);
+    // Device state of the stop rule: state[0] frozen columns (padding columns never run), state[1] the
+    // iteration count; written with the seeds' stream order, by a kernel (no memset in the captured work).
+    unsigned long long* state = b->changed + 4;
+    unsigned long long frozen0 = 0;
+    for (uint32_t j = cols; j < K; ++j) frozen0 |= 1ull << j;
+    set_state_kernel<<<1, 64, 0, s>>>(state, frozen0);
+    G1_LAUNCHED();
+    const double* deltas = g->rpart + static_cast<size_t>(nb) * kBatchCols;
+    // One iteration: SpMM, apply (frozen columns keep their ranks), the canonical L1 per column, the freeze.
+    // An iteration after every column froze changes nothing (frozen columns copy R into the other buffer).
+    auto iteration = [&](int cur, hipStream_t st) -> g1_status {
+        G1_TRY(launch_plustimes<K>(g, b->X, b->Y, b->recs, st));
+        rank_apply_batch_kernel<K><<<(nb + 256 / K - 1) / (256 / K), 256, 0, st>>>(
+            n, b->Y, 0.0f, g->seeds_dev, restart, g->iw, b->R[cur], b->R[cur ^ 1], b->X, g->rpart, nb, 0ull, state);
         G1_LAUNCHED();
-        reduce_partials_kernel<0><<<dim3(1, K), 256, 0, s>>>(g->rpart, nb, g->rpart + static_cast<size_t>(nb) * kBatchCols);
+        reduce_partials_kernel<0><<<dim3(1, K), 256, 0, st>>>(g->rpart, nb, g->rpart + static_cast<size_t>(nb) * kBatchCols);
         G1_LAUNCHED();
-        G1_HIP(hipMemcpyAsync(deltas.data(), g->rpart + static_cast<size_t>(nb) * kBatchCols, K * sizeof(double),
-                              hipMemcpyDeviceToHost, s));
-        G1_HIP(hipStreamSynchronize(s));
-        cur ^= 1;
-        ++it;
-        for (uint32_t j = 0; j < K; ++j) {
-            if ((frozen >> j) & 1ull) continue;
-            col_iters[j] = it;
-            if (deltas[j] < tol) frozen |= 1ull << j;  // the single-source run stops here too
+        ppr_freeze_kernel<K><<<1, 64, 0, st>>>(deltas, tol, state);
+        G1_LAUNCHED();
+        return G1_OK;
+    };
+    // Two iterations (R[0] -> R[1] -> R[0]) captured as one HIP graph for this call (its arguments --
+    // restart, tol -- are the call's), replayed while both fit in max_iter; the host reads the state (two
+    // words) once per replay and stops when every column froze.
+    hipGraphExec_t exec = nullptr;
+    if (max_iter >= 2) {
+        hipStream_t cs = nullptr;
+        G1_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipGraph_t graph = nullptr;
+        g1_status st = G1_OK;
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            st = iteration(0, cs);
+            if (st == G1_OK) st = iteration(1, cs);
+            e = hipStreamEndCapture(cs, &graph);
         }
+        (void)hipStreamDestroy(cs);
+        G1_TRY(st);
+        G1_HIP(e);
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        G1_HIP(e);
     }
-    // A column frozen at iteration t kept its rank in both buffers since, so R[cur] holds every column.
-    *iterations = it;
-    batch_columns_out_kernel<K, float><<<(n + 63) / 64, 256, 0, s>>>(n, b->R[cur], cols, out);
+    const unsigned long long all = (K == 64) ? ~0ull : ((1ull << K) - 1ull);
+    unsigned long long* h = g->h_ctr + 12;
+    h[0] = frozen0;
+    h[1] = 0ull;
+    uint32_t launched = 0;
+    g1_status st = G1_OK;
+    while (launched < max_iter) {
+        if (max_iter - launched >= 2) {
+            const hipError_t e = hipGraphLaunch(exec, s);
+            if (e != hipSuccess) {
+                st = set_error(G1_ERR_HIP, "ppr batch: graph launch: %s", hipGetErrorString(e));
+                break;
+            }
+            launched += 2;
+        } else {
+            if ((st = iteration(0, s)) != G1_OK) break;  // an even count of iterations ran before: R[0] is current
+            launched += 1;
+        }
+        hipError_t e = hipMemcpyAsync(h, state, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            st = set_error(G1_ERR_HIP, "ppr batch: state read: %s", hipGetErrorString(e));
+            break;
+        }
+        if (h[0] == all) break;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    G1_TRY(st);
+    // h[1] iterations ran with a running column: a frozen column kept its rank in both buffers since, and
+    // the buffer written last is R[h[1] % 2] (no-op iterations after that copy ranks only).
+    *iterations = static_cast<uint32_t>(h[1]);
+    batch_columns_out_kernel<K, float><<<(n + 63) / 64, 256, 0, s>>>(n, b->R[h[1] % 2], cols, out);
     G1_LAUNCHED();
     return G1_OK;
 }
@@ -766,7 +821,7 @@ g1_status session_step_ppr(Session* ss, uint64_t* done) {
     const unsigned long long live = ss->active & ~ss->converged;
     rank_apply_batch_kernel<K><<<(nb + 256 / K - 1) / (256 / K), 256, 0, s>>>(
         n, ss->Y, 0.0f, ss->seeds_dev, ss->restart, ss->iw, ss->R[ss->rcur], ss->R[ss->rcur ^ 1], ss->X, ss->part, nb,
-        ~live);
+        ~live, nullptr);
     G1_LAUNCHED();
     reduce_partials_kernel<0><<<dim3(1, K), 256, 0, s>>>(ss->part, nb, ss->deltas);
     G1_LAUNCHED();

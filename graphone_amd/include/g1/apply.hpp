@@ -169,7 +169,8 @@ static __global__ void __launch_bounds__(256) rank_apply_kernel(uint32_t n, cons
 
 // K columns, row-major n x K: a group of K lanes (lane j = column j) walks one block of 256 vertices in
 // vertex order with coalesced K-wide accesses; a workgroup covers 256 / K blocks. Column j with bit j of
-// `frozen` set keeps its rank (r' = r) and contributes nothing (it has converged).
+// `frozen` set keeps its rank (r' = r) and contributes nothing (it has converged). With frozen_dev the
+// mask is read from device memory instead (a captured iteration whose mask the device updates).
 template <int K>
 __global__ void __launch_bounds__(256) rank_apply_batch_kernel(uint32_t n, const float* __restrict__ Y,
                                                                float base_all, const uint32_t* __restrict__ seeds,
@@ -177,12 +178,13 @@ __global__ void __launch_bounds__(256) rank_apply_batch_kernel(uint32_t n, const
                                                                const float* __restrict__ R_old,
                                                                float* __restrict__ R_new, float* __restrict__ X,
                                                                double* __restrict__ partials, uint32_t nblocks,
-                                                               unsigned long long frozen) {
+                                                               unsigned long long frozen,
+                                                               const unsigned long long* __restrict__ frozen_dev) {
     constexpr uint32_t G = 256 / K;
     const uint32_t j = threadIdx.x % K;
     const uint32_t blk = blockIdx.x * G + threadIdx.x / K;
     if (blk >= nblocks) return;
-    const bool fz = (frozen >> j) & 1ull;
+    const bool fz = ((frozen_dev ? *frozen_dev : frozen) >> j) & 1ull;
     const uint32_t seed = seeds[j];
     double q[4];
 #pragma unroll
@@ -208,6 +210,23 @@ __global__ void __launch_bounds__(256) rank_apply_batch_kernel(uint32_t n, const
         q[w] = acc;
     }
     partials[static_cast<size_t>(j) * nblocks + blk] = ((q[0] + q[1]) + q[2]) + q[3];
+}
+
+// The stop rule of the batched personalized PageRank, per column, on the device: a column still running
+// whose FP64 L1 delta (deltas[j]) fell below tol freezes; state[0] is the frozen mask, state[1] counts the
+// iterations that began with a running column. One wave.
+template <int K>
+__global__ void __launch_bounds__(64) ppr_freeze_kernel(const double* __restrict__ deltas, double tol,
+                                                        unsigned long long* __restrict__ state) {
+    const uint32_t j = lane_id();
+    const unsigned long long all = (K == 64) ? ~0ull : ((1ull << K) - 1ull);
+    const unsigned long long f = state[0];
+    const bool stop = j < static_cast<uint32_t>(K) && !((f >> j) & 1ull) && deltas[j] < tol;
+    const unsigned long long nf = f | ballot(stop);
+    if (j == 0) {
+        if (f != all) state[1] += 1ull;
+        state[0] = nf;
+    }
 }
 
 // Folds per-block partials in block order into out[0] (sum or max); one block.
