@@ -291,9 +291,10 @@ def grid213_sssp(g1, cpu: bool) -> dict:
 
 def spmm_benchmark(g1, g, host, k: int) -> dict:
     """The step-level SpMM (g1_spmm_plustimes_f32, the GNN aggregation boundary): Y = A^T X over the graph's
-    in-edge CSR with its u32 weights as floats, X n x k row-major f32. Algorithmic bytes per call: 4(n+1)
-    row offsets + 8m (column + value) + 4k per nonzero (the gathered X row) + 4kn (Y); median of 5 calls
-    after one warm call, timed around the calls with a stream synchronization."""
+    in-edge CSR with its u32 weights as floats, X n x k row-major f32, each call partitioning the matrix
+    into merge-path tiles. Algorithmic bytes per call: 4(n+1) row offsets + 8m (column + value) + 4k per
+    nonzero (the gathered X row) + 4kn (Y); median of 5 calls after one warm call, timed around the calls
+    with a stream synchronization."""
     import ctypes as C
 
     n, m = host.n, host.m
@@ -304,8 +305,11 @@ def spmm_benchmark(g1, g, host, k: int) -> dict:
     X = g1.DeviceArray(n * k, np.float32)
     X.fill(1.0 / n)
     Y = g1.DeviceArray(n * k, np.float32)
+    wsb = g1._lib.lib().g1_spmm_workspace_bytes(n, m)
+    ws = g1.DeviceArray((wsb + 3) // 4, np.uint32)
     args_ = (C.c_uint32(n), C.c_uint32(n), C.c_uint32(m), C.c_uint32(k), C.c_void_p(ro), C.c_void_p(ci),
-             C.c_void_p(wf.ptr), C.c_void_p(X.ptr), C.c_void_p(Y.ptr), None, None)
+             C.c_void_p(wf.ptr), C.c_void_p(X.ptr), C.c_void_p(Y.ptr), None, C.c_void_p(ws.ptr), C.c_size_t(wsb),
+             None)
     g1._lib.call("g1_spmm_plustimes_f32", *args_)
     times = []
     for _ in range(5):
@@ -535,9 +539,10 @@ def serving_benchmark(g1, args) -> dict:
 
 def serving_with_binding_budget(g1, graphs, make_queries, args) -> dict:
     """One rate point (1000 q/s offered, the same mix) under an HBM budget that binds: the resident graphs
-    plus 300 MB for results and sessions (about nine RMAT23 result vectors). A consumer thread fetches (and
-    so frees) each result as soon as its query is done, in submission order, so admission waits are
-    released by fetches; every query must complete."""
+    plus 300 MB for results and sessions (about nine RMAT23 result vectors). A consumer thread polls the
+    outstanding queries and drops (and so frees) each result as soon as its query is done, in completion
+    order, so admission waits are released by those frees; every query must complete. A point that makes
+    no progress for 60 s is reported as stalled instead of waiting on."""
     import threading
 
     probe = g1.Server()
@@ -552,13 +557,31 @@ def serving_with_binding_budget(g1, graphs, make_queries, args) -> dict:
     srv = g1.Server(hbm_budget_bytes=budget, batch_k=64)
     ids = [srv.add_graph(gr) for gr in graphs]
     qids, sts = [None] * count, [None] * count
-    ready = threading.Semaphore(0)
+    submitted = [0]
+    stalled = [False]
+    lock = threading.Lock()
 
     def consume():
-        for i in range(count):
-            ready.acquire()
-            sts[i] = srv.wait(qids[i], 600000)
-            srv.cancel(qids[i])  # the result is dropped: its bytes return to the budget
+        left = set()
+        done = added = 0
+        last = time.perf_counter()
+        while done < count:
+            with lock:
+                upto = submitted[0]
+            left.update(range(added, upto))
+            added = upto
+            for i in list(left):
+                q = srv.poll(qids[i])
+                if q["state_name"] in ("done", "failed", "cancelled"):
+                    sts[i] = q
+                    srv.cancel(qids[i])  # the result is dropped: its bytes return to the budget
+                    left.discard(i)
+                    done += 1
+                    last = time.perf_counter()
+            if time.perf_counter() - last > 60.0:
+                stalled[0] = True
+                break
+            time.sleep(0.0005)
 
     th = threading.Thread(target=consume)
     th.start()
@@ -567,10 +590,14 @@ def serving_with_binding_budget(g1, graphs, make_queries, args) -> dict:
         while time.perf_counter() - start < at:
             pass
         qids[i] = srv.submit(ids[gi], kind, src, 50, 1e-7)
-        ready.release()
+        with lock:
+            submitted[0] = i + 1
     th.join()
     st = srv.stats()
     srv.close()
+    if stalled[0]:
+        return {"rate_qps": rate, "queries": count, "stalled": True, "completed": sum(q is not None for q in sts),
+                "admission_waits": st["admission_waits"], "hbm_budget_gb": round(budget / 1e9, 3)}
     lat = np.array([q["t_done_ms"] - q["t_submit_ms"] for q in sts])
     span = (max(q["t_done_ms"] for q in sts) - min(q["t_submit_ms"] for q in sts)) / 1e3
     ok = sum(1 for q in sts if q["state_name"] == "done")

@@ -447,6 +447,13 @@ class Server:
         self._graphs.append(graph)  # the graph must outlive the server
         return gid.value
 
+    def remove_graph(self, graph_id: int) -> None:
+        """Unregisters a graph once its queued and running queries are final (g1_server_remove_graph)."""
+        call("g1_server_remove_graph", self._h, C.c_uint32(graph_id))
+
+    def set_budget(self, hbm_budget_bytes: int) -> None:
+        call("g1_server_set_budget", self._h, C.c_uint64(hbm_budget_bytes))
+
     def submit(self, graph_id: int, kind: str, source: int, max_iter: int = 50, tol: float = 1e-7) -> int:
         q = _lib.Query()
         q.graph_id = int(graph_id)

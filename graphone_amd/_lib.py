@@ -199,8 +199,9 @@ _SIGNATURES = {
     "g1_apply_minplus_u32": (_st, [_u32, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "g1_pr_apply_f32": (_st, [_u32, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "g1_l1_delta_f64": (_st, [_u32, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
-    "g1_spmm_minplus_u32": (_st, [_u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "g1_spmm_plustimes_f32": (_st, [_u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "g1_spmm_workspace_bytes": (C.c_size_t, [_u32, _u32]),
+    "g1_spmm_minplus_u32": (_st, [_u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "g1_spmm_plustimes_f32": (_st, [_u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "g1_spmv_csrvec_minplus_u32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "g1_spmv_csrvec_plustimes_f32": (_st, [_u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "g1_gen_rmat": (_st, [_u32, _u32, C.c_double, C.c_double, C.c_double, _u64, _i32, C.POINTER(HostCSR)]),
@@ -228,6 +229,8 @@ _SIGNATURES = {
     "g1_server_create": (_st, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
     "g1_server_destroy": (_st, [_vp]),
     "g1_server_add_graph": (_st, [_vp, _vp, C.POINTER(_u32)]),
+    "g1_server_remove_graph": (_st, [_vp, _u32]),
+    "g1_server_set_budget": (_st, [_vp, _u64]),
     "g1_server_submit": (_st, [_vp, C.POINTER(Query), C.POINTER(_u64)]),
     "g1_server_poll": (_st, [_vp, _u64, C.POINTER(QueryStatus)]),
     "g1_server_wait": (_st, [_vp, _u64, C.c_double, C.POINTER(QueryStatus)]),

@@ -59,6 +59,10 @@ struct GraphSlot {
     std::thread worker;
     std::vector<std::thread> helpers;
     std::vector<g1_graph*> views;
+    uint64_t bytes = 0;     // the graph's reservation in the budget
+    bool draining = false;  // g1_server_remove_graph: no new queries; the graph's queries finish
+    bool closing = false;   // its threads exit
+    bool removed = false;   // threads joined, views destroyed, bytes returned
 };
 
 }  // namespace
@@ -174,8 +178,8 @@ void single_loop(g1_server* sv, GraphSlot* slot, g1_graph* view) {
         std::vector<Query*> road;  // BFS / SSSP on a low-degree graph: several run in one launch
         {
             std::unique_lock<std::mutex> lock(sv->mu);
-            sv->cv_work.wait(lock, [&] { return sv->stop || !slot->singles.empty(); });
-            if (sv->stop) break;
+            sv->cv_work.wait(lock, [&] { return sv->stop || slot->closing || !slot->singles.empty(); });
+            if (sv->stop || slot->closing) break;
             q = slot->singles.front();
             slot->singles.pop_front();
             if (q->cancel) {
@@ -432,8 +436,8 @@ struct Worker {
                 std::unique_lock<std::mutex> lock(sv->mu);
                 if (!busy())
                     sv->cv_work.wait_for(lock, std::chrono::milliseconds(20),
-                                         [&] { return sv->stop || !slot->admitted.empty(); });
-                if (sv->stop) break;
+                                         [&] { return sv->stop || slot->closing || !slot->admitted.empty(); });
+                if (sv->stop || slot->closing) break;
                 while (!slot->admitted.empty()) {
                     fresh.push_back(slot->admitted.front());
                     slot->admitted.pop_front();
@@ -455,12 +459,16 @@ struct Worker {
                 if (sess[k]) session_round(k);
             close_idle_sessions();
         }
-        // shutdown: everything still held is cancelled
+        // shutdown (or the graph's removal): everything still held is cancelled, sessions return their memory
         for (int k = 1; k < 3; ++k) {
             for (uint32_t j = 0; j < g1::kBatchCols; ++j)
                 if (col_query[k][j]) finish(col_query[k][j], G1_OK, 0, true);
             for (Query* q : pending[k]) finish(q, G1_OK, 0, true);
-            if (sess[k]) g1::session_destroy(sess[k]);
+            if (sess[k]) {
+                g1::session_destroy(sess[k]);
+                std::lock_guard<std::mutex> lock(sv->mu);
+                sv->unreserve(sess_bytes[k]);
+            }
         }
         for (int k = 0; k < 3; ++k)
             for (Query* q : backlog[k]) finish(q, G1_OK, 0, true);
@@ -571,6 +579,7 @@ g1_status g1_server_destroy(g1_server* sv) {
             for (Query* q : gs->admitted) q->cancel = true;
         }
         sv->cv_work.notify_all();
+        sv->cv_done.notify_all();
     }
     for (auto& gs : sv->graphs) {
         if (gs->worker.joinable()) gs->worker.join();
@@ -606,6 +615,7 @@ g1_status g1_server_add_graph(g1_server* sv, g1_graph* graph, uint32_t* graph_id
     auto gs = std::make_unique<GraphSlot>();
     gs->g = graph;
     gs->n = n;
+    gs->bytes = bytes;
     for (uint32_t i = 0; i < sv->cfg.single_workers; ++i) {
         g1_graph* v = nullptr;
         const g1_status st = g1_graph_create_view(graph, &v);
@@ -631,13 +641,62 @@ g1_status g1_server_add_graph(g1_server* sv, g1_graph* graph, uint32_t* graph_id
     return G1_OK;
 }
 
+g1_status g1_server_remove_graph(g1_server* sv, uint32_t graph_id) {
+    G1_CHECK(sv != nullptr, G1_ERR_INVALID_ARG, "g1_server_remove_graph: null server");
+    GraphSlot* gs = nullptr;
+    {
+        std::unique_lock<std::mutex> lock(sv->mu);
+        G1_CHECK(graph_id < sv->graphs.size() && !sv->graphs[graph_id]->draining, G1_ERR_NOT_FOUND,
+                 "g1_server_remove_graph: unknown or removed graph %u", graph_id);
+        gs = sv->graphs[graph_id].get();
+        gs->draining = true;  // submit refuses the graph from now on
+        // the graph's queued and running queries finish first (their results stay fetchable)
+        auto live = [&] {
+            for (const auto& kv : sv->queries) {
+                const Query* q = kv.second.get();
+                if (q->q.graph_id == graph_id && (q->state == G1_Q_QUEUED || q->state == G1_Q_RUNNING)) return true;
+            }
+            return false;
+        };
+        sv->cv_done.wait(lock, [&] { return sv->stop || !live(); });
+        G1_CHECK(!sv->stop, G1_ERR_CANCELLED, "g1_server_remove_graph: server is shutting down");
+        gs->closing = true;
+        sv->cv_work.notify_all();
+    }
+    // the graph's threads exit (without the lock: they take it on their way out), then its views go
+    if (gs->worker.joinable()) gs->worker.join();
+    for (auto& h : gs->helpers)
+        if (h.joinable()) h.join();
+    for (g1_graph* v : gs->views) g1_graph_destroy(v);
+    gs->views.clear();
+    std::lock_guard<std::mutex> lock(sv->mu);
+    sv->unreserve(gs->bytes);
+    gs->bytes = 0;
+    gs->removed = true;
+    for (auto& other : sv->graphs)
+        if (!other->draining) admit(sv, other.get());  // the freed bytes may admit waiting queries
+    sv->cv_work.notify_all();
+    return G1_OK;
+}
+
+g1_status g1_server_set_budget(g1_server* sv, uint64_t hbm_budget_bytes) {
+    G1_CHECK(sv != nullptr && hbm_budget_bytes > 0, G1_ERR_INVALID_ARG, "g1_server_set_budget: null server or zero budget");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    sv->cfg.hbm_budget_bytes = hbm_budget_bytes;
+    sv->st.hbm_budget = hbm_budget_bytes;
+    for (auto& gs : sv->graphs)
+        if (!gs->draining) admit(sv, gs.get());
+    sv->cv_work.notify_all();
+    return G1_OK;
+}
+
 g1_status g1_server_submit(g1_server* sv, const g1_query* query, uint64_t* query_id) {
     G1_CHECK(sv != nullptr && query != nullptr && query_id != nullptr, G1_ERR_INVALID_ARG,
              "g1_server_submit: null argument");
     std::lock_guard<std::mutex> lock(sv->mu);
     G1_CHECK(!sv->stop, G1_ERR_CANCELLED, "g1_server_submit: server is shutting down");
-    G1_CHECK(query->graph_id < sv->graphs.size(), G1_ERR_NOT_FOUND, "g1_server_submit: unknown graph %u",
-             query->graph_id);
+    G1_CHECK(query->graph_id < sv->graphs.size() && !sv->graphs[query->graph_id]->draining, G1_ERR_NOT_FOUND,
+             "g1_server_submit: unknown or removed graph %u", query->graph_id);
     G1_CHECK(query->kind == G1_QUERY_BFS || query->kind == G1_QUERY_SSSP || query->kind == G1_QUERY_PPR,
              G1_ERR_INVALID_ARG, "g1_server_submit: unknown query kind %d", query->kind);
     GraphSlot* gs = sv->graphs[query->graph_id].get();

@@ -3,13 +3,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "g1/apply.hpp"
 #include "g1/convert.hpp"
 #include "g1/semiring.hpp"
+#include "g1/spmv_merge.hpp"
 #include "g1/step.hpp"
 #include "graph.hpp"
 #include "internal.hpp"
+#include "spmv_launch.hpp"
 
 namespace g1 {
 namespace {
@@ -103,19 +106,42 @@ g1_status advance(uint32_t n, const uint32_t* row_off, const uint32_t* col, cons
     return G1_OK;
 }
 
+// SpMM workspace: the merge-path partition of the matrix, then the tile records of one 64-column pass
+// (sized for the larger record, the compensated plus-times one).
+size_t spmm_ws_bytes(uint32_t n_rows, uint32_t nnz) {
+    const size_t t = merge_num_tiles(n_rows, nnz);
+    return al((t + 1) * sizeof(MergeCoord)) + t * sizeof(SpmmTileRec<PlusTimesKahanF32, 64>) + 256;
+}
+
 template <typename S>
 g1_status spmm(uint32_t n_rows, uint32_t n_cols, uint32_t nnz, uint32_t k, const uint32_t* row_off, const uint32_t* col,
                const typename S::value_t* vals, const typename S::x_t* X, typename S::y_t* Y,
-               const unsigned long long* mask, hipStream_t s, const char* what) {
+               const unsigned long long* mask, void* ws, size_t ws_bytes, hipStream_t s, const char* what) {
     G1_CHECK(n_rows == 0 || k == 0 || (row_off && Y), G1_ERR_INVALID_ARG, "%s: null row_offsets or Y", what);
     G1_CHECK(nnz == 0 || k == 0 || (col && X), G1_ERR_INVALID_ARG, "%s: null col_indices or X", what);
     G1_CHECK(n_cols > 0 || nnz == 0, G1_ERR_INVALID_ARG, "%s: nnz > 0 with zero columns", what);
+    G1_CHECK(nnz < kInf, G1_ERR_INVALID_ARG, "%s: nnz must be below 2^32-1", what);
     if (n_rows == 0 || k == 0) return G1_OK;
-    const uint64_t items = static_cast<uint64_t>(n_rows) * cdiv(k, kWave);
-    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(cdiv(items, 4), 8192));
-    if (vals) step_spmm_kernel<S, true><<<blocks, 256, 0, s>>>(n_rows, k, row_off, col, vals, X, Y, mask);
-    else step_spmm_kernel<S, false><<<blocks, 256, 0, s>>>(n_rows, k, row_off, col, vals, X, Y, mask);
-    G1_LAUNCHED();
+    G1_CHECK(ws != nullptr && ws_bytes >= spmm_ws_bytes(n_rows, nnz), G1_ERR_INVALID_ARG,
+             "%s: workspace of %zu bytes, %zu needed (g1_spmm_workspace_bytes)", what, ws_bytes,
+             spmm_ws_bytes(n_rows, nnz));
+    const uint32_t tiles = merge_num_tiles(n_rows, nnz);
+    G1_TRY(launch_merge_partition(n_rows, nnz, row_off, ws, s));
+    const MergeCoord* coords = merge_coords(ws);
+    auto* recs = reinterpret_cast<SpmmTileRec<S, 64>*>(static_cast<char*>(ws) + al((tiles + 1ull) * sizeof(MergeCoord)));
+    using T = typename S::x_t;
+    const T neutral = std::is_same_v<S, MinPlusU32> ? static_cast<T>(kInf) : T(0);
+    for (uint32_t c0 = 0; c0 < k; c0 += 64) {
+        StridedColumnsPolicy<T> pol{X, Y, k, c0, std::min<uint32_t>(64u, k - c0), mask, neutral};
+        spmm_tile_kernel<S, 64, StridedColumnsPolicy<T>><<<tiles, 256, 0, s>>>(n_rows, nnz, row_off, col, vals, coords,
+                                                                             recs, pol, nullptr);
+        G1_LAUNCHED();
+        if (tiles > 1) {
+            spmm_fixup_kernel<S, 64, StridedColumnsPolicy<T>><<<(tiles * 64u + 255) / 256, 256, 0, s>>>(
+                tiles, coords, recs, pol, nullptr);
+            G1_LAUNCHED();
+        }
+    }
     return G1_OK;
 }
 
@@ -245,20 +271,22 @@ g1_status g1_l1_delta_f64(uint32_t n, const float* a, const float* b, double* ou
     return G1_OK;
 }
 
+size_t g1_spmm_workspace_bytes(uint32_t n_rows, uint32_t nnz) { return spmm_ws_bytes(n_rows, nnz); }
+
 g1_status g1_spmm_minplus_u32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz, uint32_t k, const uint32_t* row_offsets,
                               const uint32_t* col_indices, const uint32_t* values, const uint32_t* X, uint32_t* Y,
-                              const uint64_t* col_mask, g1_stream_t stream) {
+                              const uint64_t* col_mask, void* ws, size_t ws_bytes, g1_stream_t stream) {
     return spmm<MinPlusU32>(n_rows, n_cols, nnz, k, row_offsets, col_indices, values, X, Y,
-                            reinterpret_cast<const unsigned long long*>(col_mask), as_stream(stream),
+                            reinterpret_cast<const unsigned long long*>(col_mask), ws, ws_bytes, as_stream(stream),
                             "g1_spmm_minplus_u32");
 }
 
 g1_status g1_spmm_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz, uint32_t k, const uint32_t* row_offsets,
                                 const uint32_t* col_indices, const float* values, const float* X, float* Y,
-                                const uint64_t* col_mask, g1_stream_t stream) {
+                                const uint64_t* col_mask, void* ws, size_t ws_bytes, g1_stream_t stream) {
     return spmm<PlusTimesKahanF32>(n_rows, n_cols, nnz, k, row_offsets, col_indices, values, X, Y,
-                                   reinterpret_cast<const unsigned long long*>(col_mask), as_stream(stream),
-                                   "g1_spmm_plustimes_f32");
+                                   reinterpret_cast<const unsigned long long*>(col_mask), ws, ws_bytes,
+                                   as_stream(stream), "g1_spmm_plustimes_f32");
 }
 
 }  // extern "C"

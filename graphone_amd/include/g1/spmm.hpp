@@ -588,7 +588,7 @@ constexpr uint32_t kCloseBatch = 4;  // rows applied together (their old distanc
 
 // Bit q of starts[q >> 6]: a non-empty row of the CSR begins at nonzero q. starts has (nnz >> 6) + 2
 // words and must be zero on entry.
-__global__ void __launch_bounds__(256) row_starts_kernel(uint32_t n_rows, const uint32_t* __restrict__ row_offsets,
+static __global__ void __launch_bounds__(256) row_starts_kernel(uint32_t n_rows, const uint32_t* __restrict__ row_offsets,
                                                          unsigned long long* __restrict__ starts) {
     for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < n_rows; r += gridDim.x * 256) {
         const uint32_t b = row_offsets[r];
@@ -613,7 +613,7 @@ __device__ __forceinline__ uint32_t widen_dist(DT raw) {
 
 // act[v >> 5] bit (v & 31): mask[v] != 0 (the vertex changed in the last iteration). One wave per 64
 // vertices; act has 2 * ceil(n / 64) words.
-__global__ void __launch_bounds__(256) act_bits_kernel(uint32_t n, const unsigned long long* __restrict__ mask,
+static __global__ void __launch_bounds__(256) act_bits_kernel(uint32_t n, const unsigned long long* __restrict__ mask,
                                                        uint32_t* __restrict__ act) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t groups = (n + 63u) / 64u;

@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #include "g1/device.hpp"
+#include "g1/spmm.hpp"
 #include "g1/sweep.hpp"
 
 namespace g1 {
@@ -261,54 +262,39 @@ step_pr_apply_kernel(uint32_t n, const float* __restrict__ y, float base, const 
     if (threadIdx.x == 0) partials[blockIdx.x] = ((s_q[0] + s_q[1]) + s_q[2]) + s_q[3];
 }
 
-// Semiring SpMM, Y = A (x) X with k dense columns stored row-major (X: n_cols x k, Y: n_rows x k). A wave
-// owns one row at a time and a chunk of up to 64 columns (lane j: column c0 + j): per nonzero the column
-// index and value are wave-uniform and the k values of the source row arrive in one coalesced access.
-// Columns whose bit is clear in col_mask (bit j of word j / 64; null: all columns) keep their Y values.
-// Plus-times folds each row's products left to right with the compensated (Kahan-Babuska) sum of the
-// SpMV kernels.
-template <typename S, bool kHasValues>
-static __global__ void __launch_bounds__(256)
-step_spmm_kernel(uint32_t n_rows, uint32_t k, const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
-                 const typename S::value_t* __restrict__ vals, const typename S::x_t* __restrict__ X,
-                 typename S::y_t* __restrict__ Y, const unsigned long long* __restrict__ col_mask) {
-    using acc_t = typename S::acc_t;
-    const uint32_t lane = lane_id();
-    const uint32_t chunks = (k + kWave - 1) / kWave;
-    const uint64_t items = static_cast<uint64_t>(n_rows) * chunks;
-    const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
-    for (uint64_t it = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; it < items;
-         it += waves) {  // wave-uniform
-        const uint32_t r = static_cast<uint32_t>(it / chunks);
-        const uint32_t j = static_cast<uint32_t>(it % chunks) * kWave + lane;
-        bool on = j < k;
-        if (on && col_mask) on = (col_mask[j / 64] >> (j % 64)) & 1ull;
-        const uint32_t b = row_off[r], e = row_off[r + 1];
-        acc_t acc = S::identity();
-        uint32_t p = b;
-        for (; p + 4 <= e; p += 4) {  // four source rows in flight
-            uint32_t c[4];
-            typename S::value_t w[4];
-            typename S::x_t xv[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                c[q] = col[p + q];
-                w[q] = kHasValues ? vals[p + q] : typename S::value_t(1);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) xv[q] = on ? X[static_cast<size_t>(c[q]) * k + j] : typename S::x_t(0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) S::add(acc, S::combine(w[q], xv[q]));
-        }
-        for (; p < e; ++p) {
-            const uint32_t c = col[p];
-            const typename S::value_t w = kHasValues ? vals[p] : typename S::value_t(1);
-            const typename S::x_t xv = on ? X[static_cast<size_t>(c) * k + j] : typename S::x_t(0);
-            S::add(acc, S::combine(w, xv));
-        }
-        if (on) Y[static_cast<size_t>(r) * k + j] = S::finalize(acc);
+// Semiring SpMM, Y = A (x) X with k dense columns stored row-major (X: n_cols x k, Y: n_rows x k), on the
+// merge-path tile kernels of the batched engines (spmm_tile_kernel + spmm_fixup_kernel at 64 columns per
+// pass): every tile holds the same number of rows plus nonzeros, so a hub row is spread over many tiles
+// instead of one wave. This policy reads and writes the 64-column chunk starting at column c0 of the
+// caller's arrays (row stride ld = k); lanes past the chunk's kc columns, and columns whose bit is clear in
+// the device mask (bit j of word j / 64; null: every column), read nothing and write nothing, so their Y
+// values stay. Plus-times folds in the single-source merge-path SpMV's order (Kahan-Babuska), min-plus is
+// order-free.
+template <typename T>
+struct StridedColumnsPolicy {
+    using aux_t = uint32_t;
+    static constexpr bool kHasAux = false;
+    static constexpr bool kRowOld = false;
+    const T* X;
+    T* Y;
+    uint32_t ld;  // k
+    uint32_t c0;  // first column of the chunk
+    uint32_t kc;  // columns in the chunk (<= 64)
+    const unsigned long long* mask;
+    T neutral;    // what a skipped column reads (the semiring's absorbing x: INF for min-plus, 0 for plus-times)
+    __device__ __forceinline__ bool on(uint32_t j) const {
+        return j < kc && (!mask || ((mask[(c0 + j) >> 6] >> ((c0 + j) & 63u)) & 1ull));
     }
-}
+    __device__ __forceinline__ aux_t aux(uint32_t) const { return 0u; }
+    __device__ __forceinline__ T x(uint32_t u, aux_t, uint32_t j, bool has) const {
+        return (has && on(j)) ? X[static_cast<size_t>(u) * ld + c0 + j] : neutral;
+    }
+    __device__ __forceinline__ T row_old(uint32_t, uint32_t) const { return neutral; }
+    __device__ __forceinline__ uint64_t row_done(uint32_t row, uint32_t j, T y, T) const {
+        if (on(j)) Y[static_cast<size_t>(row) * ld + c0 + j] = y;
+        return 0ull;
+    }
+};
 
 // Stable compaction of the valid (non-G1_INF) entries of in[0..count), pass 1: per 256-entry block counts.
 static __global__ void __launch_bounds__(256)

@@ -208,15 +208,20 @@ g1_status g1_l1_delta_f64(uint32_t n, const float* a_dev, const float* b_dev, do
  *   plus-times f32: Y[r][j] = sum over the nonzeros of A[r,c] * X[c][j], left to right per row with the
  *                  compensated sum of the SpMV kernels
  * col_mask_dev (device, ceil(k / 64) words; NULL = every column): only the columns whose bit is set are
- * computed, the others keep their Y values. values_dev NULL: unit values. */
+ * computed, the others keep their Y values. values_dev NULL: unit values. The rows and nonzeros are
+ * cut into merge-path tiles of equal size (a row of any length spreads over tiles), 64 columns per pass;
+ * the workspace (the partition and the tiles' carries) is g1_spmm_workspace_bytes(n_rows, nnz) bytes. */
+size_t g1_spmm_workspace_bytes(uint32_t n_rows, uint32_t nnz);
 g1_status g1_spmm_minplus_u32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz, uint32_t k,
                               const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
                               const uint32_t* values_dev, const uint32_t* x_dev, uint32_t* y_dev,
-                              const uint64_t* col_mask_dev, g1_stream_t stream);
+                              const uint64_t* col_mask_dev, void* workspace_dev, size_t workspace_bytes,
+                              g1_stream_t stream);
 g1_status g1_spmm_plustimes_f32(uint32_t n_rows, uint32_t n_cols, uint32_t nnz, uint32_t k,
                                 const uint32_t* row_offsets_dev, const uint32_t* col_indices_dev,
                                 const float* values_dev, const float* x_dev, float* y_dev,
-                                const uint64_t* col_mask_dev, g1_stream_t stream);
+                                const uint64_t* col_mask_dev, void* workspace_dev, size_t workspace_bytes,
+                                g1_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Host-side synthetic graphs (seeded, splitmix64; the random streams are spelled out at the top of
@@ -481,6 +486,14 @@ g1_status g1_server_create(const g1_server_config* config, g1_server** out);
 g1_status g1_server_destroy(g1_server* server);
 /* The graph stays owned by the caller and must outlive the server; it needs its transpose. */
 g1_status g1_server_add_graph(g1_server* server, g1_graph* graph, uint32_t* graph_id);
+/* Unregisters a resident graph: new submits on it fail with G1_ERR_NOT_FOUND; the call blocks until its
+ * queued and running queries reach a final state (their results stay fetchable), then stops the graph's
+ * threads, frees its views and returns its bytes to the budget. The graph itself stays the caller's. Not
+ * to be called concurrently with g1_server_destroy. */
+g1_status g1_server_remove_graph(g1_server* server, uint32_t graph_id);
+/* Sets the HBM budget (bytes > 0); a larger budget admits waiting queries at once, a smaller one holds new
+ * admissions until usage falls below it. */
+g1_status g1_server_set_budget(g1_server* server, uint64_t hbm_budget_bytes);
 g1_status g1_server_submit(g1_server* server, const g1_query* query, uint64_t* query_id);
 g1_status g1_server_poll(g1_server* server, uint64_t query_id, g1_query_status* status);
 /* Waits until the query is done, failed or cancelled, or timeout_ms passes (G1_ERR_BUSY then). */

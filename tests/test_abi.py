@@ -79,6 +79,14 @@ def test_server_argument_errors_without_a_gpu():
         srv.wait(12345, 1.0)
     s = srv.stats()
     assert s["hbm_budget"] == 1 << 20 and s["submitted"] == 0
+    with pytest.raises(g1.G1Error) as e:
+        srv.remove_graph(3)
+    assert e.value.status == _lib.G1_ERR_NOT_FOUND
+    with pytest.raises(g1.G1Error) as e:
+        srv.set_budget(0)
+    assert e.value.status == _lib.G1_ERR_INVALID_ARG
+    srv.set_budget(2 << 20)
+    assert srv.stats()["hbm_budget"] == 2 << 20
     srv.close()
     L = _lib.lib()
     assert L.g1_server_destroy(None) == _lib.G1_OK

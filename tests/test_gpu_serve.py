@@ -121,6 +121,51 @@ def test_budget_admission_waits_for_fetch(graphs):
     srv.close()
 
 
+def test_raising_the_budget_admits_waiting_queries(graphs):
+    """A budget with room for one result holds the other queries back; g1_server_set_budget raising it
+    admits them without any fetch, and every result is right."""
+    import graphone_amd as g1
+    h, g = graphs[0]
+    graph_bytes = g.device_bytes()
+    srv = g1.Server(hbm_budget_bytes=graph_bytes + h.n * 4 + 64, batch_min_sssp=1000, batch_min_ppr=1000)
+    gid = srv.add_graph(g)
+    qids = [srv.submit(gid, "sssp", s) for s in range(6)]
+    assert srv.wait(qids[0])["state_name"] == "done"
+    assert srv.poll(qids[1])["state_name"] == "queued"
+    srv.set_budget(graph_bytes + 16 * h.n * 4)
+    for i, q in enumerate(qids):
+        assert srv.wait(q, 60000)["state_name"] == "done"
+        assert np.array_equal(srv.fetch(q, "sssp"), _want(h, "sssp", i))
+    srv.close()
+
+
+def test_remove_graph(graphs):
+    """Removing a graph waits for its queries, keeps their results fetchable, returns its bytes to the
+    budget and refuses new queries on it; the other graph keeps serving."""
+    import graphone_amd as g1
+    (h0, g0), (h1, g1_) = graphs
+    srv = g1.Server(batch_min_sssp=1000, batch_min_ppr=1000)
+    bytes_a = g0.device_bytes()  # what add_graph reserves for it
+    a = srv.add_graph(g0)
+    b = srv.add_graph(g1_)
+    used = srv.stats()["hbm_used"]
+    qa = [srv.submit(a, "bfs", s) for s in range(5)]
+    srv.remove_graph(a)
+    for i, q in enumerate(qa):
+        assert srv.poll(q)["state_name"] == "done"
+        assert np.array_equal(srv.fetch(q, "bfs"), _want(h0, "bfs", i))
+    assert srv.stats()["hbm_used"] == used - bytes_a
+    with pytest.raises(g1.G1Error) as e:
+        srv.submit(a, "bfs", 0)
+    assert e.value.status == g1._lib.G1_ERR_NOT_FOUND
+    with pytest.raises(g1.G1Error):
+        srv.remove_graph(a)
+    q = srv.submit(b, "sssp", 3)
+    assert srv.wait(q)["state_name"] == "done"
+    assert np.array_equal(srv.fetch(q, "sssp"), _want(h1, "sssp", 3))
+    srv.close()
+
+
 def test_cancel_and_errors(graphs):
     import graphone_amd as g1
     h, g = graphs[0]

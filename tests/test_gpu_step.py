@@ -209,6 +209,39 @@ def test_apply_minplus_and_pagerank_apply(gpu):
         assert l1b.numpy()[0] == l1.numpy()[0]  # same fixed order, same bits
 
 
+def test_spmm_hub_row_spreads_over_tiles(gpu):
+    """One row with 200,000 nonzeros (a hub) beside short rows: the row spans ~100 merge tiles, so its
+    columns are folded across tiles by the fixup; min-plus exact, plus-times within 2e-6 of FP64."""
+    g1 = _g1()
+    rng = np.random.default_rng(17)
+    n_cols, k = 250_000, 64
+    degs = np.concatenate([[200_000], rng.integers(0, 5, 999)])
+    rp = np.zeros(degs.size + 1, np.int64)
+    rp[1:] = np.cumsum(degs)
+    ci = np.concatenate([np.sort(rng.choice(n_cols, d, replace=False)) for d in degs]).astype(np.uint32)
+    rp = rp.astype(np.uint32)
+    n_rows, m = degs.size, int(rp[-1])
+    w = rng.integers(1, 50, m).astype(np.uint32)
+    X = rng.integers(0, 100_000, (n_cols, k)).astype(np.uint32)
+    drp, dci, dw, dX = _dev(rp), _dev(np.concatenate([ci, np.zeros(1, np.uint32)])), _dev(w), _dev(X)
+    dY = g1.DeviceArray(n_rows * k, np.uint32)
+    wsb = g1._lib.lib().g1_spmm_workspace_bytes(n_rows, m)
+    ws = g1.DeviceArray((wsb + 3) // 4, np.uint32)
+    _call("g1_spmm_minplus_u32", n_rows, n_cols, m, k, _p(drp), _p(dci), _p(dw), _p(dX), _p(dY), None, _p(ws),
+          C.c_size_t(wsb), None)
+    Y = dY.numpy().reshape(n_rows, k)
+    for j in (0, 31, 63):
+        assert np.array_equal(Y[:, j], oracle.spmv_minplus_u32(rp, ci, w, X[:, j])), j
+    Xf = rng.random((n_cols, k), dtype=np.float32)
+    dXf, dYf, dwf = _dev(Xf), g1.DeviceArray(n_rows * k, np.float32), _dev(w.astype(np.float32))
+    _call("g1_spmm_plustimes_f32", n_rows, n_cols, m, k, _p(drp), _p(dci), _p(dwf), _p(dXf), _p(dYf), None,
+          _p(ws), C.c_size_t(wsb), None)
+    Yf = dYf.numpy().reshape(n_rows, k).astype(np.float64)
+    for j in (0, 63):
+        want = oracle.spmv_plustimes_f64(rp, ci, w.astype(np.float64), Xf[:, j].astype(np.float64))
+        assert np.allclose(Yf[:, j], want, rtol=2e-6, atol=1e-30), j
+
+
 @pytest.mark.parametrize("k", [1, 7, 64, 100])
 def test_spmm_minplus_and_plustimes(gpu, k):
     g1 = _g1()
@@ -227,7 +260,10 @@ def test_spmm_minplus_and_plustimes(gpu, k):
         mask[j // 64] |= np.uint64(1) << np.uint64(j % 64)
     Y0 = rng.integers(0, 9, (n, k)).astype(np.uint32)
     dX, dY, dm = _dev(X), _dev(Y0), _dev(mask)
-    _call("g1_spmm_minplus_u32", n, n, m, k, _p(trp), _p(tci), _p(tw), _p(dX), _p(dY), _p(dm), None)
+    wsb = g1._lib.lib().g1_spmm_workspace_bytes(n, m)
+    ws = g1.DeviceArray((wsb + 3) // 4, np.uint32)
+    _call("g1_spmm_minplus_u32", n, n, m, k, _p(trp), _p(tci), _p(tw), _p(dX), _p(dY), _p(dm), _p(ws),
+          C.c_size_t(wsb), None)
     Y = dY.numpy().reshape(n, k)
     for j in range(k):
         want = oracle.spmv_minplus_u32(t.row_offsets, t.col_indices, t.weights, X[:, j]) if on[j] else Y0[:, j]
@@ -236,7 +272,8 @@ def test_spmm_minplus_and_plustimes(gpu, k):
     Xf = rng.random((n, k), dtype=np.float32)
     wf = t.weights.astype(np.float32)
     dXf, dYf, dwf = _dev(Xf), g1.DeviceArray(n * k, np.float32), _dev(np.concatenate([wf, np.zeros(1, np.float32)]))
-    _call("g1_spmm_plustimes_f32", n, n, m, k, _p(trp), _p(tci), _p(dwf), _p(dXf), _p(dYf), None, None)
+    _call("g1_spmm_plustimes_f32", n, n, m, k, _p(trp), _p(tci), _p(dwf), _p(dXf), _p(dYf), None, _p(ws),
+          C.c_size_t(wsb), None)
     Yf = dYf.numpy().reshape(n, k).astype(np.float64)
     for j in range(0, k, max(1, k // 8)):
         want = oracle.spmv_plustimes_f64(t.row_offsets, t.col_indices, wf.astype(np.float64), Xf[:, j].astype(np.float64))
