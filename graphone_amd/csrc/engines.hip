@@ -117,6 +117,18 @@ int sssp_sweep_mode() {
 }
 bool sssp_sweep_enabled() { return sssp_sweep_mode() != 0; }
 
+// The SSSP push engines check a one-byte distance bound before reading a target's word (DistBound),
+// unless G1_SSSP_DBOUND=0. Step: the largest weight / 64, rounded up (bounds cover distances up to
+// 254 steps, about four times the largest weight; a longer distance keeps "no bound").
+DistBound sssp_dist_bound(const g1_graph* g) {
+    static const bool on = [] {
+        const char* e = getenv("G1_SSSP_DBOUND");
+        return !(e && atoi(e) == 0);
+    }();
+    const uint32_t step = std::max<uint32_t>(1u, (g->max_w + 63u) / 64u);
+    return DistBound{on ? g->dbound : nullptr, step};
+}
+
 // Workgroups of a sweep launch: 1024 (four per CU), or G1_SSSP_SWEEP_BLOCKS (8..8192, rounded down to a
 // multiple of 8: the owned sweep splits the grid by XCD).
 uint32_t sssp_sweep_blocks() {
@@ -382,7 +394,7 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
     const uint32_t n = g->n;
     const uint32_t words = (n + 31) / 32;
     push_start_kernel<false><<<grid_for(n, 256, kMaxPushBlocks), 256, 0, s>>>(
-        n, src, nullptr, depth, diropt ? g->fb3 : nullptr, words, g->row_off, g->q[0], g->qoff[0], g->qcnt, g->ctr);
+        n, src, nullptr, depth, diropt ? g->fb3 : nullptr, words, g->row_off, g->q[0], g->qoff[0], g->qcnt, g->ctr, nullptr);
     G1_LAUNCHED();
     memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
     G1_HIP(hipEventRecord(g->ev[0], s));
@@ -657,8 +669,9 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     if (sssp_sweep_enabled()) G1_TRY(ensure_sweep_pieces(g, s));
     const uint32_t n = g->n;
     const uint32_t fill_blocks = grid_for(n, 256, kMaxPushBlocks);
+    const DistBound db = sssp_dist_bound(g);
     push_start_kernel<true><<<fill_blocks, 256, 0, s>>>(n, src, g->dstamp, nullptr, nullptr, 0, g->row_off, g->q[0],
-                                                        g->qoff[0], g->qcnt, g->ctr);
+                                                        g->qoff[0], g->qcnt, g->ctr, db.b);
     G1_LAUNCHED();
     memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
     G1_HIP(hipEventRecord(g->ev[0], s));
@@ -690,7 +703,7 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
                         n, g->row_off, g->col, g->w, g->dstamp, it, fbits,
                         static_cast<const SweepPiece*>(g->sweep_pieces), g->n_sweep_pieces, pq, g->hist);
             } else {
-                SsspPushNoReturn pol{g->dstamp, it + 1};
+                SsspPushNoReturn pol{g->dstamp, it + 1, db};
                 tok = prof_begin(g, "sssp_advance_nr", s);
                 push_advance_kernel<SsspPushNoReturn><<<sssp_noreturn_blocks(), kPushThreads, 0, s>>>(
                     g->row_off, g->col, g->w, pq, pol, it, g->hist);
@@ -702,7 +715,7 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
             G1_LAUNCHED();
             prof_end(g, tok_c, s, 0.0);
         } else {
-            SsspPush pol{g->dstamp, it + 1};
+            SsspPush pol{g->dstamp, it + 1, db};
             tok = prof_begin(g, "sssp_advance", s);
             push_advance_kernel<SsspPush><<<sssp_push_blocks(), kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq,
                                                                                      pol, it, g->hist);
@@ -732,7 +745,7 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
         return G1_OK;
     };
     auto small = [&](uint32_t it0, uint32_t* res) -> g1_status {
-        SsspPush pol{g->dstamp, 0};
+        SsspPush pol{g->dstamp, 0, db};
         return run_small(g, s, pol, g->w, nullptr, 0, it0, res);
     };
     // A dense frontier (G1_SSSP_PULL x m edges, 0.8 by default) runs as a pull step (sssp_pull_x_kernel):
@@ -747,7 +760,7 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
                                                                                            pq, g->hist);
         G1_LAUNCHED();
         G1_TRY(launch_spmv_merge<MinPlusU32>(n, g->m, g->t_row_off, g->t_col, g->t_w, g->xu, g->yu, g->t_ws, s));
-        sssp_pull_apply_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->yu, g->dstamp, it + 1, pq);
+        sssp_pull_apply_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->yu, g->dstamp, it + 1, pq, db);
         G1_LAUNCHED();
         // x 12n + SpMV 4(n+1) + 8m + 4n (x) + 4n (y) + apply 4n (y) + 16n (dist word) + 8n (degrees)
         prof_end(g, tok, s, 4.0 * (n + 1.0) + 8.0 * g->m + 44.0 * n);

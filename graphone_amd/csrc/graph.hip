@@ -50,6 +50,7 @@ g1_status ensure_push_work(g1_graph* g, hipStream_t s) {
     G1_TRY(alloc_n(g, &g->stamp, n));
     G1_TRY(alloc_n(g, &g->fb3, 3 * words));
     G1_TRY(alloc_n(g, &g->dstamp, n));
+    G1_TRY(alloc_n(g, &g->dbound, n + 256));
     G1_TRY(alloc_n(g, &g->hist, 2ull * kHistIters));
     G1_TRY(alloc_n(g, &g->qcnt, 3 * kSegments));
     G1_TRY(alloc_n(g, &g->slots, 4 * kMaxPushBlocks));  // four regions of per-workgroup stats

@@ -59,6 +59,7 @@ struct g1_graph {
     uint32_t* fb[2] = {nullptr, nullptr};
     uint32_t* fb3 = nullptr;                    // three rotating frontier bitmaps (device-driven BFS)
     unsigned long long* dstamp = nullptr;       // SSSP (dist << 32 | stamp) words
+    uint8_t* dbound = nullptr;                  // SSSP distance upper bounds, one byte per vertex (DistBound)
     void* sweep_pieces = nullptr;               // SweepPiece records of the long out-edge rows (lazy)
     uint32_t n_sweep_pieces = 0;
     unsigned long long* hist = nullptr;         // per-iteration history of the push engines

@@ -367,6 +367,79 @@ __global__ void apply_construct_kernel(const gather_t* __restrict__ y, uint32_t 
     }
 }
 
+// A whole advance step of a small frontier (at most kSmallIn entries, its edge count known) in one
+// workgroup of 1024 threads: the entries' degrees are scanned in LDS, each edge slot finds its entry with
+// a binary search over the LDS offsets, and the functor relaxes it (the push relaxation of FunctorPush).
+// The packed output counter is cleared here first, so the step is one launch.
+constexpr uint32_t kSmallIn = 2048;
+constexpr uint32_t kSmallThreads = 1024;
+
+template <typename Rel, typename vertex_t>
+__global__ void __launch_bounds__(kSmallThreads)
+small_advance_kernel(const vertex_t* __restrict__ f, uint32_t nin, uint32_t n_vertices,
+                     const uint32_t* __restrict__ row_off, Rel rel) {
+    __shared__ uint32_t s_off[kSmallIn + 1];
+    __shared__ uint32_t s_rs[kSmallIn];
+    __shared__ uint32_t s_src[kSmallIn];
+    __shared__ uint32_t s_red[kSmallThreads / g1::kWave];
+    if (threadIdx.x == 0) {
+        rel.packed[0] = 0ull;
+        *rel.overflow = 0u;
+    }
+    // degrees, two entries per thread, then the block scan of the pairs
+    uint32_t d[2], a[2], u[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t i = threadIdx.x * 2 + k;
+        d[k] = 0;
+        a[k] = 0;
+        u[k] = g1::kInf;
+        if (i < nin) {
+            u[k] = static_cast<uint32_t>(f[i]);
+            if (u[k] < n_vertices) {
+                a[k] = row_off[u[k]];
+                d[k] = row_off[u[k] + 1] - a[k];
+            }
+        }
+    }
+    uint32_t total = 0;
+    const uint32_t ex = g1::block_exclusive_sum<kSmallThreads>(d[0] + d[1], total, s_red);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t i = threadIdx.x * 2 + k;
+        if (i < kSmallIn) {
+            s_off[i] = ex + (k ? d[0] : 0u);
+            s_rs[i] = a[k];
+            s_src[i] = u[k];
+        }
+    }
+    if (threadIdx.x == 0) s_off[kSmallIn] = total;
+    __syncthreads();
+    // edge slots in rounds of kSweepUnroll x 1024; every thread of a wave takes part in each round
+    for (uint32_t base = 0; base < total; base += kSmallThreads * g1::kSweepUnroll) {
+        uint32_t e[g1::kSweepUnroll], src[g1::kSweepUnroll];
+        bool ok[g1::kSweepUnroll];
+#pragma unroll
+        for (int k = 0; k < g1::kSweepUnroll; ++k) {
+            const uint32_t slot = base + k * kSmallThreads + threadIdx.x;
+            ok[k] = slot < total;
+            e[k] = 0;
+            src[k] = 0;
+            if (ok[k]) {
+                // the last entry whose first slot is <= slot (entries without edges share offsets)
+                uint32_t lo = 0, hi = kSmallIn;
+                while (lo + 1 < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= slot) lo = mid; else hi = mid;
+                }
+                e[k] = s_rs[lo] + (slot - s_off[lo]);
+                src[k] = s_src[lo];
+            }
+        }
+        rel(e, src, ok);
+    }
+}
+
 // ------------------------------------------------------------------------------------------ AdvanceGC
 
 // AdvanceGC::Forward (advance_gc.h:180-244): ResizeOutput, Advance and Filter as one device-driven step.
@@ -421,6 +494,27 @@ public:
             fr.set_output_edges(0);
             return;
         }
+        // A small frontier whose edge count is known runs as one launch (small_advance_kernel).
+        if (fr.input_edges_valid() && nin <= kSmallIn) {
+            bool pull = false;
+            if constexpr (frontier_t::adaptive) {
+                const double f = frontier_t::pull_fraction();
+                pull = f > 0.0 && m > 0 && static_cast<double>(fr.input_edges()) >= f * static_cast<double>(m);
+            }
+            if (!pull) {
+                const uint32_t cap = std::max<uint32_t>(fr.input_edges(), 1u);
+                fr.reserve_output_capacity(cap);
+                FunctorPush<functor_t, gref_t, dstatus_t, HasFilter::value> rel{
+                    gref, this->d_status_, row_off, reinterpret_cast<const uint32_t*>(gref.col_indices),
+                    reinterpret_cast<uint32_t*>(fr.output().data()), cap, packed(), overflow()};
+                small_advance_kernel<decltype(rel), vertex_t><<<1, kSmallThreads>>>(fr.input().data(), nin, nv, row_off,
+                                                                                   rel);
+                archi::launch_check("small advance");
+                ++pushes_;
+                read_output();
+                return;
+            }
+        }
         // edge offsets of the input list: off_[0..nin], off_[nin] = E
         grow(off_, nin + 1);
         grow(scratch_, (nin + 1) / kScanBlock + 4);
@@ -457,7 +551,13 @@ public:
             archi::launch_check("advance");
             ++pushes_;
         }
-        // the one read of the iteration: (count, out-edges) of the output and the overflow flag
+        read_output();
+    }
+
+private:
+    // The one read of the iteration: (count, out-edges) of the output and the overflow flag.
+    void read_output() {
+        auto& fr = this->frontier_;
         unsigned long long h[2] = {0ull, 0ull};
         archi::memcpy<arch_t::cpu, arch_t::hip, unsigned long long>(h, ctr_.data(), 2);
         const unsigned long long pk = h[0];
@@ -466,7 +566,6 @@ public:
         fr.set_output_edges(static_cast<uint32_t>(pk >> 32));
     }
 
-private:
     // The pull step: the input as a bitmap, then functor_pull_kernel over the in-edge CSR (the graph's
     // transpose view, or one built here on the device at the first pull).
     void run_pull(const gref_t& gref, uint32_t nin, uint32_t nv, uint32_t m, vertex_t* out, uint32_t cap) {

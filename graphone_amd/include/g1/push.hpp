@@ -272,6 +272,25 @@ push_advance_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __rest
     advance_tiles(row_off, col, wts, pq, pol, s_vb, s_eb);
 }
 
+// Distance upper bounds for the SSSP target checks: one byte per vertex (b = ceil(d / step), 255 = no
+// bound known), 1/8 of the 8-byte distance words, so the array stays in each XCD's L2 where the words
+// cannot. A relaxation with nd >= b * step cannot improve its target (b * step >= d) and skips the random
+// read of the word; the others read the word as before. A byte is written, with a plain store, after every
+// improving atomic: racing stores may leave a larger bound than the final distance allows, never a smaller
+// one (each stored bound covers a distance the word has held or holds), so a stale byte only filters less.
+struct DistBound {
+    uint8_t* b;     // null: no filtering
+    uint32_t step;  // distance units per bound step
+    __device__ __forceinline__ bool may_improve(uint32_t byte, uint32_t nd) const {
+        return byte == 255u || nd < byte * step;
+    }
+    __device__ __forceinline__ void lower(uint32_t v, uint32_t nd) const {
+        if (!b) return;
+        const uint32_t q = (nd + step - 1) / step;
+        b[v] = static_cast<uint8_t>(q < 255u ? q : 255u);
+    }
+};
+
 // SSSP relaxation (apps/sssp_adv.cu:49-71): nd = d[u] + w; atomicMin(&d[v], nd) and keep v when
 // nd < old; the stamp makes v enter the next frontier once per iteration (the visited filter).
 // Distance and stamp share one 64-bit word (dist << 32 | stamp), so ONE atomicMin both lowers the
@@ -285,6 +304,7 @@ struct SsspPush {
     static constexpr bool kEnqueue = true;
     unsigned long long* dstamp;
     uint32_t iter;
+    DistBound db;
     __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
         return static_cast<uint32_t>(dstamp[u] >> 32);
     }
@@ -296,17 +316,30 @@ struct SsspPush {
     template <int N>
     __device__ __forceinline__ void relax_many(const uint32_t (&)[N], const uint32_t (&du)[N], const uint32_t (&v)[N],
                                                const uint32_t (&w)[N], const uint32_t (&)[N], bool (&won)[N]) const {
+        bool need[N];
+#pragma unroll
+        for (int r = 0; r < N; ++r) need[r] = v[r] != kInf;
+        if (db.b) {
+            uint32_t byte[N];
+#pragma unroll
+            for (int r = 0; r < N; ++r) byte[r] = need[r] ? db.b[v[r]] : 0u;
+#pragma unroll
+            for (int r = 0; r < N; ++r) need[r] = need[r] && db.may_improve(byte[r], du[r] + w[r]);
+        }
         uint32_t cur[N];
 #pragma unroll
-        for (int r = 0; r < N; ++r) cur[r] = v[r] != kInf ? static_cast<uint32_t>(dstamp[v[r]] >> 32) : 0u;
+        for (int r = 0; r < N; ++r) cur[r] = need[r] ? static_cast<uint32_t>(dstamp[v[r]] >> 32) : 0u;
 #pragma unroll
         for (int r = 0; r < N; ++r) {
             won[r] = false;
-            if (v[r] == kInf) continue;
+            if (!need[r]) continue;
             const uint32_t nd = du[r] + w[r];
             if (nd >= cur[r]) continue;
             const unsigned long long old = atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | iter);
-            won[r] = nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
+            if (nd < static_cast<uint32_t>(old >> 32)) {
+                db.lower(v[r], nd);
+                won[r] = static_cast<uint32_t>(old) < iter;
+            }
         }
     }
     // One slot (the small-frontier kernel): the same relaxation.
@@ -316,7 +349,7 @@ struct SsspPush {
         const unsigned long long old = atomicMin(&dstamp[v], (static_cast<unsigned long long>(nd) << 32) | iter);
         return nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
     }
-    __device__ __forceinline__ SsspPush at_iteration(uint32_t it) const { return SsspPush{dstamp, it + 1}; }
+    __device__ __forceinline__ SsspPush at_iteration(uint32_t it) const { return SsspPush{dstamp, it + 1, db}; }
     // The source's distance read past the CU's cache: a workgroup that stays resident across iterations
     // must see improvements made by its own atomics in earlier iterations.
     __device__ __forceinline__ uint32_t fresh_source_value(uint32_t u) const {
@@ -341,21 +374,36 @@ struct SsspPushNoReturn {
     static constexpr bool kEnqueue = false;
     unsigned long long* dstamp;
     uint32_t iter;
+    DistBound db;
     __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
         return static_cast<uint32_t>(dstamp[u] >> 32);
     }
+    // The bound bytes first (L2-resident), then the words of the slots they do not rule out, then the
+    // atomics of the improving slots, each followed by its bound byte.
     template <int N>
     __device__ __forceinline__ void relax_many(const uint32_t (&)[N], const uint32_t (&du)[N], const uint32_t (&v)[N],
                                                const uint32_t (&w)[N], const uint32_t (&)[N], bool (&won)[N]) const {
+        bool need[N];
+#pragma unroll
+        for (int r = 0; r < N; ++r) need[r] = v[r] != kInf;
+        if (db.b) {
+            uint32_t byte[N];
+#pragma unroll
+            for (int r = 0; r < N; ++r) byte[r] = need[r] ? db.b[v[r]] : 0u;
+#pragma unroll
+            for (int r = 0; r < N; ++r) need[r] = need[r] && db.may_improve(byte[r], du[r] + w[r]);
+        }
         uint32_t cur[N];
 #pragma unroll
-        for (int r = 0; r < N; ++r) cur[r] = v[r] != kInf ? static_cast<uint32_t>(dstamp[v[r]] >> 32) : 0u;
+        for (int r = 0; r < N; ++r) cur[r] = need[r] ? static_cast<uint32_t>(dstamp[v[r]] >> 32) : 0u;
 #pragma unroll
         for (int r = 0; r < N; ++r) {
             won[r] = false;
             const uint32_t nd = du[r] + w[r];
-            if (v[r] != kInf && nd < cur[r])
+            if (need[r] && nd < cur[r]) {
                 (void)atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | iter);
+                db.lower(v[r], nd);
+            }
         }
     }
 };
@@ -474,7 +522,7 @@ sssp_pull_x_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, ui
 // frontier (segment blockIdx % kSegments of the output queue). Each vertex has one owner, so no atomics.
 static __global__ void __launch_bounds__(kPushThreads)
 sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ y,
-                       unsigned long long* __restrict__ dstamp, uint32_t stamp, PushQueues pq) {
+                       unsigned long long* __restrict__ dstamp, uint32_t stamp, PushQueues pq, DistBound db) {
     __shared__ unsigned long long s_scan[kPushThreads / kWave];
     __shared__ unsigned long long s_base;
     __shared__ uint32_t s_red[kPushThreads / kWave];
@@ -502,6 +550,7 @@ sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const u
             cdeg[r] = 0;
             if (v < n && yv[r] < static_cast<uint32_t>(ds[r] >> 32)) {
                 dstamp[v] = (static_cast<unsigned long long>(yv[r]) << 32) | stamp;
+                db.lower(v, yv[r]);
                 ++activated;
                 const uint32_t dg = row_off[v + 1] - row_off[v];
                 if (dg > 0) {
@@ -680,12 +729,13 @@ template <bool kSssp>
 static __global__ void __launch_bounds__(256)
 push_start_kernel(uint32_t n, uint32_t src, unsigned long long* __restrict__ dstamp, uint32_t* __restrict__ depth,
                   uint32_t* __restrict__ fb, uint32_t words, const uint32_t* __restrict__ row_off, uint32_t* q,
-                  uint32_t* qoff, unsigned long long* cnt3, unsigned long long* ctr) {
+                  uint32_t* qoff, unsigned long long* cnt3, unsigned long long* ctr, uint8_t* __restrict__ dbound) {
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t v = t0; v < n; v += stride) {
         if (kSssp) dstamp[v] = (v == src) ? 0ull : (static_cast<unsigned long long>(kInf) << 32);
         else depth[v] = (v == src) ? 0u : kInf;
+        if (kSssp && dbound) dbound[v] = (v == src) ? 0u : 255u;
     }
     if (!kSssp && fb)
         for (uint32_t i = t0; i < 2 * words; i += stride) fb[i] = (i == (src >> 5)) ? (1u << (src & 31u)) : 0u;
