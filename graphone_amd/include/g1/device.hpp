@@ -81,6 +81,22 @@ __device__ __forceinline__ uint32_t block_count(bool pred, uint32_t* lds) {
     return c;
 }
 
+// Block-wide counts of threads with `a` set and with `b` set, packed as count(a) | count(b) << 16 (every
+// thread gets them). BLOCK <= 65535.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_count_pair(bool a, bool b, uint32_t* lds) {
+    constexpr int kWaves = BLOCK / kWave;
+    const uint32_t ca = static_cast<uint32_t>(__popcll(ballot(a)));
+    const uint32_t cb = static_cast<uint32_t>(__popcll(ballot(b)));
+    if (lane_id() == 0) lds[threadIdx.x / kWave] = ca | (cb << 16);
+    __syncthreads();
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) c += lds[w];
+    __syncthreads();
+    return c;
+}
+
 // Index of the last element of the ascending array a[0..len) that is <= key, found by one wave with
 // a 64-way search (upper_bound(key) - 1). Requires len >= 1 and a[0] <= key. Every lane of the wave
 // must call it; every lane gets the result.

@@ -154,41 +154,52 @@ __device__ __forceinline__ void advance_tiles(const uint32_t* __restrict__ row_o
     const uint32_t seg_out = blockIdx.x % kSegments;
     uint32_t activated = 0;
 
+    // The first queue entry of the next tile, when this tile's staging fixes it (kInf: search for it).
+    uint32_t next_i0 = kInf;
     for (uint32_t tile = t_beg; tile < t_end; ++tile) {
         const uint32_t e0 = tile * kPushTile;
         const uint32_t e1 = (E - e0 < static_cast<uint32_t>(kPushTile)) ? E : e0 + kPushTile;
-        if (threadIdx.x < kWave) {
-            const uint32_t s = search_last_le(s_eb, kSegments, e0);
-            const uint32_t* qo = pq.qoff_in + static_cast<size_t>(s) * pq.cap;
-            const uint32_t nseg = s_vb[s + 1] - s_vb[s];
-            const uint32_t loc = wave_search_last_le(qo, nseg, e0 - s_eb[s]);
-            if (threadIdx.x == 0) s_i0 = s_vb[s] + loc;
+        uint32_t i0 = next_i0;  // block-uniform
+        if (i0 == kInf) {
+            if (threadIdx.x < kWave) {
+                const uint32_t s = search_last_le(s_eb, kSegments, e0);
+                const uint32_t* qo = pq.qoff_in + static_cast<size_t>(s) * pq.cap;
+                const uint32_t nseg = s_vb[s + 1] - s_vb[s];
+                const uint32_t loc = wave_search_last_le(qo, nseg, e0 - s_eb[s]);
+                if (threadIdx.x == 0) s_i0 = s_vb[s] + loc;
+            }
+            __syncthreads();
+            i0 = s_i0;
         }
-        __syncthreads();
-        const uint32_t i0 = s_i0;
-        // Stage the tile's sources: entries i0, i0+1, ... whose first edge slot is below e1.
+        // Stage the tile's sources: entries i0, i0+1, ... whose first edge slot is below e1. The entry whose
+        // first slot is e1 exactly, if one is read, starts the next tile; otherwise the next tile starts with
+        // this tile's last entry (its edges run on past e1). The entry id is loaded with its offset.
         uint32_t cnt = 0;
+        bool exact = false;
         for (uint32_t base = 0; base < static_cast<uint32_t>(kPushTile); base += kPushThreads) {
             const uint32_t k = base + threadIdx.x;
             const uint32_t i = i0 + k;
-            bool valid = false;
+            bool valid = false, at_e1 = false;
             if (i < nf) {
                 const uint32_t s = search_last_le(s_vb, kSegments, i);
                 const size_t at = static_cast<size_t>(s) * pq.cap + (i - s_vb[s]);
                 const uint32_t qo = s_eb[s] + pq.qoff_in[at];
+                const uint32_t u = pq.q_in[at];
                 valid = (k == 0) || (qo < e1);
+                at_e1 = (k != 0) && (qo == e1);
                 if (valid) {
-                    const uint32_t u = pq.q_in[at];
                     s_qoff[k] = qo;
                     s_rs[k] = row_off[u];
                     s_su[k] = pol.source_value(u);
                 }
             }
-            const uint32_t c = block_count<kPushThreads>(valid, s_red);
-            cnt += c;
-            if (c < static_cast<uint32_t>(kPushThreads)) break;
+            const uint32_t c = block_count_pair<kPushThreads>(valid, at_e1, s_red);
+            cnt += c & 0xffffu;
+            exact = exact || (c >> 16) != 0u;
+            if ((c & 0xffffu) < static_cast<uint32_t>(kPushThreads)) break;
         }
-        // (block_count ends with a barrier, so the staged sources are visible here)
+        next_i0 = (cnt < static_cast<uint32_t>(kPushTile)) ? i0 + cnt - (exact ? 0u : 1u) : kInf;
+        // (block_count_pair ends with a barrier, so the staged sources are visible here)
         // Phase 1: map every edge slot of this thread to its source and load (v, w); phase 2: let the
         // policy relax all of them (it issues its gathers and atomics before consuming any result);
         // phase 3: degrees of the winners. Each phase keeps kPushEpt independent accesses in flight.
