@@ -3,7 +3,10 @@
 Workload (default): RMAT scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05, directed, self loops and
 duplicates removed, u32 weights uniform in [1, 255], source = the vertex of highest out-degree (lowest id
 on ties). One step = one single-source BFS (direction-optimizing push/pull engine) plus one single-source
-SSSP (push advance/filter engine) from that source, inputs resident in HBM.
+SSSP from that source, inputs resident in HBM. The SSSP runs on the delta-stepping engine (light/heavy
+buckets of the weights over the advance/filter kernels, then the advance loop; --sssp-engine advance
+times the reference's plain frontier loop instead); the same step over the other push engine is timed
+beside it (`other_sssp_engine`), with its own roofline, and both give the same distances.
 
 value = GTEPS of the step = (reached edges of the BFS + reached edges of the SSSP) / step time, where
 reached edges = sum of the out-degrees of the reached vertices (Graph500 style). The same line carries
@@ -631,6 +634,8 @@ def main() -> int:
     ap.add_argument("--serve-rates", default="100,1000,3000,10000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-max-queries", type=int, default=600)
+    ap.add_argument("--sssp-engine", choices=["advance", "delta"], default="delta",
+                    help="SSSP engine of the timed step (the other push engine is timed beside it)")
     ap.add_argument("--only", choices=["grid213", "serving"], default=None,
                     help="run one secondary workload alone and print its JSON (for profiling runs); the "
                          "benchmark line is not printed")
@@ -664,23 +669,26 @@ def main() -> int:
     depth = g1.DeviceArray(host.n, np.uint32)
     dist = g1.DeviceArray(host.n, np.uint32)
 
-    def step():
+    def step(sssp_engine):
         b = g.bfs(src, "diropt", out=depth, to_host=False)
-        s = g.sssp(src, "advance", out=dist, to_host=False)
+        s = g.sssp(src, sssp_engine, out=dist, to_host=False)
         return b.stats, s.stats
 
-    for _ in range(args.warmup):
-        step()
-    g1.synchronize()
-    t0 = time.perf_counter()
-    bstats, sstats = [], []
-    for _ in range(args.steps):
-        b, s = step()
-        bstats.append(b)
-        sstats.append(s)
-    g1.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1e3 / args.steps
+    def timed(sssp_engine):
+        for _ in range(args.warmup):
+            step(sssp_engine)
+        g1.synchronize()
+        t0 = time.perf_counter()
+        bstats, sstats = [], []
+        for _ in range(args.steps):
+            b, s = step(sssp_engine)
+            bstats.append(b)
+            sstats.append(s)
+        g1.synchronize()
+        elapsed = time.perf_counter() - t0
+        return elapsed * 1e3 / args.steps, bstats, sstats
+
+    ms_per_step, bstats, sstats = timed(args.sssp_engine)
     m_reach = bstats[0]["reached_edges"]
     assert sstats[0]["reached_edges"] == m_reach
     value = 2.0 * m_reach / (ms_per_step * 1e-3) / 1e9
@@ -690,11 +698,27 @@ def main() -> int:
     gpu_depth, gpu_dist = depth.numpy(), dist.numpy()  # the last timed step's results (checked below)
 
     # live per-kernel timing (HIP events on the run's stream) for the roofline of the dominant kernel
-    g.profile(True)
-    step()
-    prof = g.read_profile()
-    g.profile(False)
-    roof = roofline_from_profile(prof, args.scale)
+    def profiled(sssp_engine):
+        g.profile(True)
+        step(sssp_engine)
+        p = g.read_profile()
+        g.profile(False)
+        return p
+
+    roof = roofline_from_profile(profiled(args.sssp_engine), args.scale)
+
+    # the other push engine over the same step (the reference's frontier loop when the headline is delta)
+    other = "advance" if args.sssp_engine == "delta" else "delta"
+    o_ms, o_b, o_s = timed(other)
+    o_sssp_ms = float(np.mean([x["elapsed_ms"] for x in o_s]))
+    other_step = {
+        "sssp_engine": other, "ms_per_step": round(o_ms, 4),
+        "value": round(2.0 * m_reach / (o_ms * 1e-3) / 1e9, 4), "sssp_ms": round(o_sssp_ms, 4),
+        "sssp_iterations": o_s[0]["iterations"], "relaxations": o_s[0]["edges_sum"],
+        "same_distances": bool(np.array_equal(dist.numpy(), gpu_dist)),
+        "roofline": roofline_from_profile(profiled(other), args.scale),
+    }
+    log(f"{other} step {o_ms:.3f} ms (sssp {o_sssp_ms:.3f} ms), {other_step['value']:.3f} GTEPS")
 
     extras = {}
     if not args.no_extras:
@@ -770,7 +794,9 @@ def main() -> int:
             "workload": f"rmat{args.scale}_bfs_sssp_single_source",
             "scale": args.scale, "edge_factor": args.edge_factor, "abcd": [0.57, 0.19, 0.19, 0.05],
             "weights": "u32 uniform [1,255]", "seed": args.seed, "n": host.n, "m": host.m, "source": src,
-            "engines": {"bfs": "diropt (push/pull switching)", "sssp": "advance (push)"},
+            "engines": {"bfs": "diropt (push/pull switching)",
+                        "sssp": ("delta (light/heavy buckets over the advance/filter kernels, then the advance loop)"
+                                 if args.sssp_engine == "delta" else "advance (push)")},
             "device": info["arch"], "cus": info["cus"],
         },
         "bfs": {"ms": round(bfs_ms, 4), "gteps": round(m_reach / bfs_ms / 1e6, 3),
@@ -781,6 +807,7 @@ def main() -> int:
                  "relax_per_s_G": round(sstats[0]["edges_sum"] / sssp_ms / 1e6, 3)},
         "m_reach": m_reach,
         "roofline": roof,
+        "other_sssp_engine": other_step,
         "cpu_baseline": cpu,
         "bit_exact": None if cpu is None else all(v for v in cpu["bit_exact"].values() if v is not None),
         "extras": extras,

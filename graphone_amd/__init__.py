@@ -19,8 +19,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (G1_BATCH_MINPLUS, G1_BATCH_PPR, G1_ENGINE_ADVANCE, G1_ENGINE_BLAS_CSRVEC, G1_ENGINE_BLAS_MERGE,
-                   G1_ENGINE_DIROPT, G1_INF, G1_PR_L1_F64, G1_PR_MAX_ABS, BatchStats, G1Error, HostCSR, RunStats,
-                   call, lib)
+                   G1_ENGINE_DELTA, G1_ENGINE_DIROPT, G1_INF, G1_PR_L1_F64, G1_PR_MAX_ABS, BatchStats, G1Error,
+                   HostCSR, RunStats, call, lib)
 
 INF = G1_INF
 
@@ -29,6 +29,7 @@ ENGINES = {
     "merge": G1_ENGINE_BLAS_MERGE,
     "csrvec": G1_ENGINE_BLAS_CSRVEC,
     "diropt": G1_ENGINE_DIROPT,
+    "delta": G1_ENGINE_DELTA,
 }
 
 

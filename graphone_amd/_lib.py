@@ -29,6 +29,7 @@ G1_ENGINE_ADVANCE = 0
 G1_ENGINE_BLAS_MERGE = 1
 G1_ENGINE_BLAS_CSRVEC = 2
 G1_ENGINE_DIROPT = 3
+G1_ENGINE_DELTA = 4
 
 G1_PR_L1_F64 = 0
 G1_PR_MAX_ABS = 1

@@ -186,15 +186,25 @@ double sssp_pull_fraction() {
 
 // One wave sums the kSegments counters of a frontier and posts nf and E to the graph's coherent pinned
 // mailbox, then the sequence number with a system-scope release store, so the host learns a frontier's
-// size by watching host memory instead of a device-to-host copy plus a stream synchronization.
+// size by watching host memory instead of a device-to-host copy plus a stream synchronization. With
+// mins (nmins per-workgroup minima of a distance scan) their minimum is posted too.
 __global__ void __launch_bounds__(64) frontier_post_kernel(const unsigned long long* __restrict__ cnt,
-                                                           unsigned long long* mbox, unsigned long long seq) {
+                                                           unsigned long long* mbox, unsigned long long seq,
+                                                           const uint32_t* __restrict__ mins, uint32_t nmins) {
     const unsigned long long c = cnt[lane_id()];
     const unsigned long long nf = wave_sum<unsigned long long>(c & 0xFFFFFFFFull);
     const unsigned long long E = wave_sum<unsigned long long>(c >> 32);
+    uint32_t mn = kInf;
+    for (uint32_t i = lane_id(); i < nmins; i += kWave) mn = mins[i] < mn ? mins[i] : mn;
+#pragma unroll
+    for (int off = kWave / 2; off >= 1; off >>= 1) {
+        const uint32_t o = __shfl_down(mn, off, kWave);
+        mn = o < mn ? o : mn;
+    }
     if (lane_id() == 0) {
         __hip_atomic_store(mbox + 1, nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(mbox + 2, E, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(mbox + 3, static_cast<unsigned long long>(mn), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(mbox, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
@@ -204,9 +214,10 @@ __global__ void __launch_bounds__(64) frontier_post_kernel(const unsigned long l
 // iteration runs for hundreds of microseconds, and the query server polls from many threads at once);
 // every 1024 polls it asks the stream, and a stream that has drained (or failed) without the post showing
 // is an error.
-g1_status read_frontier(g1_graph* g, uint32_t set, hipStream_t s, unsigned long long* packed) {
+g1_status read_frontier(g1_graph* g, uint32_t set, hipStream_t s, unsigned long long* packed,
+                        const uint32_t* mins = nullptr, uint32_t nmins = 0, uint32_t* min_out = nullptr) {
     const unsigned long long seq = ++g->post_seq;
-    frontier_post_kernel<<<1, 64, 0, s>>>(g->qcnt + set * kSegments, g->h_mbox, seq);
+    frontier_post_kernel<<<1, 64, 0, s>>>(g->qcnt + set * kSegments, g->h_mbox, seq, mins, nmins);
     G1_LAUNCHED();
     uint32_t polls = 0;
     while (__atomic_load_n(g->h_mbox, __ATOMIC_ACQUIRE) != seq) {
@@ -228,6 +239,7 @@ g1_status read_frontier(g1_graph* g, uint32_t set, hipStream_t s, unsigned long 
     const unsigned long long E = __atomic_load_n(g->h_mbox + 2, __ATOMIC_ACQUIRE);
     G1_CHECK(E < 0xFFFFFFFFull && nf <= 0xFFFFFFFFull, G1_ERR_INTERNAL, "frontier edge count overflow");
     *packed = (E << 32) | nf;
+    if (min_out) *min_out = static_cast<uint32_t>(__atomic_load_n(g->h_mbox + 3, __ATOMIC_ACQUIRE));
     return G1_OK;
 }
 
@@ -262,15 +274,19 @@ uint32_t tail_batch(uint64_t edges) {
 //
 // With pull_min_edges > 0 an iteration whose frontier has at least that many out-edges runs as one pull
 // step instead (pull(it): scatter, SpMV, apply). The host then needs each frontier's edge count before
-// its iteration, so until the frontier shrinks below a quarter of that size iterations go one at a time.
+// its iteration, so until the frontier shrinks below a quarter of that size iterations go one at a time;
+// single_min_edges (when not 0) sets that size instead, for a launch whose kind depends on its edge count.
+// it0 is the first iteration; first_edges, when known, the edge count of its frontier.
 template <typename Launch, typename AfterLaunch, typename Small, typename Pull>
 g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& after, Small&& small,
-                     bool small_first, uint32_t* launched, Pull&& pull, uint64_t pull_min_edges) {
-    uint32_t it = 0, batch = 2;
+                     bool small_first, uint32_t* launched, Pull&& pull, uint64_t pull_min_edges, uint32_t it0 = 0,
+                     uint64_t first_edges = ~0ull, uint64_t single_min_edges = 0) {
+    if (single_min_edges == 0) single_min_edges = pull_min_edges;
+    uint32_t it = it0, batch = 2;
     bool try_small = small_first;
-    bool known = false;      // next_edges holds the edge count of the frontier of iteration `it`
-    bool shrinking = false;  // ... and it is below the count read before it
-    uint64_t next_edges = 0;
+    bool known = first_edges != ~0ull;  // next_edges holds the edge count of the frontier of iteration `it`
+    bool shrinking = false;             // ... and it is below the count read before it
+    uint64_t next_edges = known ? first_edges : 0;
     for (;;) {
         // A frontier whose out-edges fit the single-workgroup kernel runs there (see push_small_kernel)
         // until it grows past it or ends; profiling keeps one launch per iteration instead.
@@ -296,7 +312,7 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
         } else {
             uint32_t b = shrinking ? tail_batch(next_edges) : batch;
             if (g->profile) b = 1;
-            if (pull_min_edges && (!known || !shrinking || next_edges >= pull_min_edges / 4)) b = 1;
+            if (single_min_edges && (!known || !shrinking || next_edges >= single_min_edges / 4)) b = 1;
             G1_CHECK(it + b <= kHistIters, G1_ERR_INTERNAL, "push engine exceeded %u iterations", kHistIters);
             for (uint32_t k = 0; k < b; ++k) {
                 G1_TRY(launch(it, k == 0 && known ? next_edges : 0));
@@ -325,11 +341,13 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
 
 // Runs push_small_kernel from iteration it0 and reads its result: res[0] the next iteration, res[1] 1
 // if the frontier was handed back to the segmented queue (0: the run is over).
-template <typename P>
+template <typename P, typename RV = FullRows>
 g1_status run_small(g1_graph* g, hipStream_t s, const P& pol, const uint32_t* wts, uint32_t* fb, uint32_t words,
-                    uint32_t it0, uint32_t* res) {
+                    uint32_t it0, uint32_t* res, RV rv = FullRows(nullptr), const uint32_t* col = nullptr) {
     uint32_t* dres = reinterpret_cast<uint32_t*>(g->ctr + 10);
-    push_small_kernel<P><<<1, kSmallThreads, 0, s>>>(g->row_off, g->col, wts, pol, g->q[0], g->q[1], g->qoff[0],
+    if constexpr (std::is_same<RV, FullRows>::value) rv = FullRows(g->row_off);
+    if (!col) col = g->col;
+    push_small_kernel<P, RV><<<1, kSmallThreads, 0, s>>>(rv, col, wts, pol, g->q[0], g->q[1], g->qoff[0],
                                                       g->qoff[1], g->qcnt, g->seg_cap, it0, kHistIters - 1, fb,
                                                       words, g->hist, g->slots, dres);
     G1_LAUNCHED();
@@ -346,7 +364,7 @@ g1_status run_small(g1_graph* g, hipStream_t s, const P& pol, const uint32_t* wt
 // push_finish_kernel folds every slot region and writes the counters, the overflow flag and the
 // iteration history to pinned memory. The run's end event must already be recorded.
 g1_status collect_push_run(g1_graph* g, hipStream_t s, uint32_t launched, const uint32_t* d, g1_run_stats* st,
-                           bool count_edges) {
+                           bool count_edges, bool skip_empty = false) {
     reached_stats_kernel<<<grid_for(g->n, 256, kMaxPushBlocks), 256, 0, s>>>(
         g->n, d, g->row_off, g->slots + 2 * kMaxPushBlocks, g->slots + 3 * kMaxPushBlocks);
     G1_LAUNCHED();
@@ -367,7 +385,10 @@ g1_status collect_push_run(g1_graph* g, hipStream_t s, uint32_t launched, const 
     uint32_t iters = 0;
     for (uint32_t i = 0; i < launched; ++i) {
         const unsigned long long* h = g->h_hist + 2ull * i;
-        if (static_cast<uint32_t>(h[0]) == 0) break;
+        if (static_cast<uint32_t>(h[0]) == 0) {
+            if (skip_empty) continue;  // a no-op launch between the delta engine's phases
+            break;
+        }
         ++iters;
         const bool pull = (h[1] >> 63) != 0;
         if (pull) ++st->pull_iterations; else ++st->push_iterations;
@@ -394,7 +415,8 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
     const uint32_t n = g->n;
     const uint32_t words = (n + 31) / 32;
     push_start_kernel<false><<<grid_for(n, 256, kMaxPushBlocks), 256, 0, s>>>(
-        n, src, nullptr, depth, diropt ? g->fb3 : nullptr, words, g->row_off, g->q[0], g->qoff[0], g->qcnt, g->ctr, nullptr);
+        n, src, nullptr, depth, diropt ? g->fb3 : nullptr, words, g->row_off, nullptr, g->q[0], g->qoff[0], g->qcnt,
+        g->ctr, nullptr);
     G1_LAUNCHED();
     memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
     G1_HIP(hipEventRecord(g->ev[0], s));
@@ -662,19 +684,14 @@ g1_status run_bfs_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats*
     return finish_stats(g, depth, st, s);
 }
 
-g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s, bool allow_road) {
-    if (allow_road && road_graph(g) && g->w) return run_sssp_road(g, src, dist, st, s);
-    G1_TRY(ensure_push_work(g, s));
-    G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
-    if (sssp_sweep_enabled()) G1_TRY(ensure_sweep_pieces(g, s));
+// The push iterations of the SSSP advance engine from iteration it0, whose input frontier is in the
+// segmented queue (counter set it0 % 3; with a pull step possible at it0, its vertices carry stamp it0):
+// returning and no-return advance launches, the single-workgroup kernel for small frontiers, and pull
+// steps for dense ones, driven by drive_push. *launched gets the next iteration index; no-return and pull
+// iterations are counted into *dense and *pulls.
+g1_status sssp_push_iterations(g1_graph* g, hipStream_t s, const DistBound& db, uint32_t it0, bool small_first,
+                               uint32_t* launched, uint32_t* dense, uint32_t* pulls, uint64_t first_edges = ~0ull) {
     const uint32_t n = g->n;
-    const uint32_t fill_blocks = grid_for(n, 256, kMaxPushBlocks);
-    const DistBound db = sssp_dist_bound(g);
-    push_start_kernel<true><<<fill_blocks, 256, 0, s>>>(n, src, g->dstamp, nullptr, nullptr, 0, g->row_off, g->q[0],
-                                                        g->qoff[0], g->qcnt, g->ctr, db.b);
-    G1_LAUNCHED();
-    memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
-    G1_HIP(hipEventRecord(g->ev[0], s));
     int tok = -1;
     // A frontier known to hold at least G1_SSSP_NORETURN x m out-edges (0.3 by default) relaxes without
     // the winner test (SsspPushNoReturn) and its winners are collected from the stamps afterwards.
@@ -682,12 +699,11 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
         sssp_noreturn_fraction() > 0.0 ? std::max<uint64_t>(static_cast<uint64_t>(sssp_noreturn_fraction() * g->m), 1)
                                        : ~0ull;
     int tok_c = -1;  // the collect launch of a no-return iteration (profiling)
-    uint32_t dense = 0;
     auto launch = [&](uint32_t it, uint64_t edges) -> g1_status {
         PushQueues pq = make_queues(g, it);
         tok_c = -1;
         if (edges >= noreturn_min) {
-            ++dense;
+            ++*dense;
             if (sssp_sweep_enabled()) {
                 // the frontier's bitmap (g->fb[0] holds ceil(n / 64) 64-bit words), then the sweep
                 unsigned long long* fbits = reinterpret_cast<unsigned long long*>(g->fb[0]);
@@ -706,19 +722,19 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
                 SsspPushNoReturn pol{g->dstamp, it + 1, db};
                 tok = prof_begin(g, "sssp_advance_nr", s);
                 push_advance_kernel<SsspPushNoReturn><<<sssp_noreturn_blocks(), kPushThreads, 0, s>>>(
-                    g->row_off, g->col, g->w, pq, pol, it, g->hist);
+                    FullRows(g->row_off), g->col, g->w, pq, pol, it, g->hist);
             }
             G1_LAUNCHED();
             prof_end(g, tok, s, 0.0);
             tok_c = prof_begin(g, "sssp_collect", s);
-            sssp_collect_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->dstamp, it + 1, pq);
+            sssp_collect_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, FullRows(g->row_off), g->dstamp, it + 1, pq);
             G1_LAUNCHED();
             prof_end(g, tok_c, s, 0.0);
         } else {
             SsspPush pol{g->dstamp, it + 1, db};
             tok = prof_begin(g, "sssp_advance", s);
-            push_advance_kernel<SsspPush><<<sssp_push_blocks(), kPushThreads, 0, s>>>(g->row_off, g->col, g->w, pq,
-                                                                                     pol, it, g->hist);
+            push_advance_kernel<SsspPush><<<sssp_push_blocks(), kPushThreads, 0, s>>>(FullRows(g->row_off), g->col, g->w,
+                                                                                     pq, pol, it, g->hist);
             G1_LAUNCHED();
             prof_end(g, tok, s, 0.0);
         }
@@ -752,7 +768,6 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
     // one streaming pass over the in-edge CSR costs less than that many random relaxations.
     const bool can_pull = g->has_t && g->t_w != nullptr && sssp_pull_fraction() > 0.0;
     if (can_pull) G1_TRY(ensure_pull_work(g, false));
-    uint32_t pulls = 0;
     auto pull = [&](uint32_t it) -> g1_status {
         PushQueues pq = make_queues(g, it);
         tok = prof_begin(g, "sssp_pull", s);
@@ -764,16 +779,255 @@ g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats*
         G1_LAUNCHED();
         // x 12n + SpMV 4(n+1) + 8m + 4n (x) + 4n (y) + apply 4n (y) + 16n (dist word) + 8n (degrees)
         prof_end(g, tok, s, 4.0 * (n + 1.0) + 8.0 * g->m + 44.0 * n);
-        ++pulls;
+        ++*pulls;
         return G1_OK;
     };
-    uint32_t launched = 0;
-    G1_TRY(drive_push(g, s, launch, after, small, g->max_out_deg <= kSmallCap, &launched, pull,
-                      can_pull ? std::max<uint64_t>(static_cast<uint64_t>(sssp_pull_fraction() * g->m), 1) : 0));
+    return drive_push(g, s, launch, after, small, small_first, launched, pull,
+                      can_pull ? std::max<uint64_t>(static_cast<uint64_t>(sssp_pull_fraction() * g->m), 1) : 0, it0,
+                      first_edges);
+}
+
+g1_status run_sssp_push(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s, bool allow_road) {
+    if (allow_road && road_graph(g) && g->w) return run_sssp_road(g, src, dist, st, s);
+    G1_TRY(ensure_push_work(g, s));
+    G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
+    if (sssp_sweep_enabled()) G1_TRY(ensure_sweep_pieces(g, s));
+    const uint32_t n = g->n;
+    const uint32_t fill_blocks = grid_for(n, 256, kMaxPushBlocks);
+    const DistBound db = sssp_dist_bound(g);
+    push_start_kernel<true><<<fill_blocks, 256, 0, s>>>(n, src, g->dstamp, nullptr, nullptr, 0, g->row_off, nullptr,
+                                                        g->q[0], g->qoff[0], g->qcnt, g->ctr, db.b);
+    G1_LAUNCHED();
+    memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
+    G1_HIP(hipEventRecord(g->ev[0], s));
+    uint32_t launched = 0, dense = 0, pulls = 0;
+    G1_TRY(sssp_push_iterations(g, s, db, 0, g->max_out_deg <= kSmallCap, &launched, &dense, &pulls));
     dstamp_extract_kernel<<<fill_blocks, 256, 0, s>>>(n, g->dstamp, dist);
     G1_LAUNCHED();
     G1_HIP(hipEventRecord(g->ev[1], s));
     G1_TRY(collect_push_run(g, s, launched, dist, st, true));
+    st->pull_iterations += pulls;
+    st->push_iterations -= std::min(st->push_iterations, pulls);
+    st->dense_iterations = dense;
+    st->frontier_sum = 1 + g->h_ctr[3];
+    return G1_OK;
+}
+
+// ---------------------------------------------------------------- delta-stepping SSSP
+
+// Bucket width of the delta-stepping engine: the largest weight / 16, rounded up (at least 1), or
+// G1_SSSP_DELTA. RMAT22 (weights 1..255) from the hub, 10 runs each: width 4 1.57 ms (104 M relaxations),
+// 6 1.34, 8 1.34, 12 1.33, 16 1.32 (68 M), 24 1.39, 32 1.47, 48 1.62, 64 1.74-1.80 ms (85 M); the advance
+// engine 2.18 ms (170 M). A simulation of the schedule from five sources gave 74-81 M relaxations at 32
+// against 208-327 M for the plain frontier loop.
+uint32_t sssp_delta_width(const g1_graph* g) {
+    const char* e = getenv("G1_SSSP_DELTA");
+    const long v = e ? atol(e) : 0;
+    if (v > 0 && v <= 0x7FFFFFFFL) return static_cast<uint32_t>(v);
+    return std::max<uint32_t>(1u, (g->max_w + 15u) / 16u);
+}
+
+// The delta engine leaves the buckets for plain push iterations once a heavy phase has relaxed at least
+// G1_SSSP_DELTA_SWITCH (default 0.01) times the graph's edges, or after kDeltaMaxBuckets buckets. Both
+// variables are read at every run.
+double sssp_delta_switch() {
+    const char* e = getenv("G1_SSSP_DELTA_SWITCH");
+    const double v = e ? atof(e) : 0.01;
+    return v >= 0.0 ? v : 0.01;
+}
+constexpr uint32_t kDeltaMaxBuckets = 64;
+
+// A light-phase iteration of at least G1_SSSP_DELTA_NORETURN (default 0.3) times the graph's edges relaxes
+// without the winner test and collects its winners afterwards (SsspPushNoReturn + sssp_collect_kernel).
+// Light frontiers are smaller than the advance engine's dense ones; lower thresholds measured slower
+// (RMAT22, width 16: 1.40 ms at 0.005, 1.37 at 0.02, 1.33 at 0.3), the host then reading every frontier.
+double sssp_delta_noreturn() {
+    const char* e = getenv("G1_SSSP_DELTA_NORETURN");
+    const double v = e ? atof(e) : 0.3;
+    return v >= 0.0 ? v : 0.3;
+}
+
+// Zeroes queue counter sets a and b (kSegments counters each).
+__global__ void __launch_bounds__(128) clear_sets_kernel(unsigned long long* qcnt, uint32_t a, uint32_t b) {
+    const uint32_t t = threadIdx.x;
+    if (t < kSegments) qcnt[a * kSegments + t] = 0ull;
+    else qcnt[b * kSegments + (t - kSegments)] = 0ull;
+}
+
+// SSSP by delta-stepping over the push engine's kernels (Meyer and Sanders' light/heavy buckets, each
+// phase an advance + filter over a frontier queue). Bucket b holds the distances in [lo, lo + delta).
+// Light phase: push iterations over the light parts of the rows (ensure_delta_split), the bucket's limit
+// keeping improvements past it out of the frontier (they carry stamp 0); when the frontier empties, every
+// distance in the bucket is final. Heavy phase: one relaxation of the heavy parts of the bucket's rows (a
+// pull step when they hold enough edges). Then the next non-empty bucket, both frontiers built by a scan
+// of the distances (dist_range_frontier_kernel), which also finds the next bucket. Once a heavy phase
+// relaxes G1_SSSP_DELTA_SWITCH x m edges (the frontier has reached the graph's bulk), the remaining
+// vertices with finite distances form one frontier for the plain push iterations (sssp_push_iterations),
+// which finish the run. The distances are those of any other engine: each bucket's vertices relax their
+// light edges until nothing in the bucket improves, so they are final (a shortest path's last edge into
+// the bucket from inside it is light) before their heavy edges are relaxed once.
+g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s) {
+    if (road_graph(g) && g->w) return run_sssp_road(g, src, dist, st, s);
+    G1_TRY(ensure_push_work(g, s));
+    G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
+    const uint32_t delta = sssp_delta_width(g);
+    G1_TRY(ensure_delta_split(g, delta, s));
+    if (sssp_sweep_enabled()) G1_TRY(ensure_sweep_pieces(g, s));
+    const uint32_t n = g->n;
+    const uint32_t fill_blocks = grid_for(n, 256, kMaxPushBlocks);
+    const DistBound db = sssp_dist_bound(g);
+    const LightRows lrows{g->row_off, g->dl_lc};
+    const HeavyRows hrows{g->row_off, g->dl_lc};
+    const bool can_pull = g->has_t && g->t_w != nullptr && sssp_pull_fraction() > 0.0;
+    if (can_pull) G1_TRY(ensure_pull_work(g, false));
+    const uint64_t pull_min =
+        can_pull ? std::max<uint64_t>(static_cast<uint64_t>(sssp_pull_fraction() * g->m), 1) : ~0ull;
+    const uint64_t noreturn_min =
+        sssp_delta_noreturn() > 0.0 ? std::max<uint64_t>(static_cast<uint64_t>(sssp_delta_noreturn() * g->m), 1)
+                                    : ~0ull;
+    const uint64_t switch_edges = static_cast<uint64_t>(sssp_delta_switch() * g->m);
+
+    push_start_kernel<true><<<fill_blocks, 256, 0, s>>>(n, src, g->dstamp, nullptr, nullptr, 0, g->row_off, g->dl_lc,
+                                                        g->q[0], g->qoff[0], g->qcnt, g->ctr, db.b);
+    G1_LAUNCHED();
+    memset(g->h_ctr, 0, 8 * sizeof(unsigned long long));  // the host mirror of the cleared counters
+    G1_HIP(hipEventRecord(g->ev[0], s));
+
+    uint32_t it = 0, dense = 0, pulls = 0, buckets = 0;
+    uint32_t limit = 0;
+    int tok = -1, tok_c = -1;
+    auto light_launch = [&](uint32_t i, uint64_t edges) -> g1_status {
+        PushQueues pq = make_queues(g, i);
+        tok_c = -1;
+        if (edges >= noreturn_min) {
+            ++dense;
+            SsspPushNoReturn pol{g->dstamp, i + 1, db, limit};
+            tok = prof_begin(g, "sssp_delta_light_nr", s);
+            push_advance_kernel<SsspPushNoReturn, LightRows><<<sssp_noreturn_blocks(), kPushThreads, 0, s>>>(
+                lrows, g->dl_col, g->dl_w, pq, pol, i, g->hist);
+            G1_LAUNCHED();
+            prof_end(g, tok, s, 0.0);
+            tok_c = prof_begin(g, "sssp_delta_collect", s);
+            sssp_collect_kernel<LightRows><<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, lrows, g->dstamp, i + 1, pq);
+            G1_LAUNCHED();
+            prof_end(g, tok_c, s, 0.0);
+        } else {
+            SsspPush pol{g->dstamp, i + 1, db, limit};
+            tok = prof_begin(g, "sssp_delta_light", s);
+            push_advance_kernel<SsspPush, LightRows><<<sssp_push_blocks(), kPushThreads, 0, s>>>(
+                lrows, g->dl_col, g->dl_w, pq, pol, i, g->hist);
+            G1_LAUNCHED();
+            prof_end(g, tok, s, 0.0);
+        }
+        return G1_OK;
+    };
+    // profiling: the launch's algorithmic bytes, counted as for the advance engine's launches
+    auto light_after = [&](uint32_t i) -> g1_status {
+        if (!g->profile) return G1_OK;
+        const uint64_t act0 = g->h_ctr[3];
+        G1_TRY(fold_slots(g, s, true));
+        unsigned long long h[1];
+        G1_HIP(hipMemcpy(h, g->hist + 2 * i, sizeof(h), hipMemcpyDeviceToHost));
+        const double nf = static_cast<double>(static_cast<uint32_t>(h[0]));
+        const double E = static_cast<double>(h[0] >> 32);
+        const double act = static_cast<double>(g->h_ctr[3] - act0);
+        if (tok_c >= 0) {
+            prof_set_bytes(g, tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E);
+            prof_set_bytes(g, tok_c, 8.0 * n + 16.0 * act);
+        } else {
+            prof_set_bytes(g, tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E + 4.0 * act);
+        }
+        return G1_OK;
+    };
+    auto light_small = [&](uint32_t it0, uint32_t* res) -> g1_status {
+        SsspPush pol{g->dstamp, 0, db, limit};
+        return run_small(g, s, pol, g->dl_w, nullptr, 0, it0, res, lrows, g->dl_col);
+    };
+    auto no_pull = [](uint32_t) -> g1_status { return G1_OK; };
+    // A distance scan that writes the input frontier of iteration `it` (the queues of iteration it - 1,
+    // taken mod 6 so both rotations agree) with stamp `it`, and reads its size and the smallest distance
+    // at or above lo.
+    auto scan = [&](auto rv, uint64_t lo, uint64_t hi, unsigned long long* fr, uint32_t* mn) -> g1_status {
+        clear_sets_kernel<<<1, 2 * kSegments, 0, s>>>(g->qcnt, it % 3, (it + 1) % 3);
+        G1_LAUNCHED();
+        tok = prof_begin(g, "sssp_delta_scan", s);
+        dist_range_frontier_kernel<decltype(rv)><<<kMaxPushBlocks, kPushThreads, 0, s>>>(
+            n, rv, g->dstamp, static_cast<uint32_t>(std::min<uint64_t>(lo, kInf)),
+            static_cast<uint32_t>(std::min<uint64_t>(hi, kInf)), it, make_queues(g, it + 5), g->dl_mins);
+        G1_LAUNCHED();
+        prof_end(g, tok, s, 8.0 * n);
+        return read_frontier(g, it % 3, s, fr, g->dl_mins, kMaxPushBlocks, mn);
+    };
+
+    uint64_t lo = 0;
+    bool small_first = true;         // the seed
+    uint64_t first_edges = ~0ull;    // the light frontier's edge count, when a scan read it
+    unsigned long long fr = 0;
+    uint32_t mn = kInf;
+    for (;;) {
+        limit = static_cast<uint32_t>(std::min<uint64_t>(lo + delta, kInf));
+        ++buckets;
+        uint32_t launched = it;
+        if (first_edges != 0)
+            G1_TRY(drive_push(g, s, light_launch, light_after, light_small, small_first, &launched, no_pull, 0, it,
+                              first_edges, noreturn_min == ~0ull ? 0 : noreturn_min));
+        it = launched;
+        G1_CHECK(it + 2 <= kHistIters, G1_ERR_INTERNAL, "delta engine exceeded %u iterations", kHistIters);
+        // heavy phase: the bucket's vertices with heavy edges
+        G1_TRY(scan(hrows, lo, limit, &fr, &mn));
+        const uint64_t heavy_edges = fr >> 32;
+        if (static_cast<uint32_t>(fr) != 0) {
+            PushQueues pq = make_queues(g, it);
+            if (heavy_edges >= pull_min) {
+                // x = the bucket's distances (the scan stamped them `it`), y = A^T (min,+) x, apply
+                tok = prof_begin(g, "sssp_pull", s);
+                sssp_pull_x_kernel<<<grid_for(n, kPushThreads, kMaxPushBlocks), kPushThreads, 0, s>>>(
+                    n, g->dstamp, it, g->xu, pq, g->hist);
+                G1_LAUNCHED();
+                G1_TRY(launch_spmv_merge<MinPlusU32>(n, g->m, g->t_row_off, g->t_col, g->t_w, g->xu, g->yu, g->t_ws,
+                                                     s));
+                sssp_pull_apply_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->yu, g->dstamp, it + 1,
+                                                                             pq, db);
+                G1_LAUNCHED();
+                prof_end(g, tok, s, 4.0 * (n + 1.0) + 8.0 * g->m + 44.0 * n);
+                ++pulls;
+            } else {
+                SsspPushNoReturn pol{g->dstamp, it + 1, db};
+                tok = prof_begin(g, "sssp_delta_heavy", s);
+                push_advance_kernel<SsspPushNoReturn, HeavyRows><<<sssp_noreturn_blocks(), kPushThreads, 0, s>>>(
+                    hrows, g->dl_col, g->dl_w, pq, pol, it, g->hist);
+                G1_LAUNCHED();
+                prof_end(g, tok, s, 16.0 * static_cast<uint32_t>(fr) + 12.0 * heavy_edges);
+            }
+            ++it;
+        }
+        if (limit == kInf) break;
+        lo = limit;
+        if (heavy_edges >= switch_edges || buckets >= kDeltaMaxBuckets) {
+            // every vertex at or above lo with a finite distance, with all its edges, for the plain loop
+            G1_TRY(scan(FullRows(g->row_off), lo, kInf, &fr, &mn));
+            if (static_cast<uint32_t>(fr) != 0) {
+                uint32_t launched2 = it;
+                G1_TRY(sssp_push_iterations(g, s, db, it, (fr >> 32) <= kSmallCap, &launched2, &dense, &pulls,
+                                            fr >> 32));
+                it = launched2;
+            }
+            break;
+        }
+        // the next bucket's light frontier; an empty bucket is skipped to the smallest distance left
+        G1_TRY(scan(lrows, lo, lo + delta, &fr, &mn));
+        if (mn == kInf) break;  // no finite distance at or above lo: done
+        if (mn >= lo + delta) {
+            lo = mn / delta * static_cast<uint64_t>(delta);
+            G1_TRY(scan(lrows, lo, lo + delta, &fr, &mn));
+        }
+        first_edges = fr >> 32;
+        small_first = first_edges <= kSmallCap;
+    }
+    dstamp_extract_kernel<<<fill_blocks, 256, 0, s>>>(n, g->dstamp, dist);
+    G1_LAUNCHED();
+    G1_HIP(hipEventRecord(g->ev[1], s));
+    G1_TRY(collect_push_run(g, s, it, dist, st, true, true));
     st->pull_iterations += pulls;
     st->push_iterations -= std::min(st->push_iterations, pulls);
     st->dense_iterations = dense;
@@ -1017,6 +1271,7 @@ g1_status g1_sssp(g1_graph* g, uint32_t src, uint32_t* dist, int32_t engine, g1_
     const hipStream_t s = as_stream(stream);
     switch (engine) {
         case G1_ENGINE_ADVANCE: return run_sssp_push(g, src, dist, stats, s);
+        case G1_ENGINE_DELTA: return run_sssp_delta(g, src, dist, stats, s);
         case G1_ENGINE_BLAS_MERGE:
         case G1_ENGINE_BLAS_CSRVEC: return run_minplus_pull(g, src, dist, true, engine, stats, s);
         default: return set_error(G1_ERR_INVALID_ARG, "g1_sssp: unknown engine %d", engine);

@@ -63,6 +63,27 @@ g1_status ensure_push_work(g1_graph* g, hipStream_t s) {
     return G1_OK;
 }
 
+// The light/heavy copy of the out-edge CSR for delta-stepping (split_rows_kernel), built on the device
+// once per bucket width: 8 bytes per edge and 4 per vertex, plus the scan minima.
+g1_status ensure_delta_split(g1_graph* g, uint32_t delta, hipStream_t s) {
+    G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "delta-stepping needs edge weights");
+    if (g->dl_col && g->dl_delta == delta) return G1_OK;
+    if (!g->dl_col) {
+        G1_TRY(alloc_n(g, &g->dl_col, std::max<size_t>(g->m, 1)));
+        G1_TRY(alloc_n(g, &g->dl_w, std::max<size_t>(g->m, 1)));
+        G1_TRY(alloc_n(g, &g->dl_lc, std::max<size_t>(g->n, 1)));
+        G1_TRY(alloc_n(g, &g->dl_mins, kMaxPushBlocks));
+    }
+    if (g->n > 0) {
+        const uint32_t waves_per_block = 256 / kWave;
+        const uint32_t blocks = std::min<uint32_t>((g->n + waves_per_block - 1) / waves_per_block, 8192);
+        split_rows_kernel<<<blocks, 256, 0, s>>>(g->n, g->row_off, g->col, g->w, delta, g->dl_col, g->dl_w, g->dl_lc);
+        G1_HIP(hipGetLastError());
+    }
+    g->dl_delta = delta;
+    return G1_OK;
+}
+
 // The rows of the in-edge CSR (or of PageRank's relabelled copy) longer than long_min nonzeros, for the
 // CSR-vector engine's long-row kernel; built once on the host.
 g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s) {

@@ -63,6 +63,14 @@ struct g1_graph {
     void* sweep_pieces = nullptr;               // SweepPiece records of the long out-edge rows (lazy)
     uint32_t n_sweep_pieces = 0;
     unsigned long long* hist = nullptr;         // per-iteration history of the push engines
+    // the delta-stepping SSSP engine's copy of the out-edge CSR with each row's light edges (weight below
+    // dl_delta) first (lazy, ensure_delta_split): columns, weights, light counts; per-workgroup minima of
+    // its distance scans
+    uint32_t* dl_col = nullptr;
+    uint32_t* dl_w = nullptr;
+    uint32_t* dl_lc = nullptr;
+    uint32_t* dl_mins = nullptr;
+    uint32_t dl_delta = 0;
     uint32_t* road_far[2] = {nullptr, nullptr};  // far lists of the road-graph kernel (road_far_cap each)
     uint32_t road_far_cap = 0;
     void* road_q = nullptr;                     // RoadQuery records (kRoadSlots)
@@ -137,6 +145,7 @@ g1_status ensure_push_work(g1_graph* g, hipStream_t s);  // work areas cleared o
 g1_status ensure_pull_work(g1_graph* g, bool pagerank);
 g1_status ensure_sweep_pieces(g1_graph* g, hipStream_t s);  // pieces of the long rows of the out-edge CSR
 g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s);  // after ensure_pull_work(g, true)
+g1_status ensure_delta_split(g1_graph* g, uint32_t delta, hipStream_t s);  // 8m + 4n bytes, rebuilt when delta changes
 
 // On-device CSR conversions (convert.hip): the transpose (workspace of transpose_workspace_bytes), an
 // exclusive scan of u32 (partials: one entry per 4096 items), u32 -> f32.

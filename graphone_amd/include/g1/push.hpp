@@ -132,9 +132,76 @@ __device__ __forceinline__ void block_stat_add(unsigned long long* slot, uint32_
     }
 }
 
+// The part of each row a push launch walks: where it starts in the column / weight arrays and how many
+// edges it has (a queue entry's degree is that count). FullRows: the whole row. Over a copy of the CSR
+// whose rows hold their light edges (weight below a bucket width) first and their heavy edges after,
+// lc[u] light edges in row u, LightRows walks the light part and HeavyRows the heavy part.
+struct FullRows {
+    const uint32_t* row_off;
+    __host__ __device__ FullRows(const uint32_t* ro) : row_off(ro) {}
+    __device__ __forceinline__ uint32_t begin(uint32_t u) const { return row_off[u]; }
+    __device__ __forceinline__ uint32_t degree(uint32_t u) const { return row_off[u + 1] - row_off[u]; }
+};
+struct LightRows {
+    const uint32_t* row_off;
+    const uint32_t* lc;
+    __device__ __forceinline__ uint32_t begin(uint32_t u) const { return row_off[u]; }
+    __device__ __forceinline__ uint32_t degree(uint32_t u) const { return lc[u]; }
+};
+struct HeavyRows {
+    const uint32_t* row_off;
+    const uint32_t* lc;
+    __device__ __forceinline__ uint32_t begin(uint32_t u) const { return row_off[u] + lc[u]; }
+    __device__ __forceinline__ uint32_t degree(uint32_t u) const { return row_off[u + 1] - row_off[u] - lc[u]; }
+};
+
+// The light/heavy copy of a weighted CSR (delta-stepping): row u of (lcol, lw) holds the row's edges of
+// weight below delta first, then the others, each part in the row's own order; lc[u] = its light count.
+// One wave per row: a counting pass, then each 64-edge chunk's light and heavy lanes are placed by their
+// ranks among the chunk's lanes.
+static __global__ void __launch_bounds__(256)
+split_rows_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+                  const uint32_t* __restrict__ w, uint32_t delta, uint32_t* __restrict__ lcol,
+                  uint32_t* __restrict__ lw, uint32_t* __restrict__ lc) {
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t r = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < n; r += waves) {  // wave-uniform
+        const uint32_t b = row_off[r], e = row_off[r + 1];
+        uint32_t light = 0;
+        for (uint32_t k = b; k < e; k += kWave) {
+            const bool l = k + lane < e && w[k + lane] < delta;
+            light += static_cast<uint32_t>(__popcll(ballot(l)));
+        }
+        uint32_t lp = b, hp = b + light;
+        for (uint32_t k = b; k < e; k += kWave) {
+            const uint32_t i = k + lane;
+            const bool in = i < e;
+            uint32_t c = 0, wt = 0;
+            if (in) {
+                c = col[i];
+                wt = w[i];
+            }
+            const bool l = in && wt < delta;
+            const uint64_t lm = ballot(l), hm = ballot(in && !l);
+            if (l) {
+                const uint32_t at = lp + mask_prefix(lm);
+                lcol[at] = c;
+                lw[at] = wt;
+            } else if (in) {
+                const uint32_t at = hp + mask_prefix(hm);
+                lcol[at] = c;
+                lw[at] = wt;
+            }
+            lp += static_cast<uint32_t>(__popcll(lm));
+            hp += static_cast<uint32_t>(__popcll(hm));
+        }
+        if (lane == 0) lc[r] = light;
+    }
+}
+
 // Processes this workgroup's share of the input frontier's edge tiles (see the file comment).
-template <typename P>
-__device__ __forceinline__ void advance_tiles(const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+template <typename P, typename RV>
+__device__ __forceinline__ void advance_tiles(const RV& rv, const uint32_t* __restrict__ col,
                                               const uint32_t* __restrict__ wts, const PushQueues& pq, const P& pol,
                                               const uint32_t* s_vb, const uint32_t* s_eb) {
     __shared__ uint32_t s_qoff[kPushTile];
@@ -189,7 +256,7 @@ __device__ __forceinline__ void advance_tiles(const uint32_t* __restrict__ row_o
                 at_e1 = (k != 0) && (qo == e1);
                 if (valid) {
                     s_qoff[k] = qo;
-                    s_rs[k] = row_off[u];
+                    s_rs[k] = rv.begin(u);
                     s_su[k] = pol.source_value(u);
                 }
             }
@@ -226,13 +293,10 @@ __device__ __forceinline__ void advance_tiles(const uint32_t* __restrict__ row_o
         }
         uint32_t cand[kPushEpt];
         uint32_t cdeg[kPushEpt];
-        uint32_t rb[kPushEpt], re[kPushEpt];
+        uint32_t wd[kPushEpt];
 #pragma unroll
         for (int r = 0; r < kPushEpt; ++r) {
-            if (won[r]) {
-                rb[r] = row_off[vv[r]];
-                re[r] = row_off[vv[r] + 1];
-            }
+            if (won[r]) wd[r] = rv.degree(vv[r]);
         }
         uint32_t my_cnt = 0, my_deg = 0;
 #pragma unroll
@@ -241,7 +305,7 @@ __device__ __forceinline__ void advance_tiles(const uint32_t* __restrict__ row_o
             cdeg[r] = 0;
             if (won[r]) {
                 ++activated;
-                const uint32_t dg = re[r] - rb[r];
+                const uint32_t dg = wd[r];
                 if (dg > 0) {
                     cand[r] = vv[r];
                     cdeg[r] = dg;
@@ -267,11 +331,10 @@ __device__ __forceinline__ bool push_prologue(const PushQueues& pq, uint32_t* s_
     return s_vb[kSegments] != 0;
 }
 
-template <typename P>
+template <typename P, typename RV = FullRows>
 static __global__ void __launch_bounds__(kPushThreads)
-push_advance_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
-                    const uint32_t* __restrict__ wts, PushQueues pq, P pol, uint32_t it,
-                    unsigned long long* __restrict__ hist) {
+push_advance_kernel(RV rv, const uint32_t* __restrict__ col, const uint32_t* __restrict__ wts, PushQueues pq, P pol,
+                    uint32_t it, unsigned long long* __restrict__ hist) {
     __shared__ uint32_t s_vb[kSegments + 1];
     __shared__ uint32_t s_eb[kSegments + 1];
     const bool active = push_prologue(pq, s_vb, s_eb);
@@ -280,7 +343,7 @@ push_advance_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __rest
         hist[2 * it + 1] = 0ull;
     }
     if (!active) return;
-    advance_tiles(row_off, col, wts, pq, pol, s_vb, s_eb);
+    advance_tiles(rv, col, wts, pq, pol, s_vb, s_eb);
 }
 
 // Distance upper bounds for the SSSP target checks: one byte per vertex (b = ceil(d / step), 255 = no
@@ -308,6 +371,10 @@ struct DistBound {
 // distance and claims the enqueue: a strict improvement installs (nd, iter); the first improvement of
 // this iteration is the one that saw an older stamp. An equal distance never replaces the word (the
 // older stamp is smaller), so a non-improving write cannot claim the vertex.
+//
+// limit (delta-stepping's light phases): an improvement to nd >= limit lowers the word with stamp 0 instead
+// of the iteration's, so it never claims the vertex; a later improvement below limit in the same iteration
+// still does.
 struct SsspPush {
     using src_t = uint32_t;
     static constexpr bool kWeighted = true;
@@ -316,6 +383,7 @@ struct SsspPush {
     unsigned long long* dstamp;
     uint32_t iter;
     DistBound db;
+    uint32_t limit = kInf;
     __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
         return static_cast<uint32_t>(dstamp[u] >> 32);
     }
@@ -346,10 +414,12 @@ struct SsspPush {
             if (!need[r]) continue;
             const uint32_t nd = du[r] + w[r];
             if (nd >= cur[r]) continue;
-            const unsigned long long old = atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | iter);
+            const bool in = nd < limit;
+            const unsigned long long old =
+                atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | (in ? iter : 0u));
             if (nd < static_cast<uint32_t>(old >> 32)) {
                 db.lower(v[r], nd);
-                won[r] = static_cast<uint32_t>(old) < iter;
+                won[r] = in && static_cast<uint32_t>(old) < iter;
             }
         }
     }
@@ -357,10 +427,14 @@ struct SsspPush {
     __device__ __forceinline__ bool relax1(uint32_t du, uint32_t v, uint32_t w) const {
         const uint32_t nd = du + w;
         if (nd >= static_cast<uint32_t>(dstamp[v] >> 32)) return false;
-        const unsigned long long old = atomicMin(&dstamp[v], (static_cast<unsigned long long>(nd) << 32) | iter);
-        return nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
+        const bool in = nd < limit;
+        const unsigned long long old =
+            atomicMin(&dstamp[v], (static_cast<unsigned long long>(nd) << 32) | (in ? iter : 0u));
+        return in && nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
     }
-    __device__ __forceinline__ SsspPush at_iteration(uint32_t it) const { return SsspPush{dstamp, it + 1, db}; }
+    __device__ __forceinline__ SsspPush at_iteration(uint32_t it) const {
+        return SsspPush{dstamp, it + 1, db, limit};
+    }
     // The source's distance read past the CU's cache: a workgroup that stays resident across iterations
     // must see improvements made by its own atomics in earlier iterations.
     __device__ __forceinline__ uint32_t fresh_source_value(uint32_t u) const {
@@ -378,6 +452,9 @@ constexpr uint32_t kScanRound = kScanPerThread * kPushThreads;
 // vertices whose word carries this iteration's stamp afterwards, since a strict improvement installs
 // (nd, iter) and nothing else does -- are enqueued by sssp_collect_kernel in one streaming pass. A dense
 // iteration spends most of its time waiting on returning atomics one after another; this one does not.
+//
+// With limit (see SsspPush) an improvement to nd >= limit carries stamp 0, so the collect pass does not
+// pick it up.
 struct SsspPushNoReturn {
     using src_t = uint32_t;
     static constexpr bool kWeighted = true;
@@ -386,6 +463,7 @@ struct SsspPushNoReturn {
     unsigned long long* dstamp;
     uint32_t iter;
     DistBound db;
+    uint32_t limit = kInf;
     __device__ __forceinline__ uint32_t source_value(uint32_t u) const {
         return static_cast<uint32_t>(dstamp[u] >> 32);
     }
@@ -412,7 +490,7 @@ struct SsspPushNoReturn {
             won[r] = false;
             const uint32_t nd = du[r] + w[r];
             if (need[r] && nd < cur[r]) {
-                (void)atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | iter);
+                (void)atomicMin(&dstamp[v[r]], (static_cast<unsigned long long>(nd) << 32) | (nd < limit ? iter : 0u));
                 db.lower(v[r], nd);
             }
         }
@@ -421,9 +499,9 @@ struct SsspPushNoReturn {
 
 // The winners of a no-return iteration: every vertex whose word carries `stamp` (the iteration's stamp)
 // and has out-edges joins the next frontier (segment blockIdx % kSegments of the output queue).
+template <typename RV = FullRows>
 static __global__ void __launch_bounds__(kPushThreads)
-sssp_collect_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const unsigned long long* __restrict__ dstamp,
-                    uint32_t stamp, PushQueues pq) {
+sssp_collect_kernel(uint32_t n, RV rv, const unsigned long long* __restrict__ dstamp, uint32_t stamp, PushQueues pq) {
     __shared__ unsigned long long s_scan[kPushThreads / kWave];
     __shared__ unsigned long long s_base;
     __shared__ uint32_t s_red[kPushThreads / kWave];
@@ -446,7 +524,7 @@ sssp_collect_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const unsi
             cdeg[r] = 0;
             if (hit[r]) {
                 ++activated;
-                const uint32_t dg = row_off[v + 1] - row_off[v];
+                const uint32_t dg = rv.degree(v);
                 if (dg > 0) {
                     cand[r] = v;
                     cdeg[r] = dg;
@@ -460,6 +538,64 @@ sssp_collect_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const unsi
     }
     block_stat_add(pq.stats + blockIdx.x, activated, s_red);
 }
+
+// A frontier built from the distances (delta-stepping's bucket scans), in one streaming pass over the
+// distance words: every vertex whose distance lies in [lo, hi) takes `stamp` (the stamp of the iteration
+// whose input this is, as if that iteration's predecessor had improved it; a pull step selects its input
+// by it) and, when its rv part has edges, joins the output queue (segment blockIdx % kSegments). Each
+// workgroup also writes mins[blockIdx.x], the smallest distance at or above lo that it saw (kInf if none),
+// so the host learns the next non-empty bucket from the same pass.
+template <typename RV>
+static __global__ void __launch_bounds__(kPushThreads)
+dist_range_frontier_kernel(uint32_t n, RV rv, unsigned long long* __restrict__ dstamp, uint32_t lo, uint32_t hi,
+                           uint32_t stamp, PushQueues pq, uint32_t* __restrict__ mins) {
+    __shared__ unsigned long long s_scan[kPushThreads / kWave];
+    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_min[kPushThreads / kWave];
+    const uint32_t seg_out = blockIdx.x % kSegments;
+    uint32_t mn = kInf;
+    for (uint32_t base = blockIdx.x * kScanRound; base < n; base += gridDim.x * kScanRound) {  // uniform
+        uint32_t cand[kScanPerThread], cdeg[kScanPerThread];
+        uint32_t my_cnt = 0, my_deg = 0;
+        uint32_t d[kScanPerThread];
+#pragma unroll
+        for (uint32_t r = 0; r < kScanPerThread; ++r) {
+            const uint32_t v = base + r * kPushThreads + threadIdx.x;
+            d[r] = v < n ? static_cast<uint32_t>(dstamp[v] >> 32) : kInf;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kScanPerThread; ++r) {
+            const uint32_t v = base + r * kPushThreads + threadIdx.x;
+            cand[r] = kInf;
+            cdeg[r] = 0;
+            if (d[r] >= lo && d[r] != kInf && d[r] < mn) mn = d[r];
+            if (d[r] >= lo && d[r] < hi) {
+                dstamp[v] = (static_cast<unsigned long long>(d[r]) << 32) | stamp;
+                const uint32_t dg = rv.degree(v);
+                if (dg > 0) {
+                    cand[r] = v;
+                    cdeg[r] = dg;
+                    ++my_cnt;
+                    my_deg += dg;
+                }
+            }
+        }
+        push_enqueue(cand, cdeg, my_cnt, my_deg, pq, seg_out, s_scan, &s_base);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int off = kWave / 2; off >= 1; off >>= 1) {
+        const uint32_t o = __shfl_down(mn, off, kWave);
+        mn = o < mn ? o : mn;
+    }
+    if (lane_id() == 0) s_min[threadIdx.x / kWave] = mn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kPushThreads / kWave; ++w) mn = s_min[w] < mn ? s_min[w] : mn;
+        mins[blockIdx.x] = mn;
+    }
+}
+
 
 // BFS top-down step: the first visitor of v sets its depth; optionally marks v in a bitmap that a
 // following bottom-up step reads.
@@ -712,7 +848,7 @@ bfs_step_kernel(uint32_t n, uint32_t m, const uint32_t* __restrict__ row_off, co
         pull_groups(n, t_row_off, t_col, row_off, depth, fb_cur, fb_next, it + 1, pq);
     } else {
         BfsPush pol{depth, kDirOpt ? fb_next : nullptr, it + 1};
-        advance_tiles(row_off, col, nullptr, pq, pol, s_vb, s_eb);
+        advance_tiles(FullRows(row_off), col, nullptr, pq, pol, s_vb, s_eb);
     }
 }
 
@@ -735,12 +871,13 @@ static __global__ void push_seed_kernel(const uint32_t* __restrict__ row_off, ui
 // Start of a push-engine run in one launch: the per-vertex state (SSSP: distance words (dist << 32 |
 // stamp), all unreached but src; BFS: depths, and the direction-optimizing engine's first two frontier
 // bitmaps with src set in the first), and, in block 0, the seeded input queue (push_seed_kernel) and the
-// run counters ctr[0..8) cleared.
+// run counters ctr[0..8) cleared. With lc (delta-stepping) the seed's degree is its light edge count.
 template <bool kSssp>
 static __global__ void __launch_bounds__(256)
 push_start_kernel(uint32_t n, uint32_t src, unsigned long long* __restrict__ dstamp, uint32_t* __restrict__ depth,
-                  uint32_t* __restrict__ fb, uint32_t words, const uint32_t* __restrict__ row_off, uint32_t* q,
-                  uint32_t* qoff, unsigned long long* cnt3, unsigned long long* ctr, uint8_t* __restrict__ dbound) {
+                  uint32_t* __restrict__ fb, uint32_t words, const uint32_t* __restrict__ row_off,
+                  const uint32_t* __restrict__ lc, uint32_t* q, uint32_t* qoff, unsigned long long* cnt3,
+                  unsigned long long* ctr, uint8_t* __restrict__ dbound) {
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t v = t0; v < n; v += stride) {
@@ -755,7 +892,7 @@ push_start_kernel(uint32_t n, uint32_t src, unsigned long long* __restrict__ dst
         if (threadIdx.x < 8) ctr[threadIdx.x] = 0ull;
         __syncthreads();
         if (threadIdx.x == 0) {
-            const uint32_t dg = row_off[src + 1] - row_off[src];
+            const uint32_t dg = lc ? lc[src] : row_off[src + 1] - row_off[src];
             q[0] = src;
             qoff[0] = 0;
             cnt3[0] = dg ? ((static_cast<unsigned long long>(dg) << 32) | 1ull) : 0ull;
@@ -840,9 +977,9 @@ constexpr int kSmallThreads = 1024;
 constexpr uint32_t kSmallCap = 8192;
 constexpr uint32_t kSmallPer = kSmallCap / kSmallThreads;
 
-template <typename P>
+template <typename P, typename RV = FullRows>
 static __global__ void __launch_bounds__(kSmallThreads)
-push_small_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ col,
+push_small_kernel(RV rv, const uint32_t* __restrict__ col,
                   const uint32_t* __restrict__ wts, P pol, uint32_t* q0, uint32_t* q1, uint32_t* qoff0,
                   uint32_t* qoff1, unsigned long long* qcnt, uint32_t cap, uint32_t it0, uint32_t it_limit,
                   uint32_t* fb, uint32_t words, unsigned long long* __restrict__ hist,
@@ -887,8 +1024,8 @@ push_small_kernel(const uint32_t* __restrict__ row_off, const uint32_t* __restri
             dg[r] = 0;
             if (i < n) {
                 const uint32_t u = s_q[cur][i];
-                rb[r] = row_off[u];
-                dg[r] = row_off[u + 1] - rb[r];
+                rb[r] = rv.begin(u);
+                dg[r] = rv.degree(u);
             }
         }
 #pragma unroll

@@ -300,11 +300,17 @@ g1_status g1_graph_csr_dev(const g1_graph* graph, int32_t transpose, const uint3
  *   G1_ENGINE_BLAS_MERGE   pull semiring SpMV engine, merge-path kernel (engine/blas_gc.h:171-351)
  *   G1_ENGINE_BLAS_CSRVEC  the same engine over the CSR-vector kernel
  *   G1_ENGINE_DIROPT       BFS only: queue/bitmap switching push<->pull by frontier density
+ *   G1_ENGINE_DELTA        SSSP only: delta-stepping over the advance engine's kernels (light/heavy
+ *                          buckets of the edge weights, then the advance loop); the same distances as
+ *                          G1_ENGINE_ADVANCE with fewer relaxations. No counterpart in the reference
+ *                          (whose sssp_adv loop is G1_ENGINE_ADVANCE); its copy of the CSR split by
+ *                          weight takes 8 bytes per edge and 4 per vertex, built on the first call.
  * ------------------------------------------------------------------------------------------- */
 #define G1_ENGINE_ADVANCE 0
 #define G1_ENGINE_BLAS_MERGE 1
 #define G1_ENGINE_BLAS_CSRVEC 2
 #define G1_ENGINE_DIROPT 3
+#define G1_ENGINE_DELTA 4
 
 typedef struct g1_run_stats {
     uint32_t iterations;       /* engine iterations (Forward calls)                         */

@@ -48,13 +48,43 @@ def test_bfs_bitexact(graphs, engine):
             assert got.stats["reached"] == int(np.sum(want != oracle.INF))
 
 
-@pytest.mark.parametrize("engine", ["advance", "merge", "csrvec"])
+@pytest.mark.parametrize("engine", ["advance", "delta", "merge", "csrvec"])
 def test_sssp_bitexact(graphs, engine):
     for name, h, g in graphs:
         for src in _sources(h):
             want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, src)
             got = g.sssp(src, engine)
             assert np.array_equal(got.values, want), (name, src, engine)
+
+
+@pytest.mark.parametrize("width", ["1", "5", "64", "100000"])
+@pytest.mark.parametrize("switch", ["0", "0.01", "1"])
+def test_sssp_delta_schedules(graphs, monkeypatch, width, switch):
+    """Delta-stepping with bucket widths from 1 (every edge heavy, one bucket per distance) to wider than any
+    path (every edge light: one bucket), leaving the buckets for the advance loop after the first heavy
+    phase, after one that relaxes 1 % of the edges, or never (up to its bucket cap): every schedule gives
+    Dijkstra's distances, from the hub, a low-degree vertex and an isolated one."""
+    monkeypatch.setenv("G1_SSSP_DELTA", width)
+    monkeypatch.setenv("G1_SSSP_DELTA_SWITCH", switch)
+    name, h, g = graphs[0]
+    deg = h.out_degrees()
+    pos = np.flatnonzero(deg > 0)
+    for src in (int(np.argmax(deg)), int(pos[np.argmin(deg[pos])]), int(np.flatnonzero(deg == 0)[0])):
+        want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, src)
+        got = g.sssp(src, "delta")
+        assert np.array_equal(got.values, want), (name, src, width, switch)
+        assert got.stats["reached"] == int(np.sum(want != oracle.INF))
+
+
+def test_sssp_delta_zero_weights(gpu):
+    """Weights 0..3 (a quarter of the edges weigh nothing, so improvements stay inside their bucket):
+    delta-stepping from the hub and from vertex 1 equals Dijkstra."""
+    import graphone_amd as g1
+    h = g1.HostGraph.rmat(13, 16, 0.57, 0.19, 0.19, seed=31).assign_weights(0, 3, seed=32)
+    g = g1.Graph(h)
+    for src in (int(np.argmax(h.out_degrees())), 1):
+        want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, src)
+        assert np.array_equal(g.sssp(src, "delta").values, want), src
 
 
 def test_sssp_engines_match_restated_engines(graphs):

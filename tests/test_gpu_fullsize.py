@@ -50,7 +50,7 @@ def test_rmat22_bfs_bitexact(rmat22, engine):
         assert got.stats["pull_iterations"] >= 1, got.stats
 
 
-@pytest.mark.parametrize("engine", ["advance", "merge", "csrvec"])
+@pytest.mark.parametrize("engine", ["advance", "delta", "merge", "csrvec"])
 def test_rmat22_sssp_bitexact(rmat22, engine):
     h, g, src, _, want = rmat22
     got = g.sssp(src, engine)
@@ -70,8 +70,9 @@ def test_rmat22_sssp_other_sources(rmat22):
     order = np.argsort(deg[reached], kind="stable")
     for s in (int(reached[order[0]]), int(reached[order[order.size // 2]])):
         want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, s)
-        got = g.sssp(s, "advance")
-        assert np.array_equal(got.values, want), (s, _diff(got.values, want))
+        for engine in ("advance", "delta"):
+            got = g.sssp(s, engine)
+            assert np.array_equal(got.values, want), (s, engine, _diff(got.values, want))
         wb = oracle.bfs_queue(h.row_offsets, h.col_indices, s)
         gb = g.bfs(s, "diropt")
         assert np.array_equal(gb.values, wb), (s, _diff(gb.values, wb))
