@@ -174,14 +174,11 @@ uint32_t bfs_step_blocks() {
     return b;
 }
 
-// The push SSSP engine switches an iteration to a pull step when its frontier has at least
-// G1_SSSP_PULL (default 0.8) times the graph's edges; G1_SSSP_PULL=0 disables the switch.
+// The push SSSP engines switch an iteration to a pull step when its frontier has at least
+// G1_SSSP_PULL (default 0.8) times the graph's edges; G1_SSSP_PULL=0 disables the switch. Read at every run.
 double sssp_pull_fraction() {
-    static const double f = [] {
-        const char* e = getenv("G1_SSSP_PULL");
-        return e ? atof(e) : 0.8;
-    }();
-    return f;
+    const char* e = getenv("G1_SSSP_PULL");
+    return e ? atof(e) : 0.8;
 }
 
 // One wave sums the kSegments counters of a frontier and posts nf and E to the graph's coherent pinned
@@ -978,7 +975,22 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
         const uint64_t heavy_edges = fr >> 32;
         if (static_cast<uint32_t>(fr) != 0) {
             PushQueues pq = make_queues(g, it);
-            if (heavy_edges >= pull_min) {
+            if (heavy_edges >= pull_min && limit - lo <= 255u) {
+                // x = the bucket's distances less lo, one byte each (255: not in the bucket), y = A^T (min,+) x,
+                // apply with lo added back
+                tok = prof_begin(g, "sssp_delta_pull", s);
+                sssp_pull_xb_kernel<<<grid_for((n + 3) / 4, kPushThreads, kMaxPushBlocks), kPushThreads, 0, s>>>(
+                    n, g->dstamp, static_cast<uint32_t>(lo), limit, it, g->dl_xb, pq, g->hist);
+                G1_LAUNCHED();
+                G1_TRY(launch_spmv_merge<MinPlusU8X>(n, g->m, g->t_row_off, g->t_col, g->t_w, g->dl_xb, g->yu,
+                                                     g->t_ws, s));
+                sssp_pull_apply_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->yu, g->dstamp, it + 1,
+                                                                             pq, db, static_cast<uint32_t>(lo));
+                G1_LAUNCHED();
+                // x 8n + n; SpMV 4(n+1) + 8m + n (x) + 4n (y); apply 4n (y) + 16n (dist word) + 8n (degrees)
+                prof_end(g, tok, s, 4.0 * (n + 1.0) + 8.0 * g->m + 42.0 * n);
+                ++pulls;
+            } else if (heavy_edges >= pull_min) {
                 // x = the bucket's distances (the scan stamped them `it`), y = A^T (min,+) x, apply
                 tok = prof_begin(g, "sssp_pull", s);
                 sssp_pull_x_kernel<<<grid_for(n, kPushThreads, kMaxPushBlocks), kPushThreads, 0, s>>>(

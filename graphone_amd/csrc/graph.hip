@@ -64,7 +64,7 @@ g1_status ensure_push_work(g1_graph* g, hipStream_t s) {
 }
 
 // The light/heavy copy of the out-edge CSR for delta-stepping (split_rows_kernel), built on the device
-// once per bucket width: 8 bytes per edge and 4 per vertex, plus the scan minima.
+// once per bucket width: 8 bytes per edge and 5 per vertex, plus the scan minima.
 g1_status ensure_delta_split(g1_graph* g, uint32_t delta, hipStream_t s) {
     G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "delta-stepping needs edge weights");
     if (g->dl_col && g->dl_delta == delta) return G1_OK;
@@ -73,6 +73,7 @@ g1_status ensure_delta_split(g1_graph* g, uint32_t delta, hipStream_t s) {
         G1_TRY(alloc_n(g, &g->dl_w, std::max<size_t>(g->m, 1)));
         G1_TRY(alloc_n(g, &g->dl_lc, std::max<size_t>(g->n, 1)));
         G1_TRY(alloc_n(g, &g->dl_mins, kMaxPushBlocks));
+        G1_TRY(alloc_n(g, &g->dl_xb, (static_cast<size_t>(g->n) + 3) / 4 * 4 + 4));
     }
     if (g->n > 0) {
         const uint32_t waves_per_block = 256 / kWave;

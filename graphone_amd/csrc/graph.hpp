@@ -70,6 +70,7 @@ struct g1_graph {
     uint32_t* dl_w = nullptr;
     uint32_t* dl_lc = nullptr;
     uint32_t* dl_mins = nullptr;
+    uint8_t* dl_xb = nullptr;  // the heavy pull step's byte x (MinPlusU8X), n rounded up to 4
     uint32_t dl_delta = 0;
     uint32_t* road_far[2] = {nullptr, nullptr};  // far lists of the road-graph kernel (road_far_cap each)
     uint32_t road_far_cap = 0;

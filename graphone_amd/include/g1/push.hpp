@@ -665,11 +665,45 @@ sssp_pull_x_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, ui
     }
 }
 
-// Apply: every vertex whose y beats its distance takes (y, stamp) and, with out-edges, joins the next
-// frontier (segment blockIdx % kSegments of the output queue). Each vertex has one owner, so no atomics.
+// The byte x of the delta engine's heavy pull step (MinPlusU8X): xb[v] = d(v) - lo for the bucket [lo, hi)
+// (hi - lo <= 255), 255 for every other vertex; the history record as sssp_pull_x_kernel writes it.
+static __global__ void __launch_bounds__(kPushThreads)
+sssp_pull_xb_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, uint32_t lo, uint32_t hi, uint32_t it,
+                    uint8_t* __restrict__ xb, PushQueues pq, unsigned long long* __restrict__ hist) {
+    if (blockIdx.x == 0) {
+        __shared__ uint32_t s_vb[kSegments + 1];
+        __shared__ uint32_t s_eb[kSegments + 1];
+        push_prologue(pq, s_vb, s_eb);
+        if (threadIdx.x == 0) {
+            hist[2 * it] = (static_cast<unsigned long long>(s_eb[kSegments]) << 32) | s_vb[kSegments];
+            hist[2 * it + 1] = 0ull;
+        }
+    }
+    // four vertices per thread: one 32-bit store of four bytes
+    const uint32_t quads = (n + 3) / 4;
+    for (uint32_t q = blockIdx.x * kPushThreads + threadIdx.x; q < quads; q += gridDim.x * kPushThreads) {
+        uint32_t packed = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t v = q * 4 + k;
+            uint32_t b = 255u;
+            if (v < n) {
+                const uint32_t d = static_cast<uint32_t>(dstamp[v] >> 32);
+                if (d >= lo && d < hi) b = d - lo;
+            }
+            packed |= b << (8 * k);
+        }
+        reinterpret_cast<uint32_t*>(xb)[q] = packed;
+    }
+}
+
+// Apply: every vertex whose y (plus y_add, saturating) beats its distance takes (y + y_add, stamp) and, with
+// out-edges, joins the next frontier (segment blockIdx % kSegments of the output queue). Each vertex has
+// one owner, so no atomics.
 static __global__ void __launch_bounds__(kPushThreads)
 sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ y,
-                       unsigned long long* __restrict__ dstamp, uint32_t stamp, PushQueues pq, DistBound db) {
+                       unsigned long long* __restrict__ dstamp, uint32_t stamp, PushQueues pq, DistBound db,
+                       uint32_t y_add = 0) {
     __shared__ unsigned long long s_scan[kPushThreads / kWave];
     __shared__ unsigned long long s_base;
     __shared__ uint32_t s_red[kPushThreads / kWave];
@@ -686,7 +720,8 @@ sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const u
             yv[r] = kInf;
             ds[r] = 0ull;
             if (v < n) {
-                yv[r] = y[v];
+                const uint32_t yr = y[v];
+                yv[r] = (yr >= kInf - y_add) ? kInf : yr + y_add;
                 ds[r] = dstamp[v];
             }
         }

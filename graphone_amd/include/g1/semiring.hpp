@@ -39,6 +39,29 @@ struct MinPlusU32 {
     }
 };
 
+// Min-plus over a one-byte x (the delta-stepping engine's heavy pull step): x[u] is a bucket member's
+// distance minus the bucket's start (0..254) and 255 for every other vertex, so the gathered vector is a
+// quarter of the u32 one and stays in the L2 caches; y = min over the in-edges of x + w, the bucket start
+// added back by the apply. Order free, like MinPlusU32.
+struct MinPlusU8X {
+    using value_t = uint32_t;
+    using x_t = uint8_t;
+    using y_t = uint32_t;
+    struct acc_t { uint32_t v; };
+    static constexpr bool kOrderFree = true;
+
+    __device__ __forceinline__ static acc_t identity() { return acc_t{kInf}; }
+    __device__ __forceinline__ static uint32_t combine(uint32_t w, uint8_t x) {
+        return (x == 255u || w >= kInf - 255u) ? kInf : w + x;
+    }
+    __device__ __forceinline__ static void add(acc_t& a, uint32_t p) { a.v = p < a.v ? p : a.v; }
+    __device__ __forceinline__ static acc_t merge(acc_t a, acc_t b) { return acc_t{b.v < a.v ? b.v : a.v}; }
+    __device__ __forceinline__ static uint32_t finalize(acc_t a) { return a.v; }
+    __device__ __forceinline__ static acc_t shfl_down(acc_t a, int d, int width) {
+        return acc_t{static_cast<uint32_t>(__shfl_down(static_cast<int>(a.v), d, width))};
+    }
+};
+
 // PageRank semiring over float (apps/pagerank_blas.cu:82-111: combine = info * weight, reduce = +),
 // accumulated with Kahan-Babuska (Neumaier) compensation: s holds the running sum, c the rounding
 // error collected so far; the value is s + c.

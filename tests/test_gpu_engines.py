@@ -76,6 +76,26 @@ def test_sssp_delta_schedules(graphs, monkeypatch, width, switch):
         assert got.stats["reached"] == int(np.sum(want != oracle.INF))
 
 
+@pytest.mark.parametrize("hi,width,pull", [(255, "16", "0.01"), (1000, "300", "0.01"), (255, "16", "0")])
+def test_sssp_delta_heavy_steps(gpu, monkeypatch, hi, width, pull):
+    """The heavy phase as a pull step over a one-byte x (bucket width <= 255), over the u32 x (a wider
+    bucket), and as a push launch (pull steps off): Dijkstra's distances from the hub and from a low-degree
+    vertex, with the pull step taken where it is enabled."""
+    import graphone_amd as g1
+    monkeypatch.setenv("G1_SSSP_DELTA", width)
+    monkeypatch.setenv("G1_SSSP_PULL", pull)
+    h = g1.HostGraph.rmat(14, 16, seed=41).assign_weights(1, hi, seed=42)
+    g = g1.Graph(h)
+    deg = h.out_degrees()
+    pos = np.flatnonzero(deg > 0)
+    for src in (int(np.argmax(deg)), int(pos[np.argmin(deg[pos])])):
+        want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, src)
+        got = g.sssp(src, "delta")
+        assert np.array_equal(got.values, want), (src, hi, width, pull)
+        if src == int(np.argmax(deg)):
+            assert (got.stats["pull_iterations"] >= 1) == (pull != "0"), got.stats
+
+
 def test_sssp_delta_zero_weights(gpu):
     """Weights 0..3 (a quarter of the edges weigh nothing, so improvements stay inside their bucket):
     delta-stepping from the hub and from vertex 1 equals Dijkstra."""
