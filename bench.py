@@ -636,6 +636,8 @@ def main() -> int:
     ap.add_argument("--serve-max-queries", type=int, default=600)
     ap.add_argument("--sssp-engine", choices=["advance", "delta"], default="delta",
                     help="SSSP engine of the timed step (the other push engine is timed beside it)")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the step over the other push engine (profiling runs)")
     ap.add_argument("--only", choices=["grid213", "serving"], default=None,
                     help="run one secondary workload alone and print its JSON (for profiling runs); the "
                          "benchmark line is not printed")
@@ -709,16 +711,18 @@ def main() -> int:
 
     # the other push engine over the same step (the reference's frontier loop when the headline is delta)
     other = "advance" if args.sssp_engine == "delta" else "delta"
-    o_ms, o_b, o_s = timed(other)
-    o_sssp_ms = float(np.mean([x["elapsed_ms"] for x in o_s]))
-    other_step = {
-        "sssp_engine": other, "ms_per_step": round(o_ms, 4),
-        "value": round(2.0 * m_reach / (o_ms * 1e-3) / 1e9, 4), "sssp_ms": round(o_sssp_ms, 4),
-        "sssp_iterations": o_s[0]["iterations"], "relaxations": o_s[0]["edges_sum"],
-        "same_distances": bool(np.array_equal(dist.numpy(), gpu_dist)),
-        "roofline": roofline_from_profile(profiled(other), args.scale),
-    }
-    log(f"{other} step {o_ms:.3f} ms (sssp {o_sssp_ms:.3f} ms), {other_step['value']:.3f} GTEPS")
+    other_step = None
+    if not args.no_other:
+        o_ms, o_b, o_s = timed(other)
+        o_sssp_ms = float(np.mean([x["elapsed_ms"] for x in o_s]))
+        other_step = {
+            "sssp_engine": other, "ms_per_step": round(o_ms, 4),
+            "value": round(2.0 * m_reach / (o_ms * 1e-3) / 1e9, 4), "sssp_ms": round(o_sssp_ms, 4),
+            "sssp_iterations": o_s[0]["iterations"], "relaxations": o_s[0]["edges_sum"],
+            "same_distances": bool(np.array_equal(dist.numpy(), gpu_dist)),
+            "roofline": roofline_from_profile(profiled(other), args.scale),
+        }
+        log(f"{other} step {o_ms:.3f} ms (sssp {o_sssp_ms:.3f} ms), {other_step['value']:.3f} GTEPS")
 
     extras = {}
     if not args.no_extras:

@@ -837,7 +837,9 @@ constexpr uint32_t kDeltaMaxBuckets = 64;
 // A light-phase iteration of at least G1_SSSP_DELTA_NORETURN (default 0.3) times the graph's edges relaxes
 // without the winner test and collects its winners afterwards (SsspPushNoReturn + sssp_collect_kernel).
 // Light frontiers are smaller than the advance engine's dense ones; lower thresholds measured slower
-// (RMAT22, width 16: 1.40 ms at 0.005, 1.37 at 0.02, 1.33 at 0.3), the host then reading every frontier.
+// (RMAT22, width 16: 1.40 ms at 0.005, 1.37 at 0.02, 1.33 at 0.3, with the host reading every frontier).
+// Light iterations run in device-driven batches (the kind of each launch after the first of a batch
+// is then decided without its edge count): 1.27 ms against 1.28 one at a time.
 double sssp_delta_noreturn() {
     const char* e = getenv("G1_SSSP_DELTA_NORETURN");
     const double v = e ? atof(e) : 0.3;
@@ -909,9 +911,9 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
             G1_LAUNCHED();
             prof_end(g, tok_c, s, 0.0);
         } else {
-            SsspPush pol{g->dstamp, i + 1, db, limit};
+            SsspPushT<true> pol{g->dstamp, i + 1, db, limit};
             tok = prof_begin(g, "sssp_delta_light", s);
-            push_advance_kernel<SsspPush, LightRows><<<sssp_push_blocks(), kPushThreads, 0, s>>>(
+            push_advance_kernel<SsspPushT<true>, LightRows><<<sssp_push_blocks(), kPushThreads, 0, s>>>(
                 lrows, g->dl_col, g->dl_w, pq, pol, i, g->hist);
             G1_LAUNCHED();
             prof_end(g, tok, s, 0.0);
@@ -967,7 +969,7 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
         uint32_t launched = it;
         if (first_edges != 0)
             G1_TRY(drive_push(g, s, light_launch, light_after, light_small, small_first, &launched, no_pull, 0, it,
-                              first_edges, noreturn_min == ~0ull ? 0 : noreturn_min));
+                              first_edges));
         it = launched;
         G1_CHECK(it + 2 <= kHistIters, G1_ERR_INTERNAL, "delta engine exceeded %u iterations", kHistIters);
         // heavy phase: the bucket's vertices with heavy edges

@@ -375,7 +375,12 @@ struct DistBound {
 // limit (delta-stepping's light phases): an improvement to nd >= limit lowers the word with stamp 0 instead
 // of the iteration's, so it never claims the vertex; a later improvement below limit in the same iteration
 // still does.
-struct SsspPush {
+//
+// kAllAtomics: a thread issues all its atomics before testing any (for launches where most relaxations
+// improve their target, as in delta-stepping's light phases, where waiting on each in turn serializes
+// up to kPushEpt round trips per tile).
+template <bool kAllAtomics>
+struct SsspPushT {
     using src_t = uint32_t;
     static constexpr bool kWeighted = true;
     static constexpr bool kQueueLeaves = false;
@@ -408,6 +413,28 @@ struct SsspPush {
         uint32_t cur[N];
 #pragma unroll
         for (int r = 0; r < N; ++r) cur[r] = need[r] ? static_cast<uint32_t>(dstamp[v[r]] >> 32) : 0u;
+        if constexpr (kAllAtomics) {
+            unsigned long long old[N];
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                const uint32_t nd = du[r] + w[r];
+                need[r] = need[r] && nd < cur[r];
+                old[r] = 0ull;
+                if (need[r])
+                    old[r] = atomicMin(&dstamp[v[r]],
+                                       (static_cast<unsigned long long>(nd) << 32) | (nd < limit ? iter : 0u));
+            }
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                const uint32_t nd = du[r] + w[r];
+                won[r] = false;
+                if (need[r] && nd < static_cast<uint32_t>(old[r] >> 32)) {
+                    db.lower(v[r], nd);
+                    won[r] = nd < limit && static_cast<uint32_t>(old[r]) < iter;
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int r = 0; r < N; ++r) {
             won[r] = false;
@@ -432,8 +459,8 @@ struct SsspPush {
             atomicMin(&dstamp[v], (static_cast<unsigned long long>(nd) << 32) | (in ? iter : 0u));
         return in && nd < static_cast<uint32_t>(old >> 32) && static_cast<uint32_t>(old) < iter;
     }
-    __device__ __forceinline__ SsspPush at_iteration(uint32_t it) const {
-        return SsspPush{dstamp, it + 1, db, limit};
+    __device__ __forceinline__ SsspPushT at_iteration(uint32_t it) const {
+        return SsspPushT{dstamp, it + 1, db, limit};
     }
     // The source's distance read past the CU's cache: a workgroup that stays resident across iterations
     // must see improvements made by its own atomics in earlier iterations.
@@ -442,6 +469,7 @@ struct SsspPush {
     }
     __device__ __forceinline__ void set_next_bitmap(uint32_t*) {}
 };
+using SsspPush = SsspPushT<false>;
 
 // Vertex scans that enqueue (sssp_collect_kernel, sssp_pull_apply_kernel): vertices per thread per round.
 constexpr uint32_t kScanPerThread = 8;

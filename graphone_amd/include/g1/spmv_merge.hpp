@@ -123,8 +123,9 @@ __device__ __forceinline__ void merge_load_index(uint32_t tile, const uint32_t* 
         const uint32_t k = r * kMergeThreads + tid;
         if (k < rows) st.re[r] = row_offsets[st.c0.x + 1 + k];
         if (k < nzs) {
-            st.ci[r] = col[st.c0.y + k];
-            st.wv[r] = kHasValues ? vals[st.c0.y + k] : typename S::value_t(1);
+            // read once: streaming loads, so the gathered x keeps more of the L2
+            st.ci[r] = __builtin_nontemporal_load(col + st.c0.y + k);
+            st.wv[r] = kHasValues ? __builtin_nontemporal_load(vals + st.c0.y + k) : typename S::value_t(1);
         }
     }
 }
