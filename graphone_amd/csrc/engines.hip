@@ -277,9 +277,14 @@ uint32_t tail_batch(uint64_t edges) {
 template <typename Launch, typename AfterLaunch, typename Small, typename Pull>
 g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& after, Small&& small,
                      bool small_first, uint32_t* launched, Pull&& pull, uint64_t pull_min_edges, uint32_t it0 = 0,
-                     uint64_t first_edges = ~0ull, uint64_t single_min_edges = 0) {
+                     uint64_t first_edges = ~0ull, uint64_t single_min_edges = 0, uint32_t batch0 = 2,
+                     bool lazy_profile = false) {
     if (single_min_edges == 0) single_min_edges = pull_min_edges;
-    uint32_t it = it0, batch = 2;
+    // With profiling on, launches go one at a time so that `after` can attribute each one's counts, unless
+    // the caller attributes them afterwards from the iteration history (lazy_profile): then the launches
+    // keep their batches and their event times are those of the unprofiled run.
+    const bool one_by_one = g->profile && !lazy_profile;
+    uint32_t it = it0, batch = batch0;
     bool try_small = small_first;
     bool known = first_edges != ~0ull;  // next_edges holds the edge count of the frontier of iteration `it`
     bool shrinking = false;             // ... and it is below the count read before it
@@ -287,12 +292,12 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
     for (;;) {
         // A frontier whose out-edges fit the single-workgroup kernel runs there (see push_small_kernel)
         // until it grows past it or ends; profiling keeps one launch per iteration instead.
-        if (try_small && !g->profile) {
+        if (try_small && !one_by_one) {
             uint32_t res[2];
             G1_TRY(small(it, res));
             it = res[0];
             if (res[1] == 0) break;
-            batch = 2;
+            batch = batch0;
             shrinking = false;
             // the frontier handed back is past the small kernel's size: read it so that this iteration can
             // already be a pull step or a dense launch
@@ -308,7 +313,7 @@ g1_status drive_push(g1_graph* g, hipStream_t s, Launch&& launch, AfterLaunch&& 
             ++it;
         } else {
             uint32_t b = shrinking ? tail_batch(next_edges) : batch;
-            if (g->profile) b = 1;
+            if (one_by_one) b = 1;
             if (single_min_edges && (!known || !shrinking || next_edges >= single_min_edges / 4)) b = 1;
             G1_CHECK(it + b <= kHistIters, G1_ERR_INTERNAL, "push engine exceeded %u iterations", kHistIters);
             for (uint32_t k = 0; k < b; ++k) {
@@ -921,26 +926,46 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
         return G1_OK;
     };
     // profiling: the launch's algorithmic bytes, counted as for the advance engine's launches
+    // Profiling: each light launch's algorithmic bytes (as counted for the advance engine's launches) are
+    // set after its phase, from the iteration history: its input (nf, E) and its output queue (the next
+    // iteration's nf); no synchronization between the launches.
+    struct LightTok {
+        int tok, tok_c;
+        uint32_t it;
+    };
+    std::vector<LightTok> light_toks;
     auto light_after = [&](uint32_t i) -> g1_status {
-        if (!g->profile) return G1_OK;
-        const uint64_t act0 = g->h_ctr[3];
-        G1_TRY(fold_slots(g, s, true));
-        unsigned long long h[1];
-        G1_HIP(hipMemcpy(h, g->hist + 2 * i, sizeof(h), hipMemcpyDeviceToHost));
-        const double nf = static_cast<double>(static_cast<uint32_t>(h[0]));
-        const double E = static_cast<double>(h[0] >> 32);
-        const double act = static_cast<double>(g->h_ctr[3] - act0);
-        if (tok_c >= 0) {
-            prof_set_bytes(g, tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E);
-            prof_set_bytes(g, tok_c, 8.0 * n + 16.0 * act);
-        } else {
-            prof_set_bytes(g, tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E + 4.0 * act);
+        if (g->profile) light_toks.push_back(LightTok{tok, tok_c, i});
+        return G1_OK;
+    };
+    auto light_bytes = [&](uint32_t it_end) -> g1_status {
+        if (light_toks.empty()) return G1_OK;
+        const uint32_t i0 = light_toks.front().it;
+        std::vector<unsigned long long> h(2ull * (it_end - i0 + 1), 0ull);
+        G1_HIP(hipMemcpy(h.data(), g->hist + 2ull * i0, sizeof(unsigned long long) * 2ull * (it_end - i0),
+                         hipMemcpyDeviceToHost));
+        for (const LightTok& t : light_toks) {
+            const unsigned long long hi = h[2ull * (t.it - i0)];
+            const double nf = static_cast<double>(static_cast<uint32_t>(hi));
+            const double E = static_cast<double>(hi >> 32);
+            const double out = t.it + 1 < it_end ? static_cast<double>(static_cast<uint32_t>(h[2ull * (t.it + 1 - i0)]))
+                                                 : 0.0;
+            if (t.tok_c >= 0) {
+                prof_set_bytes(g, t.tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E);
+                prof_set_bytes(g, t.tok_c, 8.0 * n + 16.0 * out);
+            } else {
+                prof_set_bytes(g, t.tok, nf == 0.0 ? 0.0 : 16.0 * nf + 12.0 * E + 4.0 * out);
+            }
         }
+        light_toks.clear();
         return G1_OK;
     };
     auto light_small = [&](uint32_t it0, uint32_t* res) -> g1_status {
         SsspPush pol{g->dstamp, 0, db, limit};
-        return run_small(g, s, pol, g->dl_w, nullptr, 0, it0, res, lrows, g->dl_col);
+        tok = prof_begin(g, "sssp_delta_light_small", s);
+        const g1_status st_small = run_small(g, s, pol, g->dl_w, nullptr, 0, it0, res, lrows, g->dl_col);
+        prof_end(g, tok, s, 0.0);
+        return st_small;
     };
     auto no_pull = [](uint32_t) -> g1_status { return G1_OK; };
     // A distance scan that writes the input frontier of iteration `it` (the queues of iteration it - 1,
@@ -959,7 +984,14 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
     };
 
     uint64_t lo = 0;
-    bool small_first = true;         // the seed
+    // The seed's bucket starts on the advance kernel, whatever the seed's degree (from a hub, the
+    // single-workgroup kernel spends ~45 us on the first iteration's few thousand edges before handing the
+    // frontier back), and light iterations go in device-driven batches of kLightBatch launches (a
+    // readback between batches leaves the GPU idle ~25 us; a launch past the frontier's end costs ~5 us).
+    // RMAT22 from the hub, SSSP: 1.238 ms with the small kernel first and batches of 2, 1.191 without it,
+    // 1.176 / 1.169 / 1.164 with batches of 4 / 8 / 12.
+    bool small_first = false;
+    constexpr uint32_t kLightBatch = 8;
     uint64_t first_edges = ~0ull;    // the light frontier's edge count, when a scan read it
     unsigned long long fr = 0;
     uint32_t mn = kInf;
@@ -969,7 +1001,8 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
         uint32_t launched = it;
         if (first_edges != 0)
             G1_TRY(drive_push(g, s, light_launch, light_after, light_small, small_first, &launched, no_pull, 0, it,
-                              first_edges));
+                              first_edges, 0, kLightBatch, true));
+            G1_TRY(light_bytes(launched));
         it = launched;
         G1_CHECK(it + 2 <= kHistIters, G1_ERR_INTERNAL, "delta engine exceeded %u iterations", kHistIters);
         // heavy phase: the bucket's vertices with heavy edges
