@@ -146,3 +146,17 @@ def test_noalloc_spmv_rejects_oversize_before_device_work():
     st = L.g1_spmv_merge_minplus_u32_noalloc(n_rows, n_rows, nnz, None, None, None, None, None, None)
     assert st == _lib.G1_ERR_INVALID_ARG
     assert b"scratch" in L.g1_last_error_message()
+
+
+def test_header_constants_match_bindings():
+    """Every integer #define G1_* of include/graphone_hip.h that the Python layer also names has the same
+    value there (engine ids, status codes, modes)."""
+    import os
+    import re
+    header = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "graphone_hip.h")
+    defines = dict(re.findall(r"^#define (G1_[A-Z0-9_]+) (-?\d+)\s*$", open(header).read(), flags=re.M))
+    named = {k: v for k, v in vars(_lib).items() if k.startswith("G1_") and isinstance(v, int)}
+    shared = sorted(set(defines) & set(named))
+    assert "G1_ENGINE_DELTA" in shared and "G1_ENGINE_ADVANCE" in shared
+    for k in shared:
+        assert int(defines[k]) == named[k], k
