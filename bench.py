@@ -187,8 +187,9 @@ def batched_queries(g1, g, host, count: int, seed: int) -> dict:
     """Multi-source queries on the same graph: `count` SSSP sources and `count` personalized-PageRank seeds
     (restart 0.15), drawn uniformly without replacement from the vertices with out-degree > 0. Each runs
     once as a batch (columns of one SpMM, every width timed, the autotuned one reported) and once as
-    sequential single-source calls (SSSP: the push engine, the fastest single-source engine; PPR: g1_ppr);
-    every batched column is compared with its single-source result."""
+    sequential single-source calls (SSSP: the advance engine, the reference's plain frontier loop, and the
+    delta-stepping engine, the fastest single-source engine; PPR: g1_ppr); every batched column is compared
+    with its single-source result."""
     rng = np.random.default_rng(seed)
     deg = host.out_degrees()
     srcs = rng.choice(np.flatnonzero(deg > 0), size=count, replace=False).astype(np.uint32)
@@ -207,6 +208,8 @@ def batched_queries(g1, g, host, count: int, seed: int) -> dict:
     t_batch = _wall(lambda: g.sssp_batch(srcs, k=k_auto, out=dist, to_host=False))
     g.sssp(int(srcs[0]), "advance", out=one, to_host=False)
     t_seq = _wall(lambda: [g.sssp(int(s), "advance", out=one, to_host=False) for s in srcs])
+    g.sssp(int(srcs[0]), "delta", out=one, to_host=False)
+    t_seq_delta = _wall(lambda: [g.sssp(int(s), "delta", out=one, to_host=False) for s in srcs])
     allcols = dist.numpy().reshape(count, n)
     same = sum(int(np.array_equal(allcols[c], g.sssp(int(s), "advance", out=one).values))
                for c, s in enumerate(srcs))
@@ -216,6 +219,8 @@ def batched_queries(g1, g, host, count: int, seed: int) -> dict:
         "batch_ms": round(t_batch * 1e3, 3), "sequential_ms": round(t_seq * 1e3, 3),
         "batch_qps": round(count / t_batch, 1), "sequential_qps": round(count / t_seq, 1),
         "speedup": round(t_seq / t_batch, 3), "iterations_sum": res.stats["iterations_sum"],
+        "sequential_delta_ms": round(t_seq_delta * 1e3, 3), "sequential_delta_qps": round(count / t_seq_delta, 1),
+        "speedup_vs_delta": round(t_seq_delta / t_batch, 3),
         "columns_bit_exact": f"{same}/{count}",
     }
     del dist, allcols
