@@ -273,6 +273,16 @@ class Graph:
         call("g1_graph_info", self._h, None, None, C.byref(b))
         return b.value
 
+    def work_bytes(self, work: int) -> int:
+        """Device bytes the engines' work areas in `work` (G1_WORK_* bits) would add to this handle now."""
+        b = C.c_uint64()
+        call("g1_graph_work_bytes", self._h, int(work), C.byref(b))
+        return b.value
+
+    def prepare(self, work: int, stream=None) -> None:
+        """Allocates and builds the work areas in `work` now (G1_WORK_* bits)."""
+        call("g1_graph_prepare", self._h, int(work), stream)
+
     def csr_dev(self, transpose: bool = False):
         ro, ci, w = C.c_void_p(), C.c_void_p(), C.c_void_p()
         call("g1_graph_csr_dev", self._h, int(transpose), C.byref(ro), C.byref(ci), C.byref(w))

@@ -34,6 +34,14 @@ G1_ENGINE_DELTA = 4
 G1_PR_L1_F64 = 0
 G1_PR_MAX_ABS = 1
 
+G1_WORK_PUSH = 1
+G1_WORK_PULL = 2
+G1_WORK_RANK = 4
+G1_WORK_RELABEL = 8
+G1_WORK_DELTA = 16
+G1_WORK_ROAD = 32
+G1_WORK_ALL = 63
+
 G1_BATCH_MINPLUS = 0
 G1_BATCH_PPR = 1
 
@@ -215,6 +223,8 @@ _SIGNATURES = {
     "g1_graph_destroy": (_st, [_vp]),
     "g1_graph_create_view": (_st, [_vp, C.POINTER(_vp)]),
     "g1_graph_info": (_st, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u64)]),
+    "g1_graph_work_bytes": (_st, [_vp, _i32, C.POINTER(_u64)]),
+    "g1_graph_prepare": (_st, [_vp, _i32, _vp]),
     "g1_graph_csr_dev": (_st, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "g1_profile_enable": (_st, [_vp, _i32]),
     "g1_profile_read": (_st, [_vp, C.POINTER(KernelProfile), _u32, C.POINTER(_u32)]),

@@ -508,6 +508,12 @@ uint32_t road_delta(const g1_graph* g, bool lds) {
 
 constexpr uint32_t kRoadSlots = 64;  // queries per batched launch
 
+uint64_t road_work_bytes(const g1_graph* g) {
+    if (g->road_q) return 0;
+    const uint64_t cap = 2ull * g->n + 1024;
+    return 2 * cap * sizeof(uint32_t) + kRoadSlots * sizeof(RoadQuery) + kRoadSlots * sizeof(uint32_t*);
+}
+
 g1_status ensure_road_work(g1_graph* g, hipStream_t s) {
     G1_TRY(ensure_push_work(g, s));
     if (g->road_q) return G1_OK;
@@ -1280,6 +1286,9 @@ g1_status check_run(g1_graph* g, uint32_t src, const void* out, g1_run_stats* st
 }  // namespace
 
 bool is_road_graph(const g1_graph* g) { return road_graph(g); }
+g1_status prepare_road_work(g1_graph* g, hipStream_t s) { return ensure_road_work(g, s); }
+uint64_t road_work_size(const g1_graph* g) { return road_work_bytes(g); }
+uint32_t delta_width(const g1_graph* g) { return sssp_delta_width(g); }
 
 g1_status road_queries(g1_graph* g, const uint32_t* kinds, const uint32_t* sources, uint32_t count, uint32_t** outs,
                        uint8_t* ok, uint32_t* iters, hipStream_t s) {

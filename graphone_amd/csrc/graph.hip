@@ -55,6 +55,8 @@ g1_status ensure_push_work(g1_graph* g, hipStream_t s) {
     G1_TRY(alloc_n(g, &g->qcnt, 3 * kSegments));
     G1_TRY(alloc_n(g, &g->slots, 4 * kMaxPushBlocks));  // four regions of per-workgroup stats
     G1_TRY(alloc_n(g, &g->overflow, 4));
+    if (!g->h_hist)  // the host copy of the iteration history (pinned)
+        G1_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->h_hist), 2ull * kHistIters * sizeof(unsigned long long)));
     // Cleared on the caller's stream, ahead of its first launch: a memset on the null stream is not
     // ordered before work on a non-blocking stream, and the allocation may hold an earlier buffer's data.
     const g1_stream_t gs = reinterpret_cast<g1_stream_t>(s);
@@ -439,6 +441,95 @@ g1_status g1_profile_read(g1_graph* g, g1_kernel_profile* out, uint32_t capacity
         out[i].algorithmic_bytes = g->prof[i].bytes;
     }
     g->prof.clear();
+    return G1_OK;
+}
+
+// ------------------------------------------------------------------ work areas: size query and prepare
+
+namespace g1 {
+
+// Device bytes each ensure_* call would allocate now (0 for what the handle already holds). The long-row
+// lists of the CSR-vector engine depend on the data: counted at their bound, 4 bytes per vertex.
+static uint64_t push_work_bytes(const g1_graph* g) {
+    if (g->q[0]) return 0;
+    const uint64_t n1 = g->n ? g->n : 1;
+    const uint64_t words = (n1 + 31) / 32 + 1;
+    const uint64_t items = g->m > g->n ? g->m : g->n;
+    const uint64_t cap = push_segment_capacity(n1, items);
+    return 4 * cap * kSegments * 4 + 2 * words * 4 + n1 * 4 + 3 * words * 4 + n1 * 8 + (n1 + 256) +
+           2ull * kHistIters * 8 + 3ull * kSegments * 8 + 4ull * kMaxPushBlocks * 8 + 4 * 4;
+}
+static uint64_t long_rows_bound(const g1_graph* g) { return 4ull * (g->n ? g->n : 1); }
+static uint64_t pull_work_bytes(const g1_graph* g) {
+    const uint64_t n1 = g->n ? g->n : 1;
+    uint64_t b = g->xu ? 0 : 2 * n1 * 4;
+    if (g->has_t && !g->t_long) b += long_rows_bound(g);
+    return b;
+}
+static uint64_t rank_work_bytes(const g1_graph* g) {
+    const uint64_t n1 = g->n ? g->n : 1;
+    uint64_t b = g->xu ? 0 : 2 * n1 * 4;
+    if (!g->xf) b += 5 * n1 * 4;
+    if (!g->osum) b += n1 * 4;
+    if (g->t_w && !g->t_wf) b += (static_cast<uint64_t>(g->m) + kCsrPad) * 4;
+    if (!g->rpart) b += (static_cast<uint64_t>(rank_blocks(static_cast<uint32_t>(n1))) * kBatchCols + kBatchCols + 8) * 8 +
+                        kBatchCols * 4;
+    return b;
+}
+static uint64_t relabel_work_bytes(const g1_graph* g) {
+    if (g->pr_perm || !g->has_t || g->is_view || g->n == 0) return 0;
+    const uint64_t n1 = g->n, m = g->m;
+    uint64_t b = n1 * 4 + (n1 + 1) * 4 + (m + kCsrPad) * 4 + n1 * 4 + merge_workspace_bytes(g->n, g->m);
+    if (g->t_w) b += (m + kCsrPad) * 4;
+    return b + long_rows_bound(g);
+}
+static uint64_t delta_work_bytes(const g1_graph* g) {
+    if (g->dl_col || !g->w) return 0;
+    const uint64_t m1 = g->m ? g->m : 1, n1 = g->n ? g->n : 1;
+    return 2 * m1 * 4 + n1 * 4 + kMaxPushBlocks * 4 + (static_cast<uint64_t>(g->n) + 3) / 4 * 4 + 4;
+}
+
+}  // namespace g1
+
+g1_status g1_graph_work_bytes(const g1_graph* g, int32_t work, uint64_t* bytes) {
+    using namespace g1;
+    G1_CHECK(g != nullptr && bytes != nullptr, G1_ERR_INVALID_ARG, "g1_graph_work_bytes: null argument");
+    G1_CHECK((work & ~G1_WORK_ALL) == 0 && work != 0, G1_ERR_INVALID_ARG, "g1_graph_work_bytes: unknown work set %d",
+             work);
+    if (work & G1_WORK_RELABEL) work |= G1_WORK_RANK;  // the relabel needs the rank vectors' out-weight sums
+    std::lock_guard<std::mutex> lock(const_cast<g1_graph*>(g)->mu);
+    uint64_t b = 0;
+    const bool push = (work & (G1_WORK_PUSH | G1_WORK_ROAD)) != 0;
+    if (push) b += push_work_bytes(g);
+    if (work & G1_WORK_ROAD) b += road_work_size(g);
+    if (work & G1_WORK_RANK) b += rank_work_bytes(g);
+    if ((work & G1_WORK_PULL) && !(work & G1_WORK_RANK)) b += pull_work_bytes(g);
+    if ((work & G1_WORK_PULL) && (work & G1_WORK_RANK) && g->has_t && !g->t_long) b += long_rows_bound(g);
+    if (work & G1_WORK_RELABEL) b += relabel_work_bytes(g);
+    if (work & G1_WORK_DELTA) b += delta_work_bytes(g);
+    *bytes = b;
+    return G1_OK;
+}
+
+g1_status g1_graph_prepare(g1_graph* g, int32_t work, g1_stream_t stream) {
+    using namespace g1;
+    G1_CHECK(g != nullptr, G1_ERR_INVALID_ARG, "g1_graph_prepare: null graph");
+    G1_CHECK((work & ~G1_WORK_ALL) == 0 && work != 0, G1_ERR_INVALID_ARG, "g1_graph_prepare: unknown work set %d",
+             work);
+    if (work & G1_WORK_RELABEL) work |= G1_WORK_RANK;
+    std::lock_guard<std::mutex> lock(g->mu);
+    const hipStream_t s = as_stream(stream);
+    if (work & (G1_WORK_PUSH | G1_WORK_ROAD)) G1_TRY(ensure_push_work(g, s));
+    if (work & G1_WORK_ROAD) G1_TRY(prepare_road_work(g, s));
+    if (work & (G1_WORK_PULL | G1_WORK_RANK)) G1_TRY(ensure_pull_work(g, (work & G1_WORK_RANK) != 0));
+    if (work & G1_WORK_RANK) G1_TRY(ensure_rank_partials(g));
+    if ((work & G1_WORK_PULL) && g->has_t) G1_TRY(ensure_long_rows(g, false, s));
+    if ((work & G1_WORK_RELABEL) && g->has_t && !g->is_view && g->n > 0) {
+        G1_TRY(ensure_pr_relabel(g, s));
+        G1_TRY(ensure_long_rows(g, true, s));
+    }
+    if ((work & G1_WORK_DELTA) && g->w) G1_TRY(ensure_delta_split(g, delta_width(g), s));
+    G1_HIP(hipStreamSynchronize(s));
     return G1_OK;
 }
 

@@ -156,6 +156,10 @@ g1_status device_transpose(uint32_t n, uint32_t m, const uint32_t* row_off, cons
 g1_status exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* partials, hipStream_t s);
 g1_status device_u32_to_f32(const uint32_t* in, float* out, size_t count, hipStream_t s);
 g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s);
+// engines.hip: the road engine's work areas (allocate / bytes they would add) and the delta engine's width
+g1_status prepare_road_work(g1_graph* g, hipStream_t s);
+uint64_t road_work_size(const g1_graph* g);
+uint32_t delta_width(const g1_graph* g);
 
 // Profiling: prof_begin records a start event (returns -1 when profiling is off), prof_end the stop
 // event with the launch's algorithmic bytes; prof_flush (after the stream is synchronized) adds the

@@ -290,6 +290,34 @@ g1_status g1_graph_destroy(g1_graph* graph);
  * No counterpart in the reference (one graph, one engine per process). */
 g1_status g1_graph_create_view(g1_graph* base, g1_graph** view);
 g1_status g1_graph_info(const g1_graph* graph, uint32_t* n, uint32_t* m, uint64_t* device_bytes);
+
+/* Work areas. A handle allocates the device memory an engine works in on that engine's first run and
+ * keeps it until it is destroyed (the reference's engines allocate theirs when constructed and grow the
+ * output frontier per iteration: engine/advance_gc.h:177-188, engine/blas_gc.h:137-141,
+ * frontier/sparse_dbl.h:66-90). No counterpart in the reference: g1_graph_work_bytes returns how
+ * many bytes the given set would add now (0 for what the handle already holds; the CSR-vector engine's
+ * long-row lists, whose size depends on the data, at their bound of 4 bytes per vertex), and
+ * g1_graph_prepare allocates and builds them on `stream` (synchronously), so a caller can budget and
+ * place the allocation. g1_graph_info's device_bytes is the handle's total.
+ *   G1_WORK_PUSH     queues, counters, distance words and history of the push engines (every
+ *                    BFS / SSSP push engine; also the stats of the SpMV engines)
+ *   G1_WORK_PULL     SpMV x / y vectors (min-plus engines, the push engines' pull steps) and the
+ *                    in-edge CSR's long-row list
+ *   G1_WORK_RANK     PageRank / PPR vectors, out-weight sums, float weights, L1 partials
+ *   G1_WORK_RELABEL  plain PageRank's hub-first relabelled in-edge copy (implies G1_WORK_RANK; none for a
+ *                    view, which keeps the graph's ids)
+ *   G1_WORK_DELTA    the delta-stepping engine's weight-split CSR copy (the engine also needs PUSH | PULL)
+ *   G1_WORK_ROAD     the road engine's far lists and query records (implies G1_WORK_PUSH)
+ * Unknown bits are rejected with G1_ERR_INVALID_ARG. */
+#define G1_WORK_PUSH 1
+#define G1_WORK_PULL 2
+#define G1_WORK_RANK 4
+#define G1_WORK_RELABEL 8
+#define G1_WORK_DELTA 16
+#define G1_WORK_ROAD 32
+#define G1_WORK_ALL 63
+g1_status g1_graph_work_bytes(const g1_graph* graph, int32_t work, uint64_t* bytes);
+g1_status g1_graph_prepare(g1_graph* graph, int32_t work, g1_stream_t stream);
 g1_status g1_graph_csr_dev(const g1_graph* graph, int32_t transpose, const uint32_t** row_offsets_dev,
                            const uint32_t** col_indices_dev, const uint32_t** weights_dev);
 

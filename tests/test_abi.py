@@ -160,3 +160,11 @@ def test_header_constants_match_bindings():
     assert "G1_ENGINE_DELTA" in shared and "G1_ENGINE_ADVANCE" in shared
     for k in shared:
         assert int(defines[k]) == named[k], k
+
+
+def test_work_area_calls_reject_bad_arguments():
+    L = _lib.lib()
+    b = C.c_uint64()
+    assert L.g1_graph_work_bytes(None, _lib.G1_WORK_PUSH, C.byref(b)) == _lib.G1_ERR_INVALID_ARG
+    assert L.g1_graph_prepare(None, _lib.G1_WORK_PUSH, None) == _lib.G1_ERR_INVALID_ARG
+    assert b"null" in L.g1_last_error_message()

@@ -218,6 +218,45 @@ def test_pagerank_rmat20_l1(gpu):
     assert abs(got.stats["iterations"] - ost["iterations"]) <= 1
 
 
+def test_work_areas_sized_and_prepared_ahead(gpu):
+    """g1_graph_work_bytes gives the bytes each engine's work areas add (exact, the long-row lists within
+    their bound), g1_graph_prepare allocates them, and afterwards the engines run without allocating."""
+    import graphone_amd as g1
+    from graphone_amd import _lib
+    h = g1.HostGraph.rmat(14, 16, seed=51).assign_weights(1, 255, seed=52)
+    g = g1.Graph(h)
+    n = h.n
+    for work, bound in ((_lib.G1_WORK_PUSH, 0), (_lib.G1_WORK_PULL, 4 * n), (_lib.G1_WORK_RANK, 0),
+                        (_lib.G1_WORK_RELABEL, 4 * n), (_lib.G1_WORK_DELTA, 0)):
+        want = g.work_bytes(work)
+        before = g.device_bytes()
+        g.prepare(work)
+        grew = g.device_bytes() - before
+        assert grew <= want <= grew + bound, (work, want, grew)
+        assert g.work_bytes(work) == 0, work
+    assert g.work_bytes(_lib.G1_WORK_ALL & ~_lib.G1_WORK_ROAD) == 0
+    total = g.device_bytes()
+    src = int(np.argmax(h.out_degrees()))
+    for eng in ("advance", "delta", "merge", "csrvec"):
+        g.sssp(src, eng)
+    g.bfs(src, "diropt")
+    g.pagerank(0.85, 20, 1e-7, "l1", "merge")
+    g.pagerank(0.85, 20, 1e-7, "l1", "csrvec")
+    g.pagerank(0.85, 20, 1e-6, "max", "merge")
+    g.ppr(src, 0.15, 20, 1e-7)
+    assert g.device_bytes() == total
+    # a road-like graph: the road engine's work
+    r = g1.Graph(g1.HostGraph.grid(32, 32, 0.7, seed=53).assign_weights(1, 9, seed=54, symmetric=True), symmetric=True)
+    want = r.work_bytes(_lib.G1_WORK_ROAD)
+    before = r.device_bytes()
+    r.prepare(_lib.G1_WORK_ROAD)
+    assert r.device_bytes() - before == want
+    r.sssp(3, "advance")
+    assert r.device_bytes() - before == want
+    with pytest.raises(g1.G1Error):
+        g.prepare(128)
+
+
 def test_views_run_concurrently(graphs):
     """Two views of one graph run SSSP (with its pull steps) and personalized PageRank on their own
     streams from two threads at once; every result equals the serial run (views own their work areas,
