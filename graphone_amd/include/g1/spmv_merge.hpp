@@ -156,6 +156,36 @@ __device__ __forceinline__ void merge_to_lds(const MergeStage<S>& st, uint32_t* 
     if (tid == 0) s_rowend[rows] = kInf;  // no further row end inside this tile
 }
 
+// The last thread below t that closed a row (-1 if none), from the waves' ballots of the closing threads
+// (masks[2w], masks[2w + 1]: the 64 flags of wave w): at most kMergeThreads / 64 words read.
+__device__ __forceinline__ int merge_last_end_before(const uint32_t* masks, uint32_t t) {
+    for (int w = static_cast<int>((t - 1) / kWave); t > 0 && w >= 0; --w) {
+        uint64_t m = (static_cast<uint64_t>(masks[2 * w + 1]) << 32) | masks[2 * w];
+        const uint32_t below = t - static_cast<uint32_t>(w) * kWave;  // flags of lanes < below count
+        if (below < static_cast<uint32_t>(kWave)) m &= (1ull << below) - 1ull;
+        if (m) return w * kWave + 63 - __clzll(m);
+    }
+    return -1;
+}
+
+// The left-to-right fold of the tails of threads [v0, v1): the loads issued eight at a time ahead of their
+// merges, so a fold across a long row does not wait on each LDS read in turn (same order, same result).
+template <typename S>
+__device__ __forceinline__ typename S::acc_t merge_fold_tails(const typename S::acc_t* s_tail, uint32_t v0,
+                                                              uint32_t v1) {
+    typename S::acc_t chain = S::identity();
+    uint32_t v = v0;
+    for (; v + 8 <= v1; v += 8) {
+        typename S::acc_t t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t[k] = s_tail[v + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) chain = S::merge(chain, t[k]);
+    }
+    for (; v < v1; ++v) chain = S::merge(chain, s_tail[v]);
+    return chain;
+}
+
 // The walk of one staged tile (LDS filled, after a barrier): each thread's kMergeItems steps, the rows
 // it closes, the tails chain, and the tile record. Contains one barrier; every thread must call it.
 template <typename S>
@@ -206,14 +236,17 @@ __device__ __forceinline__ void merge_walk(uint32_t tile, MergeCoord c0, MergeCo
         }
     }
     s_tail[tid] = acc;
-    s_hasend[tid] = hasend ? 1u : 0u;
+    // which threads closed a row: one 64-bit ballot per wave, as two words (s_hasend[2w], s_hasend[2w + 1])
+    const uint64_t ends = ballot(hasend);
+    if (lane_id() == 0) {
+        s_hasend[2 * (tid / kWave)] = static_cast<uint32_t>(ends);
+        s_hasend[2 * (tid / kWave) + 1] = static_cast<uint32_t>(ends >> 32);
+    }
     __syncthreads();
 
     if (hasend) {
-        int u = static_cast<int>(tid) - 1;
-        while (u >= 0 && s_hasend[u] == 0u) --u;
-        acc_t chain = S::identity();
-        for (int v = (u >= 0 ? u : 0); v < static_cast<int>(tid); ++v) chain = S::merge(chain, s_tail[v]);
+        const int u = merge_last_end_before(s_hasend, tid);
+        const acc_t chain = merge_fold_tails<S>(s_tail, u >= 0 ? static_cast<uint32_t>(u) : 0u, tid);
         const acc_t whole = S::merge(chain, head);
         // The tile's first closed row began in an earlier tile: leave it for the fixup kernel.
         if (u < 0 && row_offsets[first_row] < c0.y) {
@@ -223,10 +256,8 @@ __device__ __forceinline__ void merge_walk(uint32_t tile, MergeCoord c0, MergeCo
         }
     }
     if (tid == kMergeThreads - 1) {
-        int u = kMergeThreads - 1;
-        while (u >= 0 && s_hasend[u] == 0u) --u;
-        acc_t carry = S::identity();
-        for (int v = (u >= 0 ? u : 0); v < kMergeThreads; ++v) carry = S::merge(carry, s_tail[v]);
+        const int u = merge_last_end_before(s_hasend, kMergeThreads);
+        const acc_t carry = merge_fold_tails<S>(s_tail, u >= 0 ? static_cast<uint32_t>(u) : 0u, kMergeThreads);
         recs[tile].carry = carry;
         uint32_t flags = (u >= 0) ? kTileHasEnd : 0u;
         // Deferred exactly when the tile's first closed row (row c0.x) began in an earlier tile; the
