@@ -1014,6 +1014,10 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
         // heavy phase: the bucket's vertices with heavy edges
         G1_TRY(scan(hrows, lo, limit, &fr, &mn));
         const uint64_t heavy_edges = fr >> 32;
+        // after this heavy phase the engine leaves the buckets for the plain loop (see below); a pull step
+        // then builds that loop's first frontier in its apply instead of a separate scan
+        const bool leaving = limit != kInf && (heavy_edges >= switch_edges || buckets >= kDeltaMaxBuckets);
+        bool fused = false;
         if (static_cast<uint32_t>(fr) != 0) {
             PushQueues pq = make_queues(g, it);
             if (heavy_edges >= pull_min && limit - lo <= 255u) {
@@ -1025,12 +1029,13 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
                 G1_LAUNCHED();
                 G1_TRY(launch_spmv_merge<MinPlusU8X>(n, g->m, g->t_row_off, g->t_col, g->t_w, g->dl_xb, g->yu,
                                                      g->t_ws, s));
-                sssp_pull_apply_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(n, g->row_off, g->yu, g->dstamp, it + 1,
-                                                                             pq, db, static_cast<uint32_t>(lo));
+                sssp_pull_apply_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(
+                    n, g->row_off, g->yu, g->dstamp, it + 1, pq, db, static_cast<uint32_t>(lo), leaving ? limit : kInf);
                 G1_LAUNCHED();
                 // x 8n + n; SpMV 4(n+1) + 8m + n (x) + 4n (y); apply 4n (y) + 16n (dist word) + 8n (degrees)
                 prof_end(g, tok, s, 4.0 * (n + 1.0) + 8.0 * g->m + 42.0 * n);
                 ++pulls;
+                fused = leaving;
             } else if (heavy_edges >= pull_min) {
                 // x = the bucket's distances (the scan stamped them `it`), y = A^T (min,+) x, apply
                 tok = prof_begin(g, "sssp_pull", s);
@@ -1056,9 +1061,11 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
         }
         if (limit == kInf) break;
         lo = limit;
-        if (heavy_edges >= switch_edges || buckets >= kDeltaMaxBuckets) {
-            // every vertex at or above lo with a finite distance, with all its edges, for the plain loop
-            G1_TRY(scan(FullRows(g->row_off), lo, kInf, &fr, &mn));
+        if (leaving) {
+            // every vertex at or above lo with a finite distance, with all its edges, for the plain loop:
+            // already enqueued by the pull step's apply, or from a scan
+            if (fused) G1_TRY(read_frontier(g, it % 3, s, &fr));
+            else G1_TRY(scan(FullRows(g->row_off), lo, kInf, &fr, &mn));
             if (static_cast<uint32_t>(fr) != 0) {
                 uint32_t launched2 = it;
                 G1_TRY(sssp_push_iterations(g, s, db, it, (fr >> 32) <= kSmallCap, &launched2, &dense, &pulls,

@@ -727,11 +727,13 @@ sssp_pull_xb_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, u
 
 // Apply: every vertex whose y (plus y_add, saturating) beats its distance takes (y + y_add, stamp) and, with
 // out-edges, joins the next frontier (segment blockIdx % kSegments of the output queue). Each vertex has
-// one owner, so no atomics.
+// one owner, so no atomics. With frontier_lo below kInf (the delta engine's last heavy phase) the next
+// frontier is instead every vertex whose distance, after the apply, is finite and at least frontier_lo,
+// each restamped with `stamp`: the frontier its distance scan would build.
 static __global__ void __launch_bounds__(kPushThreads)
 sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ y,
                        unsigned long long* __restrict__ dstamp, uint32_t stamp, PushQueues pq, DistBound db,
-                       uint32_t y_add = 0) {
+                       uint32_t y_add = 0, uint32_t frontier_lo = kInf) {
     __shared__ unsigned long long s_scan[kPushThreads / kWave];
     __shared__ unsigned long long s_base;
     __shared__ uint32_t s_red[kPushThreads / kWave];
@@ -758,10 +760,16 @@ sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const u
             const uint32_t v = base + r * kPushThreads + threadIdx.x;
             cand[r] = kInf;
             cdeg[r] = 0;
-            if (v < n && yv[r] < static_cast<uint32_t>(ds[r] >> 32)) {
-                dstamp[v] = (static_cast<unsigned long long>(yv[r]) << 32) | stamp;
+            const uint32_t d_old = static_cast<uint32_t>(ds[r] >> 32);
+            const bool improved = v < n && yv[r] < d_old;
+            const uint32_t d_new = improved ? yv[r] : d_old;
+            const bool enq = frontier_lo == kInf ? improved : (v < n && d_new >= frontier_lo && d_new != kInf);
+            if (improved) {
                 db.lower(v, yv[r]);
                 ++activated;
+            }
+            if (improved || enq) dstamp[v] = (static_cast<unsigned long long>(d_new) << 32) | stamp;
+            if (enq) {
                 const uint32_t dg = row_off[v + 1] - row_off[v];
                 if (dg > 0) {
                     cand[r] = v;
