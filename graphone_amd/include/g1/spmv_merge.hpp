@@ -302,10 +302,33 @@ __global__ void __launch_bounds__(256) spmv_merge_fixup_kernel(uint32_t num_tile
     if (t >= num_tiles || t == 0) return;
     if ((recs[t].flags & kTileDeferred) == 0u) return;
     const uint32_t row = coords[t].x;
+    // the nearest tile u < t that closes a row (or tile 0), eight flags loaded per step: a hub row crosses
+    // dozens of tiles, and one dependent load per tile would be as many round trips
     uint32_t u = t - 1;
-    while (u > 0 && (recs[u].flags & kTileHasEnd) == 0u) --u;
+    for (;;) {
+        uint32_t f[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) f[k] = k < u ? recs[u - k].flags : kTileHasEnd;  // tile 0 ends the search
+        uint32_t hit = 8;
+#pragma unroll
+        for (int k = 7; k >= 0; --k)
+            if (f[k] & kTileHasEnd) hit = static_cast<uint32_t>(k);
+        if (hit < 8) {
+            u -= hit;
+            break;
+        }
+        u -= 8;  // u > 7 here: otherwise k = u (tile 0) would have hit
+    }
     acc_t acc = S::identity();
-    for (uint32_t v = u; v < t; ++v) acc = S::merge(acc, recs[v].carry);
+    uint32_t v = u;
+    for (; v + 8 <= t; v += 8) {
+        acc_t cv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cv[k] = recs[v + k].carry;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc = S::merge(acc, cv[k]);
+    }
+    for (; v < t; ++v) acc = S::merge(acc, recs[v].carry);
     acc = S::merge(acc, recs[t].head);
     y[row] = S::finalize(acc);
 }
