@@ -857,6 +857,40 @@ double sssp_delta_noreturn() {
     return v >= 0.0 ? v : 0.3;
 }
 
+// The heavy pull's form: over the rows whose distance is at or past the bucket's end only
+// (sssp_pull_rows_kernel, G1_SSSP_DELTA_PULL_EDGES = 1, 4, 8 or 16 edges per lane per round, default 1:
+// each round's column and weight loads are then fully coalesced; on RMAT22 from the hub the kernel took
+// 102 / 113 / 125 us at 1 / 4 / 8), or with G1_SSSP_DELTA_PULL_ROWS=0 the merge-path SpMV over every row.
+bool sssp_delta_pull_rows() {
+    const char* e = getenv("G1_SSSP_DELTA_PULL_ROWS");
+    return !(e && atoi(e) == 0);
+}
+uint32_t sssp_delta_pull_edges() {
+    const char* e = getenv("G1_SSSP_DELTA_PULL_EDGES");
+    const int v = e ? atoi(e) : 0;
+    return (v == 1 || v == 4 || v == 8 || v == 16) ? static_cast<uint32_t>(v) : 1u;
+}
+
+// The delta engine's heavy pull over the rows at or past the bucket's end (sssp_pull_rows_kernel): x in
+// dl_xb, y in yu; rows longer than huge_min are the long-row list's first t_nhuge entries.
+g1_status launch_pull_rows(g1_graph* g, uint32_t edges, hipStream_t s) {
+    const uint32_t blocks = g->t_nhuge + (g->n + kPullRowsPerBlock - 1) / kPullRowsPerBlock;
+#define G1_PULL_ROWS_CASE(E_)                                                                                     \
+    case E_:                                                                                                     \
+        sssp_pull_rows_kernel<E_><<<blocks, kPushThreads, 0, s>>>(g->n, g->t_row_off, g->t_col, g->t_w, g->dl_xb, \
+                                                                 g->yu, g->huge_min, g->t_long, g->t_nhuge);     \
+        break
+    switch (edges) {
+        G1_PULL_ROWS_CASE(4);
+        G1_PULL_ROWS_CASE(8);
+        G1_PULL_ROWS_CASE(16);
+        default: G1_PULL_ROWS_CASE(1);
+    }
+#undef G1_PULL_ROWS_CASE
+    G1_LAUNCHED();
+    return G1_OK;
+}
+
 // Zeroes queue counter sets a and b (kSegments counters each).
 __global__ void __launch_bounds__(128) clear_sets_kernel(unsigned long long* qcnt, uint32_t a, uint32_t b) {
     const uint32_t t = threadIdx.x;
@@ -889,7 +923,10 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
     const LightRows lrows{g->row_off, g->dl_lc};
     const HeavyRows hrows{g->row_off, g->dl_lc};
     const bool can_pull = g->has_t && g->t_w != nullptr && sssp_pull_fraction() > 0.0;
+    const bool pull_rows = sssp_delta_pull_rows();
+    const uint32_t pull_edges = sssp_delta_pull_edges();
     if (can_pull) G1_TRY(ensure_pull_work(g, false));
+    if (can_pull && pull_rows) G1_TRY(ensure_long_rows(g, false, s));
     const uint64_t pull_min =
         can_pull ? std::max<uint64_t>(static_cast<uint64_t>(sssp_pull_fraction() * g->m), 1) : ~0ull;
     const uint64_t noreturn_min =
@@ -1020,20 +1057,44 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
         bool fused = false;
         if (static_cast<uint32_t>(fr) != 0) {
             PushQueues pq = make_queues(g, it);
-            if (heavy_edges >= pull_min && limit - lo <= 255u) {
-                // x = the bucket's distances less lo, one byte each (255: not in the bucket), y = A^T (min,+) x,
-                // apply with lo added back
+            if (heavy_edges >= pull_min && limit - lo <= 254u) {
+                // x = the bucket's distances less lo, one byte each (254 below the bucket, 255 past it),
+                // y = A^T (min,+) x over the rows at or past the bucket's end, apply with lo added back
+                double bytes = 4.0 * (n + 1.0) + 8.0 * g->m + 42.0 * n;
+                if (g->profile && pull_rows && n >= 512) {  // 2 words per counting workgroup fit in y
+                    // the rows the pull visits and their in-edges, counted outside its timed region
+                    const uint32_t cb = grid_for(n, kPushThreads, kMaxPushBlocks);
+                    sssp_pull_rows_count_kernel<<<cb, kPushThreads, 0, s>>>(n, g->dstamp, limit, g->t_row_off,
+                                                                             g->yu);
+                    G1_LAUNCHED();
+                    std::vector<uint32_t> part(2ull * cb);
+                    G1_HIP(hipMemcpyAsync(part.data(), g->yu, sizeof(uint32_t) * part.size(), hipMemcpyDeviceToHost,
+                                          s));
+                    G1_HIP(hipStreamSynchronize(s));
+                    double rows = 0.0, edges = 0.0;
+                    for (uint32_t b = 0; b < cb; ++b) {
+                        rows += part[2ull * b];
+                        edges += part[2ull * b + 1];
+                    }
+                    // x 8n + n; pull n (row bytes) + 4n (y), 8 per row visited (in-edge range), 9 per
+                    // in-edge (column, weight, x); apply 4n (y) + 16n (dist word) + 8n (degrees)
+                    bytes = 42.0 * n + 8.0 * rows + 9.0 * edges;
+                }
                 tok = prof_begin(g, "sssp_delta_pull", s);
                 sssp_pull_xb_kernel<<<grid_for((n + 3) / 4, kPushThreads, kMaxPushBlocks), kPushThreads, 0, s>>>(
                     n, g->dstamp, static_cast<uint32_t>(lo), limit, it, g->dl_xb, pq, g->hist);
                 G1_LAUNCHED();
-                G1_TRY(launch_spmv_merge<MinPlusU8X>(n, g->m, g->t_row_off, g->t_col, g->t_w, g->dl_xb, g->yu,
-                                                     g->t_ws, s));
+                if (pull_rows)
+                    G1_TRY(launch_pull_rows(g, pull_edges, s));
+                else
+                    G1_TRY(launch_spmv_merge<MinPlusU8X>(n, g->m, g->t_row_off, g->t_col, g->t_w, g->dl_xb, g->yu,
+                                                         g->t_ws, s));
                 sssp_pull_apply_kernel<<<kMaxPushBlocks, kPushThreads, 0, s>>>(
                     n, g->row_off, g->yu, g->dstamp, it + 1, pq, db, static_cast<uint32_t>(lo), leaving ? limit : kInf);
                 G1_LAUNCHED();
-                // x 8n + n; SpMV 4(n+1) + 8m + n (x) + 4n (y); apply 4n (y) + 16n (dist word) + 8n (degrees)
-                prof_end(g, tok, s, 4.0 * (n + 1.0) + 8.0 * g->m + 42.0 * n);
+                // merge path: x 8n + n; SpMV 4(n+1) + 8m + n (x) + 4n (y); apply 4n (y) + 16n (dist word) +
+                // 8n (degrees)
+                prof_end(g, tok, s, bytes);
                 ++pulls;
                 fused = leaving;
             } else if (heavy_edges >= pull_min) {

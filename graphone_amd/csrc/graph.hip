@@ -88,7 +88,8 @@ g1_status ensure_delta_split(g1_graph* g, uint32_t delta, hipStream_t s) {
 }
 
 // The rows of the in-edge CSR (or of PageRank's relabelled copy) longer than long_min nonzeros, for the
-// CSR-vector engine's long-row kernel; built once on the host.
+// CSR-vector engine's long-row kernel, longest first (ties by id); built once on the host. The in-edge
+// list also counts its rows longer than huge_min (the delta engine's heavy pull).
 g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s) {
     uint32_t** list = relabelled ? &g->pr_long : &g->t_long;
     uint32_t* count = relabelled ? &g->pr_nlong : &g->t_nlong;
@@ -102,6 +103,14 @@ g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s) {
     G1_HIP(hipMemcpy(ro.data(), ro_dev, sizeof(uint32_t) * (n + 1ull), hipMemcpyDeviceToHost));
     for (uint32_t v = 0; v < n; ++v)
         if (ro[v + 1] - ro[v] > g->long_min) rows.push_back(v);
+    std::stable_sort(rows.begin(), rows.end(),
+                     [&](uint32_t a, uint32_t b) { return ro[a + 1] - ro[a] > ro[b + 1] - ro[b]; });
+    if (!relabelled) {
+        g->huge_min = std::max(kPullRowsWaveMax, g->long_min);
+        g->t_nhuge = 0;
+        while (g->t_nhuge < rows.size() && ro[rows[g->t_nhuge] + 1] - ro[rows[g->t_nhuge]] > g->huge_min)
+            ++g->t_nhuge;
+    }
     G1_TRY(alloc_n(g, list, rows.empty() ? 1 : rows.size()));
     if (!rows.empty())
         G1_HIP(hipMemcpy(*list, rows.data(), sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice));

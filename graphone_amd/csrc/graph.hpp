@@ -41,9 +41,12 @@ struct g1_graph {
     float* pr_osum = nullptr;
     void* pr_ws = nullptr;
     // CSR-vector long rows (lazy, ensure_long_rows): rows of the in-edge CSR, and of the relabelled one,
-    // longer than long_min nonzeros
+    // longer than long_min nonzeros, longest first; the in-edge list's first t_nhuge rows are longer than
+    // huge_min (the delta engine's heavy pull gives those a workgroup each)
     uint32_t* t_long = nullptr;
     uint32_t t_nlong = 0;
+    uint32_t t_nhuge = 0;
+    uint32_t huge_min = 0;
     uint32_t* pr_long = nullptr;
     uint32_t pr_nlong = 0;
     uint32_t long_min = 0;

@@ -32,6 +32,7 @@
 #include <stdint.h>
 
 #include "g1/device.hpp"
+#include "g1/spmv_merge.hpp"
 
 namespace g1 {
 
@@ -694,7 +695,8 @@ sssp_pull_x_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, ui
 }
 
 // The byte x of the delta engine's heavy pull step (MinPlusU8X): xb[v] = d(v) - lo for the bucket [lo, hi)
-// (hi - lo <= 255), 255 for every other vertex; the history record as sssp_pull_x_kernel writes it.
+// (hi - lo <= 254), 254 below it, 255 at or past hi (or no distance yet); the history record as
+// sssp_pull_x_kernel writes it. The byte also tells which rows the pull visits (sssp_pull_rows_kernel).
 static __global__ void __launch_bounds__(kPushThreads)
 sssp_pull_xb_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, uint32_t lo, uint32_t hi, uint32_t it,
                     uint8_t* __restrict__ xb, PushQueues pq, unsigned long long* __restrict__ hist) {
@@ -717,11 +719,182 @@ sssp_pull_xb_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, u
             uint32_t b = 255u;
             if (v < n) {
                 const uint32_t d = static_cast<uint32_t>(dstamp[v] >> 32);
-                if (d >= lo && d < hi) b = d - lo;
+                if (d < hi) b = d >= lo ? d - lo : 254u;
             }
             packed |= b << (8 * k);
         }
         reinterpret_cast<uint32_t*>(xb)[q] = packed;
+    }
+}
+
+// The delta engine's heavy pull over the rows it can improve. After a bucket [lo, hi)'s light iterations
+// every light edge out of the bucket is relaxed, and a heavy edge (w >= hi - lo) out of it offers at least
+// hi, so only a vertex whose distance is at or past hi (byte 255) can improve: y[v] = min over v's
+// in-edges (u, v) of w + xb[u] (MinPlusU8X) is computed for those rows only; every other row gets
+// y = infinity. On RMAT22 from the hub the first bucket's 0.5 M vertices are the targets of 87 % of the
+// in-edges; 88 % of the rows are visited, 41 % of those have in-edges (5.4 on average).
+//
+// Each wave takes kPullRowsPerWave consecutive rows: it reads their bytes and, for the visited rows, their
+// in-edge ranges, and lists the visited rows with in-edges (at most huge_min each) in LDS, ballot-
+// compacted, with the prefix sums of their lengths: the wave's in-edges as one range. Every other row
+// (but a workgroup's) gets y = infinity at once. The wave then walks that range E edges per lane per round
+// (lane l takes E consecutive edges: one search of the list, then a step to the next row where one
+// ends), issues every column and weight load of the round, then every x gather, folds each run of one
+// row in registers and lowers the row's LDS minimum once per run. Rows longer than huge_min are the
+// first n_huge entries of the long-row list (longest first), one workgroup each at the start of the
+// launch. The minimum does not depend on the fold order.
+constexpr uint32_t kPullRowsPerLane = 4;
+constexpr uint32_t kPullRowsPerWave = kPullRowsPerLane * kWave;
+constexpr uint32_t kPullRowsWaves = kPushThreads / kWave;
+constexpr uint32_t kPullRowsPerBlock = kPullRowsPerWave * kPullRowsWaves;
+constexpr uint32_t kPullRowsWaveMax = 4096;  // rows up to this length stay with a wave
+
+template <int E>
+__global__ void __launch_bounds__(kPushThreads)
+sssp_pull_rows_kernel(uint32_t n, const uint32_t* __restrict__ t_row_off, const uint32_t* __restrict__ t_col,
+                      const uint32_t* __restrict__ t_w, const uint8_t* __restrict__ xb, uint32_t* __restrict__ y,
+                      uint32_t huge_min, const uint32_t* __restrict__ huge_rows, uint32_t n_huge) {
+    using S = MinPlusU8X;
+    using acc_t = S::acc_t;
+    if (blockIdx.x < n_huge) {
+        __shared__ acc_t s_acc[kPushThreads];
+        const uint32_t row = huge_rows[blockIdx.x];
+        if (xb[row] != 255u) return;  // not visited (uniform across the workgroup)
+        acc_t acc = S::identity();
+        csrvec_fold<S, true, kPushThreads>(t_col, t_w, xb, t_row_off[row] + threadIdx.x, t_row_off[row + 1], acc);
+        s_acc[threadIdx.x] = acc;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t d = kPushThreads / 2; d >= 1; d >>= 1) {
+            if (threadIdx.x < d) s_acc[threadIdx.x] = S::merge(s_acc[threadIdx.x], s_acc[threadIdx.x + d]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) y[row] = S::finalize(s_acc[0]);
+        return;
+    }
+    __shared__ uint32_t s_row[kPullRowsWaves][kPullRowsPerWave];
+    __shared__ uint32_t s_beg[kPullRowsWaves][kPullRowsPerWave];
+    __shared__ uint32_t s_off[kPullRowsWaves][kPullRowsPerWave + 1];
+    __shared__ uint32_t s_y[kPullRowsWaves][kPullRowsPerWave];
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t base = (blockIdx.x - n_huge) * kPullRowsPerBlock + wave * kPullRowsPerWave;
+    // every row's byte and in-edge range, loaded without branches (a lane past n reads row n - 1 and
+    // visits nothing), so the loads are in flight together
+    uint32_t byte[kPullRowsPerLane], beg[kPullRowsPerLane], end[kPullRowsPerLane];
+#pragma unroll
+    for (uint32_t r = 0; r < kPullRowsPerLane; ++r) {
+        const uint32_t v = base + r * kWave + lane;
+        const uint32_t vc = v < n ? v : n - 1;
+        byte[r] = xb[vc];
+        beg[r] = t_row_off[vc];
+        end[r] = t_row_off[vc + 1];
+    }
+    uint32_t n_rows = 0, n_edges = 0;  // wave-uniform
+#pragma unroll
+    for (uint32_t r = 0; r < kPullRowsPerLane; ++r) {
+        const uint32_t v = base + r * kWave + lane;
+        const bool visit = v < n && byte[r] == 255u;
+        const uint32_t dg = end[r] - beg[r];
+        const bool listed = visit && dg > 0 && dg <= huge_min;
+        if (v < n && !listed && !(visit && dg > huge_min)) y[v] = kInf;
+        const uint64_t m = ballot(listed);
+        const uint32_t len = listed ? dg : 0u;
+        const uint32_t incl = wave_inclusive_sum(len);
+        if (listed) {
+            const uint32_t k = n_rows + mask_prefix(m);
+            s_row[wave][k] = v;
+            s_beg[wave][k] = beg[r];
+            s_off[wave][k] = n_edges + incl - len;
+            s_y[wave][k] = kInf;
+        }
+        n_rows += static_cast<uint32_t>(__popcll(m));
+        n_edges += __shfl(incl, kWave - 1);
+    }
+    if (lane == 0) s_off[wave][n_rows] = n_edges;
+    __syncthreads();
+    for (uint32_t f0 = 0; f0 < n_edges; f0 += kWave * E) {  // wave-uniform
+        const uint32_t f1 = f0 + lane * E;
+        if (f1 < n_edges) {
+            // the listed row holding edge f1: the last k with s_off[k] <= f1
+            uint32_t lo = 0, hi = n_rows;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (s_off[wave][mid] <= f1) lo = mid;
+                else hi = mid;
+            }
+            // Edge positions first (LDS only); then every load of the round without a branch between
+            // them (a position past the range reads its row's first edge and is not folded), so they
+            // are all in flight together.
+            uint32_t k = lo;
+            uint32_t kk[E], e[E], c[E], w[E];
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                const uint32_t f = f1 + i;
+                const bool valid = f < n_edges;
+                if (valid)
+                    while (s_off[wave][k + 1] <= f) ++k;
+                kk[i] = valid ? k : kInf;
+                e[i] = s_beg[wave][k] + (valid ? f - s_off[wave][k] : 0u);
+            }
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                c[i] = t_col[e[i]];
+                w[i] = t_w[e[i]];
+            }
+            uint8_t xv[E];
+#pragma unroll
+            for (int i = 0; i < E; ++i) xv[i] = xb[c[i]];
+            uint32_t run_k = kk[0], run = kInf;
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                if (kk[i] == kInf) break;
+                const uint32_t cand = S::combine(w[i], xv[i]);
+                if (kk[i] != run_k) {
+                    if (run != kInf) atomicMin(&s_y[wave][run_k], run);
+                    run_k = kk[i];
+                    run = cand;
+                } else {
+                    run = cand < run ? cand : run;
+                }
+            }
+            if (run != kInf) atomicMin(&s_y[wave][run_k], run);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = lane; k < n_rows; k += kWave) y[s_row[wave][k]] = s_y[wave][k];
+}
+
+// Profiling only: the rows the next heavy pull visits (distance at or past hi) and their in-edges, per
+// workgroup into part[2 * blockIdx.x], part[2 * blockIdx.x + 1].
+static __global__ void __launch_bounds__(kPushThreads)
+sssp_pull_rows_count_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, uint32_t hi,
+                            const uint32_t* __restrict__ t_row_off, uint32_t* __restrict__ part) {
+    __shared__ uint32_t s_r[kPushThreads / kWave], s_e[kPushThreads / kWave];
+    uint32_t rows = 0, edges = 0;
+    for (uint32_t v = blockIdx.x * kPushThreads + threadIdx.x; v < n; v += gridDim.x * kPushThreads) {
+        if (static_cast<uint32_t>(dstamp[v] >> 32) >= hi) {
+            ++rows;
+            edges += t_row_off[v + 1] - t_row_off[v];
+        }
+    }
+    for (int off = kWave / 2; off >= 1; off >>= 1) {
+        rows += __shfl_down(rows, off);
+        edges += __shfl_down(edges, off);
+    }
+    if (threadIdx.x % kWave == 0) {
+        s_r[threadIdx.x / kWave] = rows;
+        s_e[threadIdx.x / kWave] = edges;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0, e = 0;
+        for (uint32_t w = 0; w < kPushThreads / kWave; ++w) {
+            r += s_r[w];
+            e += s_e[w];
+        }
+        part[2 * blockIdx.x] = r;
+        part[2 * blockIdx.x + 1] = e;
     }
 }
 

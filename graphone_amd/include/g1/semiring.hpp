@@ -40,9 +40,10 @@ struct MinPlusU32 {
 };
 
 // Min-plus over a one-byte x (the delta-stepping engine's heavy pull step): x[u] is a bucket member's
-// distance minus the bucket's start (0..254) and 255 for every other vertex, so the gathered vector is a
-// quarter of the u32 one and stays in the L2 caches; y = min over the in-edges of x + w, the bucket start
-// added back by the apply. Order free, like MinPlusU32.
+// distance minus the bucket's start (0..253), 254 for a vertex settled in an earlier bucket and 255 for
+// every other vertex, so the gathered vector is a quarter of the u32 one and stays in the L2 caches; y =
+// min over the in-edges of x + w for x < 254, the bucket start added back by the apply. Order free, like
+// MinPlusU32.
 struct MinPlusU8X {
     using value_t = uint32_t;
     using x_t = uint8_t;
@@ -52,7 +53,7 @@ struct MinPlusU8X {
 
     __device__ __forceinline__ static acc_t identity() { return acc_t{kInf}; }
     __device__ __forceinline__ static uint32_t combine(uint32_t w, uint8_t x) {
-        return (x == 255u || w >= kInf - 255u) ? kInf : w + x;
+        return (x >= 254u || w >= kInf - 255u) ? kInf : w + x;
     }
     __device__ __forceinline__ static void add(acc_t& a, uint32_t p) { a.v = p < a.v ? p : a.v; }
     __device__ __forceinline__ static acc_t merge(acc_t a, acc_t b) { return acc_t{b.v < a.v ? b.v : a.v}; }

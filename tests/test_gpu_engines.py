@@ -76,14 +76,21 @@ def test_sssp_delta_schedules(graphs, monkeypatch, width, switch):
         assert got.stats["reached"] == int(np.sum(want != oracle.INF))
 
 
-@pytest.mark.parametrize("hi,width,pull", [(255, "16", "0.01"), (1000, "300", "0.01"), (255, "16", "0")])
-def test_sssp_delta_heavy_steps(gpu, monkeypatch, hi, width, pull):
-    """The heavy phase as a pull step over a one-byte x (bucket width <= 255), over the u32 x (a wider
-    bucket), and as a push launch (pull steps off): Dijkstra's distances from the hub and from a low-degree
-    vertex, with the pull step taken where it is enabled."""
+@pytest.mark.parametrize("hi,width,pull,rows,edges", [
+    (255, "16", "0.01", "1", "1"), (255, "16", "0.01", "1", "4"), (255, "16", "0.01", "1", "16"),
+    (255, "16", "0.01", "0", "1"), (1000, "254", "0.01", "1", "1"), (1000, "300", "0.01", "1", "1"),
+    (255, "16", "0", "1", "1")])
+def test_sssp_delta_heavy_steps(gpu, monkeypatch, hi, width, pull, rows, edges):
+    """The heavy phase as a pull step over a one-byte x (bucket width <= 254; over the rows at or past the
+    bucket's end with 1, 4 or 16 edges per lane per round, or as the merge-path SpMV over every row), over the u32 x (a
+    wider bucket), and as a push launch (pull steps off): Dijkstra's distances from the hub and from a
+    low-degree vertex, with the pull step taken where it is enabled. Width 254 puts bucket offsets next to
+    the byte's two markers (254: below the bucket, 255: past it)."""
     import graphone_amd as g1
     monkeypatch.setenv("G1_SSSP_DELTA", width)
     monkeypatch.setenv("G1_SSSP_PULL", pull)
+    monkeypatch.setenv("G1_SSSP_DELTA_PULL_ROWS", rows)
+    monkeypatch.setenv("G1_SSSP_DELTA_PULL_EDGES", edges)
     h = g1.HostGraph.rmat(14, 16, seed=41).assign_weights(1, hi, seed=42)
     g = g1.Graph(h)
     deg = h.out_degrees()
@@ -91,7 +98,7 @@ def test_sssp_delta_heavy_steps(gpu, monkeypatch, hi, width, pull):
     for src in (int(np.argmax(deg)), int(pos[np.argmin(deg[pos])])):
         want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, src)
         got = g.sssp(src, "delta")
-        assert np.array_equal(got.values, want), (src, hi, width, pull)
+        assert np.array_equal(got.values, want), (src, hi, width, pull, rows, edges)
         if src == int(np.argmax(deg)):
             assert (got.stats["pull_iterations"] >= 1) == (pull != "0"), got.stats
 
