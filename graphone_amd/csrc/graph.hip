@@ -3,6 +3,7 @@
 // and stays resident in HBM; the in-edge CSR is built from it on the device (convert.hip).
 #include <hip/hip_runtime.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -106,7 +107,13 @@ g1_status ensure_long_rows(g1_graph* g, bool relabelled, hipStream_t s) {
     std::stable_sort(rows.begin(), rows.end(),
                      [&](uint32_t a, uint32_t b) { return ro[a + 1] - ro[a] > ro[b + 1] - ro[b]; });
     if (!relabelled) {
-        g->huge_min = std::max(kPullRowsWaveMax, g->long_min);
+        // G1_SSSP_DELTA_PULL_HUGE lowers the row length past which the heavy pull gives a row a workgroup
+        // (default 4096; never below long_min, the list's own bound)
+        const char* e = getenv("G1_SSSP_DELTA_PULL_HUGE");
+        const long hv = e ? atol(e) : 0;
+        const uint32_t huge = (hv > 0 && hv < static_cast<long>(kPullRowsWaveMax)) ? static_cast<uint32_t>(hv)
+                                                                                   : kPullRowsWaveMax;
+        g->huge_min = std::max(huge, g->long_min);
         g->t_nhuge = 0;
         while (g->t_nhuge < rows.size() && ro[rows[g->t_nhuge] + 1] - ro[rows[g->t_nhuge]] > g->huge_min)
             ++g->t_nhuge;

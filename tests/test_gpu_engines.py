@@ -103,6 +103,28 @@ def test_sssp_delta_heavy_steps(gpu, monkeypatch, hi, width, pull, rows, edges):
             assert (got.stats["pull_iterations"] >= 1) == (pull != "0"), got.stats
 
 
+def test_sssp_delta_pull_workgroup_rows(gpu, monkeypatch):
+    """The heavy pull over rows with its workgroup path taken by every row past the long-row bound (128
+    in-edges here, G1_SSSP_DELTA_PULL_HUGE; the handle's long-row list is built with it) and pull steps from
+    1 % of m: from low-degree sources a heavy pull then visits hubs that are not settled yet. Dijkstra's
+    distances from those sources and from the hub."""
+    import graphone_amd as g1
+    monkeypatch.setenv("G1_SSSP_DELTA_PULL_HUGE", "1")
+    monkeypatch.setenv("G1_SSSP_PULL", "0.01")
+    h = g1.HostGraph.rmat(14, 16, seed=43).assign_weights(1, 255, seed=44)
+    g = g1.Graph(h)
+    deg = h.out_degrees()
+    pos = np.flatnonzero(deg > 0)
+    rng = np.random.default_rng(45)
+    pulls = 0
+    for src in [int(np.argmax(deg))] + [int(s) for s in rng.choice(pos[deg[pos] <= 2], 4, replace=False)]:
+        want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, src)
+        got = g.sssp(src, "delta")
+        assert np.array_equal(got.values, want), src
+        pulls += got.stats["pull_iterations"]
+    assert pulls >= 1
+
+
 def test_sssp_delta_zero_weights(gpu):
     """Weights 0..3 (a quarter of the edges weigh nothing, so improvements stay inside their bucket):
     delta-stepping from the hub and from vertex 1 equals Dijkstra."""
