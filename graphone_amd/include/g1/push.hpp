@@ -295,10 +295,10 @@ __device__ __forceinline__ void advance_tiles(const RV& rv, const uint32_t* __re
         uint32_t cand[kPushEpt];
         uint32_t cdeg[kPushEpt];
         uint32_t wd[kPushEpt];
+        // every slot loads a degree (a slot that did not win reads vertex 0's), so no load waits behind
+        // another's branch
 #pragma unroll
-        for (int r = 0; r < kPushEpt; ++r) {
-            if (won[r]) wd[r] = rv.degree(vv[r]);
-        }
+        for (int r = 0; r < kPushEpt; ++r) wd[r] = rv.degree(won[r] ? vv[r] : 0u);
         uint32_t my_cnt = 0, my_deg = 0;
 #pragma unroll
         for (int r = 0; r < kPushEpt; ++r) {
@@ -586,11 +586,19 @@ dist_range_frontier_kernel(uint32_t n, RV rv, unsigned long long* __restrict__ d
     for (uint32_t base = blockIdx.x * kScanRound; base < n; base += gridDim.x * kScanRound) {  // uniform
         uint32_t cand[kScanPerThread], cdeg[kScanPerThread];
         uint32_t my_cnt = 0, my_deg = 0;
-        uint32_t d[kScanPerThread];
+        // the distance words, then the degrees of the vertices in range, each stage's loads issued
+        // together (clamped indices, no load behind a branch)
+        uint32_t d[kScanPerThread], dgs[kScanPerThread];
 #pragma unroll
         for (uint32_t r = 0; r < kScanPerThread; ++r) {
             const uint32_t v = base + r * kPushThreads + threadIdx.x;
-            d[r] = v < n ? static_cast<uint32_t>(dstamp[v] >> 32) : kInf;
+            const uint32_t dv = static_cast<uint32_t>(dstamp[v < n ? v : n - 1] >> 32);
+            d[r] = v < n ? dv : kInf;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kScanPerThread; ++r) {
+            const uint32_t v = base + r * kPushThreads + threadIdx.x;
+            dgs[r] = rv.degree(d[r] >= lo && d[r] < hi ? v : 0u);
         }
 #pragma unroll
         for (uint32_t r = 0; r < kScanPerThread; ++r) {
@@ -600,12 +608,11 @@ dist_range_frontier_kernel(uint32_t n, RV rv, unsigned long long* __restrict__ d
             if (d[r] >= lo && d[r] != kInf && d[r] < mn) mn = d[r];
             if (d[r] >= lo && d[r] < hi) {
                 dstamp[v] = (static_cast<unsigned long long>(d[r]) << 32) | stamp;
-                const uint32_t dg = rv.degree(v);
-                if (dg > 0) {
+                if (dgs[r] > 0) {
                     cand[r] = v;
-                    cdeg[r] = dg;
+                    cdeg[r] = dgs[r];
                     ++my_cnt;
-                    my_deg += dg;
+                    my_deg += dgs[r];
                 }
             }
         }
@@ -640,9 +647,11 @@ struct BfsPush {
     template <int N>
     __device__ __forceinline__ void relax_many(const uint32_t (&)[N], const uint32_t (&)[N], const uint32_t (&v)[N],
                                                const uint32_t (&)[N], const uint32_t (&)[N], bool (&won)[N]) const {
+        // The depth loads go out together (an empty slot reads entry 0: no load behind a branch), then
+        // the claims; the bitmap bits only once every claim has returned, so no wait sits between them.
         uint32_t cur[N];
 #pragma unroll
-        for (int r = 0; r < N; ++r) cur[r] = (v[r] != kInf) ? depth[v[r]] : 0u;
+        for (int r = 0; r < N; ++r) cur[r] = depth[v[r] != kInf ? v[r] : 0u];
         uint32_t old[N];
 #pragma unroll
         for (int r = 0; r < N; ++r) {
@@ -650,9 +659,11 @@ struct BfsPush {
             if (v[r] != kInf && cur[r] == kInf) old[r] = atomicCAS(&depth[v[r]], kInf, next_level);
         }
 #pragma unroll
-        for (int r = 0; r < N; ++r) {
-            won[r] = v[r] != kInf && cur[r] == kInf && old[r] == kInf;
-            if (won[r] && fb_next) atomicOr(&fb_next[v[r] >> 5], 1u << (v[r] & 31u));
+        for (int r = 0; r < N; ++r) won[r] = v[r] != kInf && cur[r] == kInf && old[r] == kInf;
+        if (fb_next) {
+#pragma unroll
+            for (int r = 0; r < N; ++r)
+                if (won[r]) atomicOr(&fb_next[v[r] >> 5], 1u << (v[r] & 31u));
         }
     }
     __device__ __forceinline__ bool relax1(uint32_t, uint32_t v, uint32_t) const {
@@ -712,15 +723,26 @@ sssp_pull_xb_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, u
     // four vertices per thread: one 32-bit store of four bytes
     const uint32_t quads = (n + 3) / 4;
     for (uint32_t q = blockIdx.x * kPushThreads + threadIdx.x; q < quads; q += gridDim.x * kPushThreads) {
+        // the quad's four distance words in two 16-byte loads (the last, partial quad word by word)
+        uint32_t d[4];
+        if (q * 4 + 3 < n) {
+            const ulonglong2* p = reinterpret_cast<const ulonglong2*>(dstamp + static_cast<size_t>(q) * 4);
+            const ulonglong2 a = p[0], b = p[1];
+            d[0] = static_cast<uint32_t>(a.x >> 32);
+            d[1] = static_cast<uint32_t>(a.y >> 32);
+            d[2] = static_cast<uint32_t>(b.x >> 32);
+            d[3] = static_cast<uint32_t>(b.y >> 32);
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t v = q * 4 + k;
+                d[k] = v < n ? static_cast<uint32_t>(dstamp[v] >> 32) : kInf;
+            }
+        }
         uint32_t packed = 0;
 #pragma unroll
         for (uint32_t k = 0; k < 4; ++k) {
-            const uint32_t v = q * 4 + k;
-            uint32_t b = 255u;
-            if (v < n) {
-                const uint32_t d = static_cast<uint32_t>(dstamp[v] >> 32);
-                if (d < hi) b = d >= lo ? d - lo : 254u;
-            }
+            const uint32_t b = d[k] < hi ? (d[k] >= lo ? d[k] - lo : 254u) : 255u;
             packed |= b << (8 * k);
         }
         reinterpret_cast<uint32_t*>(xb)[q] = packed;
@@ -913,43 +935,52 @@ sssp_pull_apply_kernel(uint32_t n, const uint32_t* __restrict__ row_off, const u
     const uint32_t seg_out = blockIdx.x % kSegments;
     uint32_t activated = 0;
     for (uint32_t base = blockIdx.x * kScanRound; base < n; base += gridDim.x * kScanRound) {  // uniform
+        // Loads go out a stage at a time with clamped indices (a lane past n reads vertex n - 1 and does
+        // nothing with it), so no load waits behind another's branch: y and the distance words, then the
+        // degrees of the vertices to enqueue.
         uint32_t cand[kScanPerThread], cdeg[kScanPerThread];
         uint32_t my_cnt = 0, my_deg = 0;
-        uint32_t yv[kScanPerThread];
+        uint32_t yv[kScanPerThread], d_new[kScanPerThread];
         unsigned long long ds[kScanPerThread];
+        bool improved[kScanPerThread], enq[kScanPerThread];
 #pragma unroll
         for (uint32_t r = 0; r < kScanPerThread; ++r) {
             const uint32_t v = base + r * kPushThreads + threadIdx.x;
-            yv[r] = kInf;
-            ds[r] = 0ull;
-            if (v < n) {
-                const uint32_t yr = y[v];
-                yv[r] = (yr >= kInf - y_add) ? kInf : yr + y_add;
-                ds[r] = dstamp[v];
-            }
+            const uint32_t vc = v < n ? v : n - 1;
+            yv[r] = y[vc];
+            ds[r] = dstamp[vc];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kScanPerThread; ++r) {
+            const uint32_t v = base + r * kPushThreads + threadIdx.x;
+            yv[r] = (yv[r] >= kInf - y_add) ? kInf : yv[r] + y_add;
+            const uint32_t d_old = static_cast<uint32_t>(ds[r] >> 32);
+            improved[r] = v < n && yv[r] < d_old;
+            d_new[r] = improved[r] ? yv[r] : d_old;
+            enq[r] = frontier_lo == kInf ? improved[r] : (v < n && d_new[r] >= frontier_lo && d_new[r] != kInf);
+        }
+        uint32_t dgs[kScanPerThread];
+#pragma unroll
+        for (uint32_t r = 0; r < kScanPerThread; ++r) {
+            const uint32_t v = base + r * kPushThreads + threadIdx.x;
+            const uint32_t vc = enq[r] ? v : 0u;
+            dgs[r] = row_off[vc + 1] - row_off[vc];
         }
 #pragma unroll
         for (uint32_t r = 0; r < kScanPerThread; ++r) {
             const uint32_t v = base + r * kPushThreads + threadIdx.x;
             cand[r] = kInf;
             cdeg[r] = 0;
-            const uint32_t d_old = static_cast<uint32_t>(ds[r] >> 32);
-            const bool improved = v < n && yv[r] < d_old;
-            const uint32_t d_new = improved ? yv[r] : d_old;
-            const bool enq = frontier_lo == kInf ? improved : (v < n && d_new >= frontier_lo && d_new != kInf);
-            if (improved) {
+            if (improved[r]) {
                 db.lower(v, yv[r]);
                 ++activated;
             }
-            if (improved || enq) dstamp[v] = (static_cast<unsigned long long>(d_new) << 32) | stamp;
-            if (enq) {
-                const uint32_t dg = row_off[v + 1] - row_off[v];
-                if (dg > 0) {
-                    cand[r] = v;
-                    cdeg[r] = dg;
-                    ++my_cnt;
-                    my_deg += dg;
-                }
+            if (improved[r] || enq[r]) dstamp[v] = (static_cast<unsigned long long>(d_new[r]) << 32) | stamp;
+            if (enq[r] && dgs[r] > 0) {
+                cand[r] = v;
+                cdeg[r] = dgs[r];
+                ++my_cnt;
+                my_deg += dgs[r];
             }
         }
         push_enqueue(cand, cdeg, my_cnt, my_deg, pq, seg_out, s_scan, &s_base);
@@ -981,27 +1012,55 @@ __device__ __forceinline__ void pull_groups(uint32_t n, const uint32_t* __restri
     // kPullGroups groups per round and one queue reservation per round: waves run through their vertices
     // without waiting for each other group by group
     for (uint32_t g0 = g_beg; g0 < g_end; g0 += kPullGroups) {  // uniform per workgroup
-        uint32_t vv[kPullGroups], rb[kPullGroups], re[kPullGroups];
+        // Every load of a stage is issued before any is waited on, with indices clamped rather than loads
+        // behind branches (a lane with nothing to load reads entry 0): the groups' depths, then the
+        // unvisited vertices' in-edge ranges, then the first two in-edges of every group and their
+        // bitmap words. Only vertices with no hit among those and more in-edges go on alone.
+        uint32_t vv[kPullGroups], rb[kPullGroups], re[kPullGroups], dep[kPullGroups];
         bool found[kPullGroups];
 #pragma unroll
         for (uint32_t r = 0; r < kPullGroups; ++r) {
             vv[r] = (g0 + r) * kPushThreads + threadIdx.x;
-            found[r] = g0 + r < g_end && vv[r] < n && depth[vv[r]] == kInf;  // unvisited (scan below)
+            const bool in = g0 + r < g_end && vv[r] < n;
+            dep[r] = depth[in ? vv[r] : 0u];
+            found[r] = in;
         }
 #pragma unroll
         for (uint32_t r = 0; r < kPullGroups; ++r) {
-            rb[r] = 0;
-            re[r] = 0;
-            if (found[r]) {
-                rb[r] = t_row_off[vv[r]];
-                re[r] = t_row_off[vv[r] + 1];
+            found[r] = found[r] && dep[r] == kInf;  // unvisited (scan below)
+            const uint32_t vc = found[r] ? vv[r] : 0u;
+            rb[r] = t_row_off[vc];
+            re[r] = t_row_off[vc + 1];
+            if (!found[r]) re[r] = rb[r];
+        }
+        constexpr int kFirst = 2;
+        uint32_t u0[kPullGroups][kFirst], w0[kPullGroups][kFirst];
+#pragma unroll
+        for (uint32_t r = 0; r < kPullGroups; ++r) {
+#pragma unroll
+            for (int q = 0; q < kFirst; ++q) {
+                const uint32_t k = rb[r] + q;
+                const uint32_t c = t_col[k < re[r] ? k : 0u];
+                u0[r][q] = k < re[r] ? c : kInf;
             }
         }
 #pragma unroll
         for (uint32_t r = 0; r < kPullGroups; ++r) {
+#pragma unroll
+            for (int q = 0; q < kFirst; ++q) w0[r][q] = fb_cur[(u0[r][q] != kInf ? u0[r][q] : 0u) >> 5];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kPullGroups; ++r) {
             bool hit = false;
-            // four in-edges per step, their bitmap words loaded together
-            for (uint32_t k = rb[r]; k < re[r] && !hit; k += 4) {
+#pragma unroll
+            for (int q = 0; q < kFirst; ++q) {
+                if (!hit && u0[r][q] != kInf) {
+                    ++scanned;
+                    if ((w0[r][q] >> (u0[r][q] & 31u)) & 1u) hit = true;
+                }
+            }
+            // the rest of the in-edges, four per step, their bitmap words loaded together
+            for (uint32_t k = rb[r] + kFirst; k < re[r] && !hit; k += 4) {
                 uint32_t u[4], wd[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) u[q] = (k + q < re[r]) ? t_col[k + q] : kInf;
@@ -1016,6 +1075,12 @@ __device__ __forceinline__ void pull_groups(uint32_t n, const uint32_t* __restri
                 }
             }
             found[r] = hit;
+        }
+        uint32_t dgs[kPullGroups];
+#pragma unroll
+        for (uint32_t r = 0; r < kPullGroups; ++r) {
+            const uint32_t vc = found[r] ? vv[r] : 0u;
+            dgs[r] = row_off[vc + 1] - row_off[vc];
         }
         uint32_t cand[kPullGroups], cdeg[kPullGroups];
         uint32_t my_cnt = 0, my_deg = 0;
@@ -1032,7 +1097,7 @@ __device__ __forceinline__ void pull_groups(uint32_t n, const uint32_t* __restri
             if (found[r]) {
                 depth[vv[r]] = next_level;
                 ++activated;
-                const uint32_t dg = row_off[vv[r] + 1] - row_off[vv[r]];
+                const uint32_t dg = dgs[r];
                 if (dg > 0) {
                     cand[r] = vv[r];
                     cdeg[r] = dg;
