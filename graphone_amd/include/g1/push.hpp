@@ -247,19 +247,25 @@ __device__ __forceinline__ void advance_tiles(const RV& rv, const uint32_t* __re
         for (uint32_t base = 0; base < static_cast<uint32_t>(kPushTile); base += kPushThreads) {
             const uint32_t k = base + threadIdx.x;
             const uint32_t i = i0 + k;
-            bool valid = false, at_e1 = false;
+            // the entry's offset and vertex together, then its row start and source value together (a
+            // lane past the frontier or past the tile reads vertex 0's), so staging waits twice, not
+            // three times
+            uint32_t qo = 0, u = 0;
             if (i < nf) {
                 const uint32_t s = search_last_le(s_vb, kSegments, i);
                 const size_t at = static_cast<size_t>(s) * pq.cap + (i - s_vb[s]);
-                const uint32_t qo = s_eb[s] + pq.qoff_in[at];
-                const uint32_t u = pq.q_in[at];
-                valid = (k == 0) || (qo < e1);
-                at_e1 = (k != 0) && (qo == e1);
-                if (valid) {
-                    s_qoff[k] = qo;
-                    s_rs[k] = rv.begin(u);
-                    s_su[k] = pol.source_value(u);
-                }
+                qo = s_eb[s] + pq.qoff_in[at];
+                u = pq.q_in[at];
+            }
+            const bool valid = i < nf && ((k == 0) || (qo < e1));
+            const bool at_e1 = i < nf && (k != 0) && (qo == e1);
+            const uint32_t uc = valid ? u : 0u;
+            const uint32_t rs = rv.begin(uc);
+            const typename P::src_t su_k = pol.source_value(uc);
+            if (valid) {
+                s_qoff[k] = qo;
+                s_rs[k] = rs;
+                s_su[k] = su_k;
             }
             const uint32_t c = block_count_pair<kPushThreads>(valid, at_e1, s_red);
             cnt += c & 0xffffu;
