@@ -118,27 +118,36 @@ __device__ __forceinline__ void merge_load_index(uint32_t tile, const uint32_t* 
     st.c0 = coords[tile];
     st.c1 = coords[tile + 1];
     const uint32_t rows = st.c1.x - st.c0.x, nzs = st.c1.y - st.c0.y;
+    // Every slot loads (a slot past the tile's rows or nonzeros reads the tile's last one and is not
+    // used), so no load waits behind another's branch. Column and value loads stream (read once), so the
+    // gathered x keeps more of the L2.
+    const uint32_t last_nz = nzs > 0 ? st.c0.y + nzs - 1 : 0u;
 #pragma unroll
     for (int r = 0; r < kMergeItems; ++r) {
         const uint32_t k = r * kMergeThreads + tid;
-        if (k < rows) st.re[r] = row_offsets[st.c0.x + 1 + k];
-        if (k < nzs) {
-            // read once: streaming loads, so the gathered x keeps more of the L2
-            st.ci[r] = __builtin_nontemporal_load(col + st.c0.y + k);
-            st.wv[r] = kHasValues ? __builtin_nontemporal_load(vals + st.c0.y + k) : typename S::value_t(1);
+        st.re[r] = row_offsets[k < rows ? st.c0.x + 1 + k : st.c1.x];
+        st.ci[r] = 0u;
+        st.wv[r] = typename S::value_t(1);
+        if (nzs > 0) {  // uniform across the workgroup (col and vals may be null without nonzeros)
+            const uint32_t e = k < nzs ? st.c0.y + k : last_nz;
+            st.ci[r] = __builtin_nontemporal_load(col + e);
+            if (kHasValues) st.wv[r] = __builtin_nontemporal_load(vals + e);
         }
     }
 }
 
-// Issues the gathers of the tile's x values (its column loads must have arrived).
+// Issues the gathers of the tile's x values (its column loads must have arrived). Every gather is issued
+// (a slot past the tile's nonzeros reads x[0], unused), so none waits behind another's branch; a tile
+// without nonzeros reads nothing (x may be null when the matrix has none).
 template <typename S>
 __device__ __forceinline__ void merge_load_x(const typename S::x_t* __restrict__ x, MergeStage<S>& st) {
     const uint32_t tid = threadIdx.x;
     const uint32_t nzs = st.c1.y - st.c0.y;
+    if (nzs == 0) return;  // uniform across the workgroup
 #pragma unroll
     for (int r = 0; r < kMergeItems; ++r) {
         const uint32_t k = r * kMergeThreads + tid;
-        if (k < nzs) st.xv[r] = x[st.ci[r]];
+        st.xv[r] = x[k < nzs ? st.ci[r] : 0u];
     }
 }
 
