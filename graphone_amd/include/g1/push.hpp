@@ -1020,8 +1020,8 @@ __device__ __forceinline__ void pull_groups(uint32_t n, const uint32_t* __restri
     for (uint32_t g0 = g_beg; g0 < g_end; g0 += kPullGroups) {  // uniform per workgroup
         // Every load of a stage is issued before any is waited on, with indices clamped rather than loads
         // behind branches (a lane with nothing to load reads entry 0): the groups' depths, then the
-        // unvisited vertices' in-edge ranges, then the first two in-edges of every group and their
-        // bitmap words. Only vertices with no hit among those and more in-edges go on alone.
+        // unvisited vertices' in-edge ranges, then the first in-edge of every group and its bitmap
+        // word (one per vertex keeps 71 VGPRs, six waves per SIMD; two took 85 and five, four 115 and four). Only vertices with no hit among those and more in-edges go on alone.
         uint32_t vv[kPullGroups], rb[kPullGroups], re[kPullGroups], dep[kPullGroups];
         bool found[kPullGroups];
 #pragma unroll
@@ -1039,7 +1039,7 @@ __device__ __forceinline__ void pull_groups(uint32_t n, const uint32_t* __restri
             re[r] = t_row_off[vc + 1];
             if (!found[r]) re[r] = rb[r];
         }
-        constexpr int kFirst = 2;
+        constexpr int kFirst = 1;
         uint32_t u0[kPullGroups][kFirst], w0[kPullGroups][kFirst];
 #pragma unroll
         for (uint32_t r = 0; r < kPullGroups; ++r) {
