@@ -1021,7 +1021,8 @@ __device__ __forceinline__ void pull_groups(uint32_t n, const uint32_t* __restri
         // Every load of a stage is issued before any is waited on, with indices clamped rather than loads
         // behind branches (a lane with nothing to load reads entry 0): the groups' depths, then the
         // unvisited vertices' in-edge ranges, then the first in-edge of every group and its bitmap
-        // word (one per vertex keeps 71 VGPRs, six waves per SIMD; two took 85 and five, four 115 and four). Only vertices with no hit among those and more in-edges go on alone.
+        // word (one per vertex keeps 71 VGPRs, six waves per SIMD; two took 85 and five, four 115 and
+        // four). Only vertices with no hit there and more in-edges go on alone.
         uint32_t vv[kPullGroups], rb[kPullGroups], re[kPullGroups], dep[kPullGroups];
         bool found[kPullGroups];
 #pragma unroll
