@@ -14,7 +14,7 @@
 // each workgroup takes a contiguous run of tiles. A tile finds its first queue entry with a search over
 // the segment bases (LDS) and a wave-wide 64-way search inside the segment, stages the tile's sources in
 // LDS, then maps each edge slot to its source with a binary search in LDS. Every tile holds the same
-// number of edges whatever the degree distribution (a 97k-edge hub is spread over ~48 tiles), unlike the
+// number of edges whatever the degree distribution (a 97k-edge hub is spread over 48 to 190 tiles), unlike the
 // reference's per-block balancing (advance_gc.h:119-157).
 //
 // Filter fused with compaction. The user filter (advance_gc.h:212-244) becomes the policy's relax(): it
@@ -37,8 +37,9 @@
 namespace g1 {
 
 constexpr int kPushThreads = 256;
-constexpr int kPushEpt = 8;                          // edge slots per thread per tile
-constexpr int kPushTile = kPushThreads * kPushEpt;   // 2048 edge slots per tile
+constexpr int kPushEpt = 8;                          // edge slots per thread per tile, at most
+constexpr int kPushTile = kPushThreads * kPushEpt;   // 2048 edge slots per tile, at most
+constexpr uint32_t kPushTileMin = 512;               // the smallest tile a launch cuts its frontier into
 constexpr int kSegments = 64;                        // queue segments (one wave scans their counters)
 constexpr uint32_t kMaxPushBlocks = 2048;            // producers per launch (bounds segment fill)
 
@@ -215,7 +216,15 @@ __device__ __forceinline__ void advance_tiles(const RV& rv, const uint32_t* __re
 
     const uint32_t nf = s_vb[kSegments];
     const uint32_t E = s_eb[kSegments];
-    const uint32_t tiles = (E + kPushTile - 1) / kPushTile;
+    // The tile size: the largest power of two from kPushTileMin to kPushTile that still gives every
+    // workgroup a tile (uniform across the launch). A frontier of a few hundred thousand edges in
+    // 2048-slot tiles left most workgroups idle; a large one keeps the big tiles and their lower
+    // per-edge staging cost. RMAT22 step: fixed 2048-slot tiles 1.09 ms; fixed 512: 1.02 ms, but the
+    // advance loop's large frontier 0.175 -> 0.20 ms and push-only BFS 0.78 -> 0.90 ms; chosen per launch
+    // 0.98 ms (a floor of 256 instead of 512: 0.99-1.01 ms).
+    uint32_t T = kPushTile;
+    while (T > kPushTileMin && (E + T - 1) / T < gridDim.x) T >>= 1;
+    const uint32_t tiles = (E + T - 1) / T;
     const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
     const uint32_t t_beg = blockIdx.x * per;
     const uint32_t t_end = (t_beg + per < tiles) ? t_beg + per : tiles;
@@ -225,8 +234,8 @@ __device__ __forceinline__ void advance_tiles(const RV& rv, const uint32_t* __re
     // The first queue entry of the next tile, when this tile's staging fixes it (kInf: search for it).
     uint32_t next_i0 = kInf;
     for (uint32_t tile = t_beg; tile < t_end; ++tile) {
-        const uint32_t e0 = tile * kPushTile;
-        const uint32_t e1 = (E - e0 < static_cast<uint32_t>(kPushTile)) ? E : e0 + kPushTile;
+        const uint32_t e0 = tile * T;
+        const uint32_t e1 = (E - e0 < T) ? E : e0 + T;
         uint32_t i0 = next_i0;  // block-uniform
         if (i0 == kInf) {
             if (threadIdx.x < kWave) {
@@ -244,7 +253,7 @@ __device__ __forceinline__ void advance_tiles(const RV& rv, const uint32_t* __re
         // this tile's last entry (its edges run on past e1). The entry id is loaded with its offset.
         uint32_t cnt = 0;
         bool exact = false;
-        for (uint32_t base = 0; base < static_cast<uint32_t>(kPushTile); base += kPushThreads) {
+        for (uint32_t base = 0; base < T; base += kPushThreads) {
             const uint32_t k = base + threadIdx.x;
             const uint32_t i = i0 + k;
             // the entry's offset and vertex together, then its row start and source value together (a
@@ -272,7 +281,7 @@ __device__ __forceinline__ void advance_tiles(const RV& rv, const uint32_t* __re
             exact = exact || (c >> 16) != 0u;
             if ((c & 0xffffu) < static_cast<uint32_t>(kPushThreads)) break;
         }
-        next_i0 = (cnt < static_cast<uint32_t>(kPushTile)) ? i0 + cnt - (exact ? 0u : 1u) : kInf;
+        next_i0 = (cnt < T) ? i0 + cnt - (exact ? 0u : 1u) : kInf;
         // (block_count_pair ends with a barrier, so the staged sources are visible here)
         // Phase 1: map every edge slot of this thread to its source and load (v, w); phase 2: let the
         // policy relax all of them (it issues its gathers and atomics before consuming any result);
