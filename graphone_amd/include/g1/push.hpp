@@ -780,7 +780,9 @@ sssp_pull_xb_kernel(uint32_t n, const unsigned long long* __restrict__ dstamp, u
 // row in registers and lowers the row's LDS minimum once per run. Rows longer than huge_min are the
 // first n_huge entries of the long-row list (longest first), one workgroup each at the start of the
 // launch. The minimum does not depend on the fold order.
-constexpr uint32_t kPullRowsPerLane = 4;
+// Rows per lane: 1 (64 rows per wave). The RMAT22 heavy pull took 170 / 172 / 179 / 219 us at 1 / 2 / 4 /
+// 8 rows per lane.
+constexpr uint32_t kPullRowsPerLane = 1;
 constexpr uint32_t kPullRowsPerWave = kPullRowsPerLane * kWave;
 constexpr uint32_t kPullRowsWaves = kPushThreads / kWave;
 constexpr uint32_t kPullRowsPerBlock = kPullRowsPerWave * kPullRowsWaves;
