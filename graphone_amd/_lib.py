@@ -40,7 +40,8 @@ G1_WORK_RANK = 4
 G1_WORK_RELABEL = 8
 G1_WORK_DELTA = 16
 G1_WORK_ROAD = 32
-G1_WORK_ALL = 63
+G1_WORK_HUB = 64
+G1_WORK_ALL = 127
 
 G1_BATCH_MINPLUS = 0
 G1_BATCH_PPR = 1

@@ -182,7 +182,7 @@ g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s) {
         }
         if (st == G1_OK) st = exclusive_scan_u32(lens, g->pr_row_off, n + 1, aux, s);
         if (st == G1_OK && m > 0) {
-            relabel_copy_kernel<<<blocks_for((static_cast<uint64_t>(m) + kExpandPer - 1) / kExpandPer), 256, 0, s>>>(
+            relabel_copy_kernel<float><<<blocks_for((static_cast<uint64_t>(m) + kExpandPer - 1) / kExpandPer), 256, 0, s>>>(
                 n, m, g->pr_row_off, g->pr_perm, keys, g->t_row_off, g->t_col, g->t_wf, g->pr_col, g->pr_wf);
             if (hipGetLastError() != hipSuccess) st = set_error(G1_ERR_HIP, "relabel: copy launch failed");
         }
@@ -196,6 +196,87 @@ g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s) {
     G1_TRY(graph_alloc(g, &g->pr_ws, merge_workspace_bytes(n, m)));
     G1_TRY(launch_merge_partition(n, m, g->pr_row_off, g->pr_ws, s));
     G1_HIP(hipStreamSynchronize(s));
+    return G1_OK;
+}
+
+// The push engines' hub-first working copy of the out-edge CSR (ensure_hub_copy): vertices ordered by
+// out-degree group, largest first (group k: degrees in [2^k - 1, 2^(k+1) - 1)), and by id within a group:
+// vertex v is new id inv[v] (device) and inv_host[v] (host array of n). The hubs still come first and share
+// cache lines, and a gather in the original ids (the results mapped back) reads a few dozen ascending
+// streams rather than one random line per vertex, as an exact degree order gives. The rows are
+// copied renamed, then transposed: the transpose is a stable sort by column, so each row of its output
+// lists its entries by ascending source. Transposing that once more gives the out-edge CSR with each row's
+// columns ascending (row_off, col, w); the intermediate is the copy's in-edge CSR (t_row_off, t_col, t_w)
+// when those are given. For a symmetric graph the first transpose already is the sorted out-edge CSR:
+// pass symmetric = true and it lands in (row_off, col, w). Temporary arrays are freed before it returns.
+g1_status build_hub_csr(const g1_graph* g, bool symmetric, uint32_t* inv, uint32_t* inv_host, uint32_t* row_off, uint32_t* col, uint32_t* w,
+                        uint32_t* t_row_off, uint32_t* t_col, uint32_t* t_w, hipStream_t s) {
+    const uint32_t n = g->n, m = g->m;
+    const bool weighted = g->w != nullptr;
+    const bool keep_t = t_row_off != nullptr;
+    const size_t arr = align256(std::max<size_t>(n, 1) * sizeof(uint32_t));
+    const size_t n1 = align256((static_cast<size_t>(n) + 1) * sizeof(uint32_t));
+    const size_t marr = align256((static_cast<size_t>(m) + kCsrPad) * sizeof(uint32_t));
+    // perm sort: keys, ids, 2 key + 1 id ping-pong buffers, perm, lens, sort aux; the renamed rows (n + 1
+    // offsets, columns, weights); the intermediate transpose when it is not kept; the transpose work areas
+    const size_t sort_bytes = arr * 6 + n1 + align256(sort_aux_bytes(n));  // (the sixth: perm)
+    const size_t rows_bytes = n1 + marr * (weighted ? 2 : 1);
+    const size_t mid_bytes = (symmetric || keep_t) ? 0 : rows_bytes;
+    const size_t bytes = sort_bytes + rows_bytes + mid_bytes + transpose_workspace_bytes(n, m, weighted);
+    void* tmp = nullptr;
+    G1_HIP(hipMalloc(&tmp, bytes));
+    uint8_t* p = static_cast<uint8_t*>(tmp);
+    auto take = [&](size_t b) {
+        uint32_t* a = reinterpret_cast<uint32_t*>(p);
+        p += b;
+        return a;
+    };
+    uint32_t* keys = take(arr);
+    uint32_t* ids = take(arr);
+    uint32_t* kbuf[2] = {take(arr), take(arr)};
+    uint32_t* ubuf[2] = {take(arr), ids};
+    uint32_t* wbuf[2] = {nullptr, nullptr};
+    uint32_t* perm = take(arr);
+    uint32_t* lens = take(n1);
+    uint32_t* aux = take(align256(sort_aux_bytes(n)));
+    uint32_t* r_off = take(n1);
+    uint32_t* r_col = take(marr);
+    uint32_t* r_w = weighted ? take(marr) : nullptr;
+    if (!symmetric && !keep_t) {
+        t_row_off = take(n1);
+        t_col = take(marr);
+        t_w = weighted ? take(marr) : nullptr;
+    }
+    void* tws = p;
+    if (symmetric) {  // the first transpose is the result
+        t_row_off = row_off;
+        t_col = col;
+        t_w = w;
+    }
+    g1_status st = G1_OK;
+    const uint32_t grid = std::min<uint32_t>(blocks_for(static_cast<uint64_t>(n) + 1), 8192);
+    const uint32_t* sorted = nullptr;
+    degree_group_keys_kernel<<<grid, 256, 0, s>>>(n, g->row_off, keys, ids);
+    if (hipGetLastError() != hipSuccess) st = set_error(G1_ERR_HIP, "hub copy: degree keys launch failed");
+    if (st == G1_OK) st = radix_sort_pairs(n, 33, keys, ids, nullptr, kbuf, ubuf, wbuf, perm, nullptr, aux, &sorted, s);
+    if (st == G1_OK) {
+        hub_rows_kernel<<<grid, 256, 0, s>>>(n, perm, g->row_off, inv, lens);
+        if (hipGetLastError() != hipSuccess) st = set_error(G1_ERR_HIP, "hub copy: row launch failed");
+    }
+    if (st == G1_OK) st = exclusive_scan_u32(lens, r_off, n + 1, aux, s);
+    if (st == G1_OK && m > 0) {
+        relabel_copy_kernel<uint32_t><<<blocks_for((static_cast<uint64_t>(m) + kExpandPer - 1) / kExpandPer), 256, 0, s>>>(
+            n, m, r_off, perm, inv, g->row_off, g->col, g->w, r_col, r_w);
+        if (hipGetLastError() != hipSuccess) st = set_error(G1_ERR_HIP, "hub copy: copy launch failed");
+    }
+    if (st == G1_OK) st = device_transpose(n, m, r_off, r_col, r_w, t_row_off, t_col, t_w, tws, s);
+    if (st == G1_OK && !symmetric) st = device_transpose(n, m, t_row_off, t_col, t_w, row_off, col, w, tws, s);
+    if (st == G1_OK && hipMemcpyAsync(inv_host, inv, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        st = set_error(G1_ERR_HIP, "hub copy: map download failed");
+    const hipError_t sync = hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+    G1_TRY(st);
+    G1_HIP(sync);
     return G1_OK;
 }
 

@@ -366,10 +366,12 @@ g1_status run_small(g1_graph* g, hipStream_t s, const P& pol, const uint32_t* wt
 // push_finish_kernel folds every slot region and writes the counters, the overflow flag and the
 // iteration history to pinned memory. The run's end event must already be recorded.
 g1_status collect_push_run(g1_graph* g, hipStream_t s, uint32_t launched, const uint32_t* d, g1_run_stats* st,
-                           bool count_edges, bool skip_empty = false) {
-    reached_stats_kernel<<<grid_for(g->n, 256, kMaxPushBlocks), 256, 0, s>>>(
-        g->n, d, g->row_off, g->slots + 2 * kMaxPushBlocks, g->slots + 3 * kMaxPushBlocks);
-    G1_LAUNCHED();
+                           bool count_edges, bool skip_empty = false, bool stats_done = false) {
+    if (!stats_done) {  // (done: a run on the hub-first copy counted them while mapping its result back)
+        reached_stats_kernel<<<grid_for(g->n, 256, kMaxPushBlocks), 256, 0, s>>>(
+            g->n, d, g->row_off, g->slots + 2 * kMaxPushBlocks, g->slots + 3 * kMaxPushBlocks);
+        G1_LAUNCHED();
+    }
     if (!g->h_hist) {
         G1_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->h_hist), 2ull * kHistIters * sizeof(unsigned long long)));
     }
@@ -405,6 +407,25 @@ g1_status collect_push_run(g1_graph* g, hipStream_t s, uint32_t launched, const 
 g1_status run_bfs_road(g1_graph* g, uint32_t src, uint32_t* depth, g1_run_stats* st, hipStream_t s, bool* ok);
 bool road_graph(const g1_graph* g);
 
+// The delta-stepping SSSP and direction-optimizing BFS run on the hub-first copy of a graph (ensure_hub_copy)
+// from 2^20 vertices up, or on any graph whose copy was prepared.
+constexpr uint32_t kHubMinVertices = 1u << 20;
+bool hub_wanted(const g1_graph* g) { return hub_eligible(g) && (g->hub != nullptr || g->n >= kHubMinVertices); }
+
+// A run on the hub-first copy: its result (u32 depths, or the distance words) into out in the ids of the
+// graph the caller holds, with the reached-vertex stats collect_push_run would count (it is then told
+// they are done).
+template <bool kWord>
+g1_status map_back(g1_graph* g, const void* src, uint32_t* out, hipStream_t s) {
+    const int tm = prof_begin(g, "hub_map_back", s);
+    map_back_stats_kernel<kWord><<<grid_for((g->n + 3) / 4, 256, kMaxPushBlocks), 256, 0, s>>>(
+        g->n, g->out_inv, src, out, g->out_row_off, g->slots + 2 * kMaxPushBlocks, g->slots + 3 * kMaxPushBlocks);
+    G1_LAUNCHED();
+    // the map, the results gathered, the row offsets read, the results written
+    prof_end(g, tm, s, (kWord ? 20.0 : 16.0) * g->n);
+    return G1_OK;
+}
+
 g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, g1_run_stats* st, hipStream_t s) {
     if (road_graph(g)) {
         bool ok = false;
@@ -412,6 +433,13 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
         if (ok) return G1_OK;
         memset(st, 0, sizeof(*st));
     }
+    if (diropt && g->has_t && hub_wanted(g)) {
+        G1_TRY(ensure_hub_copy(g, s));
+        g->hub->profile = g->profile;
+        return run_bfs_push(g->hub, g->hub_inv_h[src], depth, true, st, s);
+    }
+    uint32_t* const out = depth;
+    if (g->out_inv) depth = g->out_work;  // a run on the hub-first copy: its depths, mapped into out at the end
     G1_TRY(ensure_push_work(g, s));
     if (diropt) G1_CHECK(g->has_t, G1_ERR_INVALID_ARG, "g1_bfs: direction-optimizing BFS needs the transpose");
     const uint32_t n = g->n;
@@ -462,8 +490,9 @@ g1_status run_bfs_push(g1_graph* g, uint32_t src, uint32_t* depth, bool diropt, 
     };
     uint32_t launched = 0;
     G1_TRY(drive_push(g, s, launch, after, small, g->max_out_deg <= kSmallCap, &launched));
+    if (g->out_inv) G1_TRY(map_back<false>(g, depth, out, s));
     G1_HIP(hipEventRecord(g->ev[1], s));
-    G1_TRY(collect_push_run(g, s, launched, depth, st, true));
+    G1_TRY(collect_push_run(g, s, launched, out, st, true, false, g->out_inv != nullptr));
     st->frontier_sum = 1 + g->h_ctr[3];
     st->edges_sum += g->h_ctr[7];
     return G1_OK;
@@ -481,6 +510,8 @@ __global__ void dstamp_extract_kernel(uint32_t n, const unsigned long long* __re
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
         dist[v] = static_cast<uint32_t>(ds[v] >> 32);
 }
+
+
 
 // ---------------------------------------------------------------- road-like graphs (one workgroup)
 
@@ -912,6 +943,11 @@ __global__ void __launch_bounds__(128) clear_sets_kernel(unsigned long long* qcn
 // the bucket from inside it is light) before their heavy edges are relaxed once.
 g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats* st, hipStream_t s) {
     if (road_graph(g) && g->w) return run_sssp_road(g, src, dist, st, s);
+    if (g->w && hub_wanted(g)) {
+        G1_TRY(ensure_hub_copy(g, s));
+        g->hub->profile = g->profile;
+        return run_sssp_delta(g->hub, g->hub_inv_h[src], dist, st, s);
+    }
     G1_TRY(ensure_push_work(g, s));
     G1_CHECK(g->w != nullptr, G1_ERR_INVALID_ARG, "g1_sssp: graph has no weights");
     const uint32_t delta = sssp_delta_width(g);
@@ -1145,10 +1181,14 @@ g1_status run_sssp_delta(g1_graph* g, uint32_t src, uint32_t* dist, g1_run_stats
         first_edges = fr >> 32;
         small_first = first_edges <= kSmallCap;
     }
-    dstamp_extract_kernel<<<fill_blocks, 256, 0, s>>>(n, g->dstamp, dist);
-    G1_LAUNCHED();
+    if (g->out_inv) {
+        G1_TRY(map_back<true>(g, g->dstamp, dist, s));
+    } else {
+        dstamp_extract_kernel<<<fill_blocks, 256, 0, s>>>(n, g->dstamp, dist);
+        G1_LAUNCHED();
+    }
     G1_HIP(hipEventRecord(g->ev[1], s));
-    G1_TRY(collect_push_run(g, s, it, dist, st, true, true));
+    G1_TRY(collect_push_run(g, s, it, dist, st, true, true, g->out_inv != nullptr));
     st->pull_iterations += pulls;
     st->push_iterations -= std::min(st->push_iterations, pulls);
     st->dense_iterations = dense;

@@ -40,6 +40,19 @@ struct g1_graph {
     float* pr_wf = nullptr;
     float* pr_osum = nullptr;
     void* pr_ws = nullptr;
+    // the push engines' hub-first working copy (lazy, ensure_hub_copy): a graph of its own over the vertices
+    // in groups of out-degree (powers of two), largest first, with each row's columns ascending; vertex v
+    // here is vertex hub_inv[v] there (device, and the host copy for sources). The copy's runs write their
+    // results in this graph's ids: out_inv is set in the copy (hub_inv) and out_row_off to this graph's
+    // row offsets (for the reached-edge stats), out_work holds the copy's BFS depths before they are
+    // mapped, and its profile entries go to prof_to.
+    g1_graph* hub = nullptr;
+    uint32_t* hub_inv = nullptr;
+    std::vector<uint32_t> hub_inv_h;
+    const uint32_t* out_inv = nullptr;
+    const uint32_t* out_row_off = nullptr;
+    uint32_t* out_work = nullptr;
+    g1_graph* prof_to = nullptr;
     // CSR-vector long rows (lazy, ensure_long_rows): rows of the in-edge CSR, and of the relabelled one,
     // longer than long_min nonzeros, longest first; the in-edge list's first t_nhuge rows are longer than
     // huge_min (the delta engine's heavy pull gives those a workgroup each)
@@ -149,6 +162,13 @@ g1_status ensure_push_work(g1_graph* g, hipStream_t s);  // work areas cleared o
 g1_status ensure_pull_work(g1_graph* g, bool pagerank);
 g1_status ensure_sweep_pieces(g1_graph* g, hipStream_t s);  // pieces of the long rows of the out-edge CSR
 g1_status ensure_pr_relabel(g1_graph* g, hipStream_t s);  // after ensure_pull_work(g, true)
+// The push engines' hub-first copy (graph.hip; 8 bytes per vertex here, the copy's CSR and work areas in
+// it) and its CSR build (convert.hip). Used by the delta-stepping SSSP and direction-optimizing BFS runs
+// when hub_wanted (engines.hip).
+g1_status ensure_hub_copy(g1_graph* g, hipStream_t s);
+g1_status build_hub_csr(const g1_graph* g, bool symmetric, uint32_t* inv, uint32_t* inv_host, uint32_t* row_off, uint32_t* col, uint32_t* w,
+                        uint32_t* t_row_off, uint32_t* t_col, uint32_t* t_w, hipStream_t s);
+bool hub_eligible(const g1_graph* g);
 g1_status ensure_delta_split(g1_graph* g, uint32_t delta, hipStream_t s);  // 8m + 4n bytes, rebuilt when delta changes
 
 // On-device CSR conversions (convert.hip): the transpose (workspace of transpose_workspace_bytes), an

@@ -309,4 +309,65 @@ static __global__ void __launch_bounds__(256) reached_stats_kernel(uint32_t n, c
     }
 }
 
+// The end of a run on the hub-first copy of a graph: d[v] = the copy's result at inv[v] (kWord: the high
+// half of its 64-bit distance word, else a u32 depth), with the reached-vertex stats of d counted as
+// reached_stats_kernel counts them (row_off in d's ids). Each thread takes four consecutive v per step, so
+// its four gathers are in flight together.
+template <bool kWord>
+static __global__ void __launch_bounds__(256) map_back_stats_kernel(uint32_t n, const uint32_t* __restrict__ inv,
+                                                                    const void* __restrict__ src,
+                                                                    uint32_t* __restrict__ d,
+                                                                    const uint32_t* __restrict__ row_off,
+                                                                    unsigned long long* __restrict__ slots_cnt,
+                                                                    unsigned long long* __restrict__ slots_edges) {
+    __shared__ unsigned long long s_red[2][256 / kWave];
+    unsigned long long cnt = 0, edges = 0;
+    const uint32_t quads = (n + 3) / 4;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
+        const uint32_t v0 = 4 * q;
+        uint32_t iv[4], x[4], ro[5];
+        if (v0 + 4 <= n) {
+            const uint4 t = reinterpret_cast<const uint4*>(inv)[q];  // inv is a 256-byte aligned allocation
+            iv[0] = t.x;
+            iv[1] = t.y;
+            iv[2] = t.z;
+            iv[3] = t.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) iv[k] = inv[v0 + k < n ? v0 + k : n - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if constexpr (kWord)
+                x[k] = static_cast<uint32_t>(static_cast<const unsigned long long*>(src)[iv[k]] >> 32);
+            else
+                x[k] = static_cast<const uint32_t*>(src)[iv[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) ro[k] = row_off[v0 + k < n ? v0 + k : n];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (v0 + k < n) {
+                d[v0 + k] = x[k];
+                if (x[k] != kInf) {
+                    ++cnt;
+                    edges += ro[k + 1] - ro[k];
+                }
+            }
+        }
+    }
+    cnt = wave_sum(cnt);
+    edges = wave_sum(edges);
+    if (lane_id() == 0) {
+        s_red[0][threadIdx.x / kWave] = cnt;
+        s_red[1][threadIdx.x / kWave] = edges;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        slots_cnt[blockIdx.x] += s_red[0][0] + s_red[0][1] + s_red[0][2] + s_red[0][3];
+        slots_edges[blockIdx.x] += s_red[1][0] + s_red[1][1] + s_red[1][2] + s_red[1][3];
+    }
+}
+
 }  // namespace g1

@@ -244,16 +244,46 @@ static __global__ void __launch_bounds__(256) relabel_rows_kernel(uint32_t n, co
     }
 }
 
-// New in-edge row i is old row perm[i] with its entries in their order and every source renamed:
-// ncol[j] = inv[t_col[t_row_off[perm[i]] + (j - nro[i])]] (the weight follows unchanged). Each thread
-// takes 16 consecutive slots j.
+// keys[v] = the number of leading zero bits of out-degree(v) + 1 (0..31: the larger the degree group, the
+// smaller the key), ids[v] = v.
+static __global__ void __launch_bounds__(256) degree_group_keys_kernel(uint32_t n, const uint32_t* __restrict__ row_off,
+                                                                       uint32_t* __restrict__ keys,
+                                                                       uint32_t* __restrict__ ids) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        keys[v] = static_cast<uint32_t>(__clz(static_cast<int>(row_off[v + 1] - row_off[v] + 1u)));
+        ids[v] = v;
+    }
+}
+
+// inv[perm[i]] = i; lens[i] = out-degree of perm[i], lens[n] = 0 (scanned into the new row offsets).
+static __global__ void __launch_bounds__(256) hub_rows_kernel(uint32_t n, const uint32_t* __restrict__ perm,
+                                                              const uint32_t* __restrict__ row_off,
+                                                              uint32_t* __restrict__ inv, uint32_t* __restrict__ lens) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += stride) {
+        if (i == n) {
+            lens[n] = 0u;
+            continue;
+        }
+        const uint32_t v = perm[i];
+        inv[v] = i;
+        lens[i] = row_off[v + 1] - row_off[v];
+    }
+}
+
+// New row i is old row perm[i] of (t_row_off, t_col, t_wf) with its entries in their order and every
+// column renamed: ncol[j] = inv[t_col[t_row_off[perm[i]] + (j - nro[i])]] (the weight follows unchanged).
+// Each thread takes 16 consecutive slots j. Used for the in-edge CSR (float weights, PageRank) and for the
+// out-edge CSR (u32 weights, the push engines' copy).
+template <typename W>
 static __global__ void __launch_bounds__(256) relabel_copy_kernel(uint32_t n, uint32_t m, const uint32_t* __restrict__ nro,
                                                                   const uint32_t* __restrict__ perm,
                                                                   const uint32_t* __restrict__ inv,
                                                                   const uint32_t* __restrict__ t_row_off,
                                                                   const uint32_t* __restrict__ t_col,
-                                                                  const float* __restrict__ t_wf,
-                                                                  uint32_t* __restrict__ ncol, float* __restrict__ nwf) {
+                                                                  const W* __restrict__ t_wf,
+                                                                  uint32_t* __restrict__ ncol, W* __restrict__ nwf) {
     const uint64_t j0 = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kExpandPer;
     if (j0 >= m) return;
     const uint32_t j_end = static_cast<uint32_t>(j0 + kExpandPer < m ? j0 + kExpandPer : m);

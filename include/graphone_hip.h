@@ -308,6 +308,13 @@ g1_status g1_graph_info(const g1_graph* graph, uint32_t* n, uint32_t* m, uint64_
  *                    view, which keeps the graph's ids)
  *   G1_WORK_DELTA    the delta-stepping engine's weight-split CSR copy (the engine also needs PUSH | PULL)
  *   G1_WORK_ROAD     the road engine's far lists and query records (implies G1_WORK_PUSH)
+ *   G1_WORK_HUB      the hub-first working copy the delta-stepping SSSP and direction-optimizing BFS run
+ *                    on: the vertices relabelled by out-degree, largest first, the CSR (and its transpose)
+ *                    in those ids with each row's columns ascending, the map from the graph's ids (4 bytes
+ *                    per vertex, and a host copy), and the copy's own push, pull and delta work areas.
+ *                    Those runs map the source in and their results back, so callers see the graph's
+ *                    ids. Built on the first such run when the graph has at least 2^20 vertices, or here
+ *                    for any size; none for a view or a road-like graph. G1_HUB=0 turns it off.
  * Unknown bits are rejected with G1_ERR_INVALID_ARG. */
 #define G1_WORK_PUSH 1
 #define G1_WORK_PULL 2
@@ -315,7 +322,8 @@ g1_status g1_graph_info(const g1_graph* graph, uint32_t* n, uint32_t* m, uint64_
 #define G1_WORK_RELABEL 8
 #define G1_WORK_DELTA 16
 #define G1_WORK_ROAD 32
-#define G1_WORK_ALL 63
+#define G1_WORK_HUB 64
+#define G1_WORK_ALL 127
 g1_status g1_graph_work_bytes(const g1_graph* graph, int32_t work, uint64_t* bytes);
 g1_status g1_graph_prepare(g1_graph* graph, int32_t work, g1_stream_t stream);
 g1_status g1_graph_csr_dev(const g1_graph* graph, int32_t transpose, const uint32_t** row_offsets_dev,

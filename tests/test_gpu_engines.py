@@ -125,6 +125,50 @@ def test_sssp_delta_pull_workgroup_rows(gpu, monkeypatch):
     assert pulls >= 1
 
 
+@pytest.mark.parametrize("sym", [False, True])
+def test_hub_copy_runs_match_oracle(gpu, monkeypatch, sym):
+    """Delta-stepping SSSP and direction-optimizing BFS on the hub-first copy (prepared with G1_WORK_HUB on
+    a graph below the size the engines build it at by themselves) equal Dijkstra and the BFS queue in the
+    graph's own ids, with the reached-vertex stats of the runs without it (a second handle, and G1_HUB=0 on
+    the prepared one); directed with its transpose, and symmetric."""
+    import graphone_amd as g1
+    from graphone_amd import _lib
+    h = g1.HostGraph.rmat(14, 16, seed=61, undirected=sym).assign_weights(1, 255, seed=62, symmetric=sym)
+    g = g1.Graph(h, transpose=True, symmetric=sym)
+    g0 = g1.Graph(h, transpose=True, symmetric=sym)
+    want_bytes = g.work_bytes(_lib.G1_WORK_HUB)
+    before = g.device_bytes()
+    g.prepare(_lib.G1_WORK_HUB)
+    grew = g.device_bytes() - before
+    assert grew <= want_bytes <= grew + 4 * h.n, (want_bytes, grew)
+    assert g.work_bytes(_lib.G1_WORK_HUB) == 0
+    deg = h.out_degrees()
+    rng = np.random.default_rng(63)
+    srcs = [int(np.argmax(deg)), 1, int(np.flatnonzero(deg == 0)[0]) if np.any(deg == 0) else 2]
+    srcs += [int(v) for v in rng.choice(h.n, 3, replace=False)]
+    total = g.device_bytes()
+    for src in srcs:
+        want_d = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, src)
+        want_b = oracle.bfs_queue(h.row_offsets, h.col_indices, src)
+        d, d0 = g.sssp(src, "delta"), g0.sssp(src, "delta")
+        b, b0 = g.bfs(src, "diropt"), g0.bfs(src, "diropt")
+        assert np.array_equal(d.values, want_d), src
+        assert np.array_equal(b.values, want_b), src
+        for got, ref in ((d, d0), (b, b0)):
+            assert got.stats["reached"] == ref.stats["reached"], src
+            assert got.stats["reached_edges"] == ref.stats["reached_edges"], src
+    assert g.device_bytes() == total  # the runs allocate nothing more
+    g.profile(True)
+    g.sssp(srcs[0], "delta")
+    names = {e["name"] for e in g.read_profile()}
+    g.profile(False)
+    assert "sssp_delta_light" in names, names
+    monkeypatch.setenv("G1_HUB", "0")  # the graph's own ids again
+    assert np.array_equal(g.sssp(srcs[0], "delta").values,
+                          oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, srcs[0]))
+    assert np.array_equal(g.bfs(srcs[0], "diropt").values, oracle.bfs_queue(h.row_offsets, h.col_indices, srcs[0]))
+
+
 def test_sssp_delta_zero_weights(gpu):
     """Weights 0..3 (a quarter of the edges weigh nothing, so improvements stay inside their bucket):
     delta-stepping from the hub and from vertex 1 equals Dijkstra."""
@@ -256,7 +300,7 @@ def test_work_areas_sized_and_prepared_ahead(gpu):
     g = g1.Graph(h)
     n = h.n
     for work, bound in ((_lib.G1_WORK_PUSH, 0), (_lib.G1_WORK_PULL, 4 * n), (_lib.G1_WORK_RANK, 0),
-                        (_lib.G1_WORK_RELABEL, 4 * n), (_lib.G1_WORK_DELTA, 0)):
+                        (_lib.G1_WORK_RELABEL, 4 * n), (_lib.G1_WORK_DELTA, 0), (_lib.G1_WORK_HUB, 4 * n)):
         want = g.work_bytes(work)
         before = g.device_bytes()
         g.prepare(work)
