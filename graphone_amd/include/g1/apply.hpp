@@ -311,15 +311,17 @@ static __global__ void __launch_bounds__(256) reached_stats_kernel(uint32_t n, c
 
 // The end of a run on the hub-first copy of a graph: d[v] = the copy's result at inv[v] (kWord: the high
 // half of its 64-bit distance word, else a u32 depth), with the reached-vertex stats of d counted as
-// reached_stats_kernel counts them (row_off in d's ids). Each thread takes four consecutive v per step, so
-// its four gathers are in flight together.
+// reached_stats_kernel counts them (row_off in d's ids), added atomically into slot blockIdx.x % slots.
+// Each thread takes four consecutive v, so its four gathers are in flight together; the grid covers n in
+// one pass.
 template <bool kWord>
 static __global__ void __launch_bounds__(256) map_back_stats_kernel(uint32_t n, const uint32_t* __restrict__ inv,
                                                                     const void* __restrict__ src,
                                                                     uint32_t* __restrict__ d,
                                                                     const uint32_t* __restrict__ row_off,
                                                                     unsigned long long* __restrict__ slots_cnt,
-                                                                    unsigned long long* __restrict__ slots_edges) {
+                                                                    unsigned long long* __restrict__ slots_edges,
+                                                                    uint32_t slots) {
     __shared__ unsigned long long s_red[2][256 / kWave];
     unsigned long long cnt = 0, edges = 0;
     const uint32_t quads = (n + 3) / 4;
@@ -365,8 +367,8 @@ static __global__ void __launch_bounds__(256) map_back_stats_kernel(uint32_t n, 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        slots_cnt[blockIdx.x] += s_red[0][0] + s_red[0][1] + s_red[0][2] + s_red[0][3];
-        slots_edges[blockIdx.x] += s_red[1][0] + s_red[1][1] + s_red[1][2] + s_red[1][3];
+        atomicAdd(&slots_cnt[blockIdx.x % slots], s_red[0][0] + s_red[0][1] + s_red[0][2] + s_red[0][3]);
+        atomicAdd(&slots_edges[blockIdx.x % slots], s_red[1][0] + s_red[1][1] + s_red[1][2] + s_red[1][3]);
     }
 }
 

@@ -169,6 +169,32 @@ def test_hub_copy_runs_match_oracle(gpu, monkeypatch, sym):
     assert np.array_equal(g.bfs(srcs[0], "diropt").values, oracle.bfs_queue(h.row_offsets, h.col_indices, srcs[0]))
 
 
+def test_hub_copy_built_by_the_engines(gpu):
+    """From 2^20 vertices the two engines build the hub-first copy on their own first run (the handle's
+    device bytes grow by what G1_WORK_HUB reports) and stay bit-exact: RMAT20 from the hub and from a
+    low-degree vertex against Dijkstra and the BFS queue."""
+    import graphone_amd as g1
+    from graphone_amd import _lib
+    h = g1.HostGraph.rmat(20, 8, 0.57, 0.19, 0.19, seed=71).assign_weights(1, 255, seed=72)
+    assert h.n >= 1 << 20
+    g = g1.Graph(h, transpose=True)
+    deg = h.out_degrees()
+    srcs = [int(np.argmax(deg)), int(np.flatnonzero(deg == 1)[0])]
+    want_bytes, before = g.work_bytes(_lib.G1_WORK_HUB), g.device_bytes()
+    b = g.bfs(srcs[0], "diropt")
+    assert np.array_equal(b.values, oracle.bfs_queue(h.row_offsets, h.col_indices, srcs[0]))
+    for src in srcs:
+        want = oracle.dijkstra(h.row_offsets, h.col_indices, h.weights, src)
+        got = g.sssp(src, "delta")
+        assert np.array_equal(got.values, want), src
+        assert got.stats["reached"] == int(np.count_nonzero(want != 0xFFFFFFFF)), src
+        assert got.stats["reached_edges"] == int(deg[want != 0xFFFFFFFF].sum()), src
+        assert np.array_equal(g.bfs(src, "diropt").values, oracle.bfs_queue(h.row_offsets, h.col_indices, src)), src
+    grew = g.device_bytes() - before  # the copy and its work areas, the long-row list within its bound
+    assert grew <= want_bytes <= grew + 4 * h.n, (want_bytes, grew)
+    assert g.work_bytes(_lib.G1_WORK_HUB) == 0
+
+
 def test_sssp_delta_zero_weights(gpu):
     """Weights 0..3 (a quarter of the edges weigh nothing, so improvements stay inside their bucket):
     delta-stepping from the hub and from vertex 1 equals Dijkstra."""
