@@ -61,6 +61,18 @@ def test_rmat22_sssp_bitexact(rmat22, engine):
         assert got.stats["dense_iterations"] >= 1, got.stats
 
 
+@pytest.mark.parametrize("kind", ["bfs", "sssp"])
+def test_rmat22_own_ids_bitexact(rmat22, monkeypatch, kind):
+    """The two engines that run on the hub-first copy at this size (direction-optimizing BFS, delta-stepping
+    SSSP), on the graph's own ids instead (G1_HUB=0): the same results and reached counts."""
+    h, g, src, want_bfs, want_sssp = rmat22
+    monkeypatch.setenv("G1_HUB", "0")
+    got = g.bfs(src, "diropt") if kind == "bfs" else g.sssp(src, "delta")
+    want = want_bfs if kind == "bfs" else want_sssp
+    assert np.array_equal(got.values, want), (kind, _diff(got.values, want))
+    assert got.stats["reached"] == int(np.sum(want != oracle.INF))
+
+
 def test_rmat22_sssp_other_sources(rmat22):
     """Two more sources (a low-degree vertex and a vertex of median degree among the reached ones): other
     frontier shapes through the same engines."""
