@@ -665,7 +665,10 @@ g1_status g1_graph_info(const g1_graph* g, uint32_t* n, uint32_t* m, uint64_t* d
     G1_CHECK(g != nullptr, G1_ERR_INVALID_ARG, "g1_graph_info: null graph");
     if (n) *n = g->n;
     if (m) *m = g->m;
-    if (device_bytes) *device_bytes = g->device_bytes + (g->hub ? g->hub->device_bytes : 0);
+    if (device_bytes) {  // (the lock: a run may be building the hub-first copy)
+        std::lock_guard<std::mutex> lock(const_cast<g1_graph*>(g)->mu);
+        *device_bytes = g->device_bytes + (g->hub ? g->hub->device_bytes : 0);
+    }
     return G1_OK;
 }
 
