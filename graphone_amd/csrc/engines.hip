@@ -418,7 +418,7 @@ bool hub_wanted(const g1_graph* g) { return hub_eligible(g) && (g->hub != nullpt
 template <bool kWord>
 g1_status map_back(g1_graph* g, const void* src, uint32_t* out, hipStream_t s) {
     const int tm = prof_begin(g, "hub_map_back", s);
-    map_back_stats_kernel<kWord><<<grid_for((g->n + 3) / 4, 256, 1u << 20), 256, 0, s>>>(
+    map_back_stats_kernel<kWord><<<grid_for((g->n + 1023) / 1024, 1, 1u << 20), 256, 0, s>>>(
         g->n, g->out_inv, src, out, g->out_row_off, g->slots + 2 * kMaxPushBlocks, g->slots + 3 * kMaxPushBlocks,
         kMaxPushBlocks);
     G1_LAUNCHED();

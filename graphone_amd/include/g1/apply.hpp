@@ -312,8 +312,9 @@ static __global__ void __launch_bounds__(256) reached_stats_kernel(uint32_t n, c
 // The end of a run on the hub-first copy of a graph: d[v] = the copy's result at inv[v] (kWord: the high
 // half of its 64-bit distance word, else a u32 depth), with the reached-vertex stats of d counted as
 // reached_stats_kernel counts them (row_off in d's ids), added atomically into slot blockIdx.x % slots.
-// Each thread takes four consecutive v, so its four gathers are in flight together; the grid covers n in
-// one pass.
+// A workgroup takes 1024 consecutive v per step, each thread four of them 256 apart, so every load and
+// store instruction covers 256 contiguous bytes and a thread's four gathers are in flight together; the
+// grid covers n in one pass.
 template <bool kWord>
 static __global__ void __launch_bounds__(256) map_back_stats_kernel(uint32_t n, const uint32_t* __restrict__ inv,
                                                                     const void* __restrict__ src,
@@ -324,20 +325,16 @@ static __global__ void __launch_bounds__(256) map_back_stats_kernel(uint32_t n, 
                                                                     uint32_t slots) {
     __shared__ unsigned long long s_red[2][256 / kWave];
     unsigned long long cnt = 0, edges = 0;
-    const uint32_t quads = (n + 3) / 4;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
-        const uint32_t v0 = 4 * q;
-        uint32_t iv[4], x[4], ro[5];
-        if (v0 + 4 <= n) {
-            const uint4 t = reinterpret_cast<const uint4*>(inv)[q];  // inv is a 256-byte aligned allocation
-            iv[0] = t.x;
-            iv[1] = t.y;
-            iv[2] = t.z;
-            iv[3] = t.w;
-        } else {
+    const uint32_t tiles = (n + 1023) / 1024;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        uint32_t v[4], iv[4], x[4], r0[4], r1[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) iv[k] = inv[v0 + k < n ? v0 + k : n - 1];
+        for (int k = 0; k < 4; ++k) {
+            v[k] = t * 1024 + k * 256 + threadIdx.x;
+            const uint32_t c = v[k] < n ? v[k] : n - 1;  // clamped: every load is issued
+            iv[k] = inv[c];
+            r0[k] = row_off[c];
+            r1[k] = row_off[c + 1];
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -347,14 +344,12 @@ static __global__ void __launch_bounds__(256) map_back_stats_kernel(uint32_t n, 
                 x[k] = static_cast<const uint32_t*>(src)[iv[k]];
         }
 #pragma unroll
-        for (int k = 0; k < 5; ++k) ro[k] = row_off[v0 + k < n ? v0 + k : n];
-#pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (v0 + k < n) {
-                d[v0 + k] = x[k];
+            if (v[k] < n) {
+                d[v[k]] = x[k];
                 if (x[k] != kInf) {
                     ++cnt;
-                    edges += ro[k + 1] - ro[k];
+                    edges += r1[k] - r0[k];
                 }
             }
         }
