@@ -169,6 +169,37 @@ def test_hub_copy_runs_match_oracle(gpu, monkeypatch, sym):
     assert np.array_equal(g.bfs(srcs[0], "diropt").values, oracle.bfs_queue(h.row_offsets, h.col_indices, srcs[0]))
 
 
+def test_hub_copy_edge_cases(gpu):
+    """The hub-first copy of an unweighted graph: direction-optimizing BFS on it equals the BFS queue, and
+    delta-stepping SSSP is refused as on the graph's own ids. Views and road-like graphs have no copy (the
+    HUB set adds nothing and prepares nothing)."""
+    import graphone_amd as g1
+    from graphone_amd import _lib
+    h = g1.HostGraph.rmat(13, 16, seed=81)
+    g = g1.Graph(h, transpose=True)
+    assert g.work_bytes(_lib.G1_WORK_HUB) > 0
+    g.prepare(_lib.G1_WORK_HUB)
+    assert g.work_bytes(_lib.G1_WORK_HUB) == 0
+    deg = h.out_degrees()
+    for src in (int(np.argmax(deg)), int(np.flatnonzero(deg == 0)[0]) if np.any(deg == 0) else 3):
+        assert np.array_equal(g.bfs(src, "diropt").values, oracle.bfs_queue(h.row_offsets, h.col_indices, src)), src
+    with pytest.raises(g1.G1Error):
+        g.sssp(0, "delta")
+    v = g.view()
+    assert v.work_bytes(_lib.G1_WORK_HUB) == 0
+    before = v.device_bytes()
+    v.prepare(_lib.G1_WORK_HUB)
+    assert v.device_bytes() == before
+    src = int(np.argmax(deg))
+    assert np.array_equal(v.bfs(src, "diropt").values, oracle.bfs_queue(h.row_offsets, h.col_indices, src))
+    v.close()
+    r = g1.Graph(g1.HostGraph.grid(32, 32, 0.7, seed=83).assign_weights(1, 9, seed=84, symmetric=True), symmetric=True)
+    assert r.work_bytes(_lib.G1_WORK_HUB) == 0
+    before = r.device_bytes()
+    r.prepare(_lib.G1_WORK_HUB)
+    assert r.device_bytes() == before
+
+
 def test_hub_copy_built_by_the_engines(gpu):
     """From 2^20 vertices the two engines build the hub-first copy on their own first run (the handle's
     device bytes grow by what G1_WORK_HUB reports) and stay bit-exact: RMAT20 from the hub and from a
